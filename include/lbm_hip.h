@@ -1,0 +1,177 @@
+/*
+ * lbm_hip.h -- C ABI of liblbm_hip.so, the MI355X (gfx950) implementation of
+ * the D3Q19 TRT lattice-Boltzmann step and its squirmer coupling.
+ *
+ * Plain C: pointers, sizes and PODs only.  Every call returns 0 on success
+ * and a negative LBM_HIP_E* code on failure; lbm_hip_last_error() then holds
+ * a message (thread-local).  One context owns one device allocation set and
+ * one HIP stream on one GPU; calls enqueue work on that stream and return
+ * without waiting, except the getters and transfers, which synchronise.
+ * A context is not thread-safe (like the reference's class LBM).
+ *
+ * Each entry point names the reference interface it replaces
+ * (byjesh1998/LBM_solver_3D, file:line).  The C++ facade in
+ * lbm_solver_3d_amd/host/ re-exposes them under the reference's own class
+ * and method names (lbm::LBM, lbm::Squirmer).
+ */
+#ifndef LBM_HIP_H
+#define LBM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ---- */
+#define LBM_HIP_OK 0
+#define LBM_HIP_EINVAL (-1)    /* invalid argument / parameters (validate() rules) */
+#define LBM_HIP_ENOMEM (-2)    /* device or host allocation failed */
+#define LBM_HIP_EDEVICE (-3)   /* HIP runtime error, or no usable GPU */
+#define LBM_HIP_ESTATE (-4)    /* requested data was not produced by the last step */
+#define LBM_HIP_EOVERFLOW (-5) /* remap changed more nodes than the device list holds */
+
+/* ---- population storage precision ---- */
+#define LBM_HIP_FP64 0  /* f_i as double, reference arithmetic order (parity mode) */
+#define LBM_HIP_FP32 1  /* f_i - w_i as float, FP32 collision arithmetic */
+#define LBM_HIP_FP16 2  /* (f_i - w_i) * 2^15 as IEEE half, FP32 collision arithmetic */
+#define LBM_HIP_BF16 3  /* f_i - w_i as bfloat16, FP32 collision arithmetic */
+
+/* ---- lbm_hip_step flags ---- */
+#define LBM_STEP_MACRO 0x1      /* write rho, ux, uy, uz of every fluid node (lbm.cpp:152) */
+#define LBM_STEP_WALL_FORCE 0x2 /* accumulate the wall momentum exchange (lbm.cpp:209, 219) */
+#define LBM_STEP_MASS 0x4       /* emit the fluid-mass partial sums used by correct_mass */
+#define LBM_STEP_ALL (LBM_STEP_MACRO | LBM_STEP_WALL_FORCE | LBM_STEP_MASS)
+
+/*
+ * Parameters: the fields of the reference struct Params that its LBM
+ * constructor reads (src/utils/params.hpp:14-56), with the same meaning and
+ * defaults (lbm_hip_default_params), plus:
+ *   plane_xz  -- orientation angle and planar confinement in the x-z plane
+ *                instead of x-y (the reference hard-codes x-y,
+ *                squirmer.cpp:26-27, 58, 65); 0 = reference behaviour.
+ *   precision -- LBM_HIP_FP64 / FP32 / FP16 / BF16 population storage.
+ *   device    -- HIP device ordinal.
+ */
+typedef struct lbm_hip_params {
+    int nx, ny, nz;
+    double tau, lambda, fx, fy, fz;
+    int walls_y, walls_z;
+    int squirmer;
+    double radius, b1, beta, rho_p, x0, y0, z0, angle;
+    int planar;
+    double eps, hc, hmin;
+    int plane_xz;
+    int precision;
+    int device;
+} lbm_hip_params;
+
+/* struct Squirmer (src/utils/squirmer.hpp:14-37): model, state, forces. */
+typedef struct lbm_squirmer_state {
+    double R, B1, beta, M, I, FS;
+    double X[3], e[3], U[3], W[3];
+    double Fh[3], Th[3], Fc[3], Tc[3], Fw[3];
+    double eps, hc, hmin;
+    int64_t contacts;
+    int32_t planar, plane_xz;
+} lbm_squirmer_state;
+
+/* LBM diagnostics (src/utils/lbm.cpp:342-364). */
+typedef struct lbm_hip_diag {
+    int64_t count_fluid;
+    double mean_density, max_speed, sum_ux;
+} lbm_hip_diag;
+
+/* Sizes and derived constants of a context. */
+typedef struct lbm_hip_info {
+    int nx, ny, nz, precision;
+    uint64_t n;             /* nodes */
+    int64_t steps_done;     /* time steps taken so far */
+    double tau, om_plus, om_minus, nu; /* lbm.cpp:67-69 */
+    double bytes_per_lup;   /* algorithmic HBM bytes per lattice update: 2*19*sizeof(storage) */
+    uint64_t device_bytes;  /* device memory held by the context */
+    int has_squirmer, walls_y, walls_z;
+} lbm_hip_info;
+
+typedef struct lbm_hip_ctx lbm_hip_ctx;
+
+/* Reference defaults of struct Params (params.hpp:16-56); precision FP64, device 0. */
+void lbm_hip_default_params(lbm_hip_params* p);
+
+/* Params::validate (params.cpp:120-140): 0 if valid, else LBM_HIP_EINVAL with
+ * the reference's message in lbm_hip_last_error(). */
+int lbm_hip_validate(const lbm_hip_params* p);
+
+/* LBM::LBM(const Params&) (lbm.cpp:60-94): allocate, set wall flags, f = w_i,
+ * initialise the squirmer and its initial flags.  Validates first. */
+int lbm_hip_create(const lbm_hip_params* p, lbm_hip_ctx** out);
+
+/* ~LBM: release every device resource of the context. */
+void lbm_hip_destroy(lbm_hip_ctx* ctx);
+
+/* LBM::step (lbm.cpp:128-224): one fused collide-stream update with TRT + Guo
+ * forcing, halfway bounce-back on WALL links and moving-boundary bounce-back
+ * on PARTICLE links, and the momentum-exchange force and torque on the
+ * squirmer.  `flags` (LBM_STEP_*) selects which by-products are written:
+ * the reference writes all of them every step (LBM_STEP_ALL reproduces it). */
+int lbm_hip_step(lbm_hip_ctx* ctx, int flags);
+
+/* LBM::advanceSquirmer (lbm.cpp:257-263): wall potential, Newton-Euler
+ * update, hard gap and the sequential remap of the sphere, on the device in
+ * FP64.  No-op without a squirmer. */
+int lbm_hip_advance_squirmer(lbm_hip_ctx* ctx);
+
+/* LBM::correctMass (lbm.cpp:324-339): restore the mean fluid density to 1.
+ * The shift f_i += w_i d is applied as the next step loads f. */
+int lbm_hip_correct_mass(lbm_hip_ctx* ctx);
+
+/* The driver loop of main.cpp:60-64 for `steps` steps: step, then with a
+ * squirmer advanceSquirmer and correctMass every `mass_every` steps
+ * (0 = never).  `last_flags` is passed to the final step's lbm_hip_step; the
+ * other steps write only what the coupling needs.  Captured once into a HIP
+ * graph per context and replayed. */
+int lbm_hip_run(lbm_hip_ctx* ctx, int steps, int mass_every, int last_flags);
+
+/* Read / write the public struct Squirmer member LBM::sq (lbm.hpp:78). */
+int lbm_hip_get_squirmer(lbm_hip_ctx* ctx, lbm_squirmer_state* out);
+int lbm_hip_set_squirmer(lbm_hip_ctx* ctx, const lbm_squirmer_state* in);
+
+/* LBM::wallForce (lbm.hpp:75) after the last step; LBM_HIP_ESTATE if that
+ * step ran without LBM_STEP_WALL_FORCE. */
+int lbm_hip_get_wall_force(lbm_hip_ctx* ctx, double out[3]);
+
+/* countFluid / meanDensity / maxSpeed / sumUx (lbm.cpp:342-364). meanDensity,
+ * maxSpeed and sumUx read the moments of the last step: LBM_HIP_ESTATE if it
+ * ran without LBM_STEP_MACRO. */
+int lbm_hip_diagnostics(lbm_hip_ctx* ctx, lbm_hip_diag* out);
+
+/* Copy the public arrays of class LBM (lbm.hpp:71-73) to host buffers in the
+ * reference layout: f[i*N + n] with n = x + NX*(y + NY*z) (19*N doubles),
+ * rho/ux/uy/uz (N doubles each), flag (N bytes, NodeType values).  Any pointer
+ * may be NULL.  Populations of non-fluid nodes are not defined (the reference
+ * never reads them).  rho/u: LBM_HIP_ESTATE unless the last step wrote them. */
+int lbm_hip_download(lbm_hip_ctx* ctx, double* f, double* rho, double* ux, double* uy, double* uz,
+                     uint8_t* flag);
+
+/* Overwrite the populations of every node from f[i*N + n] (reference layout). */
+int lbm_hip_upload_populations(lbm_hip_ctx* ctx, const double* f);
+
+/* Wait for all work of the context. */
+int lbm_hip_synchronize(lbm_hip_ctx* ctx);
+
+int lbm_hip_get_info(lbm_hip_ctx* ctx, lbm_hip_info* out);
+
+/* Bench support: run `steps` driver-loop steps (as lbm_hip_run) and return the
+ * wall time and the summed duration of the fused step kernels, both measured
+ * with HIP events on the context's stream. */
+int lbm_hip_time_run(lbm_hip_ctx* ctx, int steps, int mass_every, double* total_ms,
+                     double* step_kernel_ms);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* lbm_hip_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LBM_HIP_H */

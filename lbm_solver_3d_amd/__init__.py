@@ -1,0 +1,284 @@
+"""lbm_solver_3d_amd -- MI355X-native D3Q19 TRT lattice Boltzmann step with
+squirmer coupling.
+
+The compute path is liblbm_hip.so (hand-written HIP for gfx950) behind the C
+ABI declared in include/lbm_hip.h.  This module is a thin ctypes binding of
+that ABI, mirroring the reference's class LBM (src/utils/lbm.hpp:19-84):
+
+    >>> L = Lattice(nx=64, ny=34, nz=34, tau=0.8, fx=5e-5, walls_y=1, walls_z=1)
+    >>> L.step(); L.advance_squirmer(); L.correct_mass()
+
+There is no CPU fallback: constructing a Lattice without the built library
+or without a GPU raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+ROOT = PKG.parent
+HEADER = ROOT / "include" / "lbm_hip.h"
+LIB_PATH = PKG / "liblbm_hip.so"
+Q = 19
+
+FP64, FP32, FP16, BF16 = 0, 1, 2, 3
+PRECISIONS = {"fp64": FP64, "fp32": FP32, "fp16": FP16, "bf16": BF16}
+STEP_MACRO, STEP_WALL_FORCE, STEP_MASS = 0x1, 0x2, 0x4
+STEP_ALL = STEP_MACRO | STEP_WALL_FORCE | STEP_MASS
+
+ERRORS = {-1: "EINVAL", -2: "ENOMEM", -3: "EDEVICE", -4: "ESTATE", -5: "EOVERFLOW"}
+
+
+class LbmHipError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Params(C.Structure):
+    """struct lbm_hip_params (include/lbm_hip.h) == reference struct Params fields."""
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+        ("tau", C.c_double), ("lambda_", C.c_double),
+        ("fx", C.c_double), ("fy", C.c_double), ("fz", C.c_double),
+        ("walls_y", C.c_int), ("walls_z", C.c_int),
+        ("squirmer", C.c_int),
+        ("radius", C.c_double), ("b1", C.c_double), ("beta", C.c_double), ("rho_p", C.c_double),
+        ("x0", C.c_double), ("y0", C.c_double), ("z0", C.c_double), ("angle", C.c_double),
+        ("planar", C.c_int),
+        ("eps", C.c_double), ("hc", C.c_double), ("hmin", C.c_double),
+        ("plane_xz", C.c_int),
+        ("precision", C.c_int),
+        ("device", C.c_int),
+    ]
+
+
+class SquirmerState(C.Structure):
+    _fields_ = [
+        ("R", C.c_double), ("B1", C.c_double), ("beta", C.c_double), ("M", C.c_double), ("I", C.c_double),
+        ("FS", C.c_double),
+        ("X", C.c_double * 3), ("e", C.c_double * 3), ("U", C.c_double * 3), ("W", C.c_double * 3),
+        ("Fh", C.c_double * 3), ("Th", C.c_double * 3), ("Fc", C.c_double * 3), ("Tc", C.c_double * 3),
+        ("Fw", C.c_double * 3),
+        ("eps", C.c_double), ("hc", C.c_double), ("hmin", C.c_double),
+        ("contacts", C.c_int64),
+        ("planar", C.c_int32), ("plane_xz", C.c_int32),
+    ]
+
+
+class Diag(C.Structure):
+    _fields_ = [("count_fluid", C.c_int64), ("mean_density", C.c_double), ("max_speed", C.c_double),
+                ("sum_ux", C.c_double)]
+
+
+class Info(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("precision", C.c_int),
+        ("n", C.c_uint64), ("steps_done", C.c_int64),
+        ("tau", C.c_double), ("om_plus", C.c_double), ("om_minus", C.c_double), ("nu", C.c_double),
+        ("bytes_per_lup", C.c_double), ("device_bytes", C.c_uint64),
+        ("has_squirmer", C.c_int), ("walls_y", C.c_int), ("walls_z", C.c_int),
+    ]
+
+
+# Reference parameter keys and aliases (params.cpp:40-70) -> Params fields.
+KEY_ALIASES = {
+    "nx": "nx", "ny": "ny", "nz": "nz", "tau": "tau", "lambda": "lambda_",
+    "fx": "fx", "fy": "fy", "fz": "fz",
+    "walls_y": "walls_y", "wallsy": "walls_y", "walls_z": "walls_z", "wallsz": "walls_z",
+    "squirmer": "squirmer", "sq": "squirmer", "radius": "radius", "r": "radius",
+    "b1": "b1", "beta": "beta", "rho_p": "rho_p", "rhop": "rho_p",
+    "x0": "x0", "y0": "y0", "z0": "z0", "angle": "angle", "planar": "planar",
+    "eps": "eps", "hc": "hc", "hmin": "hmin",
+    "plane_xz": "plane_xz", "precision": "precision", "device": "device",
+}
+
+_lib = None
+
+
+def exported_symbols() -> list[str]:
+    """Entry points declared in include/lbm_hip.h."""
+    import re
+    txt = HEADER.read_text()
+    return sorted(set(re.findall(r"\b(lbm_hip_\w+)\s*\(", txt)))
+
+
+def load_library(path: Path | None = None):
+    """Load liblbm_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise RuntimeError(f"{p} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`")
+    lib = C.CDLL(str(p))
+    P, ctx = C.POINTER(Params), C.c_void_p
+    dp = C.POINTER(C.c_double)
+    lib.lbm_hip_default_params.argtypes = [P]
+    lib.lbm_hip_default_params.restype = None
+    lib.lbm_hip_validate.argtypes = [P]
+    lib.lbm_hip_create.argtypes = [P, C.POINTER(C.c_void_p)]
+    lib.lbm_hip_destroy.argtypes = [ctx]
+    lib.lbm_hip_destroy.restype = None
+    lib.lbm_hip_step.argtypes = [ctx, C.c_int]
+    lib.lbm_hip_advance_squirmer.argtypes = [ctx]
+    lib.lbm_hip_correct_mass.argtypes = [ctx]
+    lib.lbm_hip_run.argtypes = [ctx, C.c_int, C.c_int, C.c_int]
+    lib.lbm_hip_get_squirmer.argtypes = [ctx, C.POINTER(SquirmerState)]
+    lib.lbm_hip_set_squirmer.argtypes = [ctx, C.POINTER(SquirmerState)]
+    lib.lbm_hip_get_wall_force.argtypes = [ctx, dp]
+    lib.lbm_hip_diagnostics.argtypes = [ctx, C.POINTER(Diag)]
+    lib.lbm_hip_download.argtypes = [ctx, dp, dp, dp, dp, dp, C.POINTER(C.c_uint8)]
+    lib.lbm_hip_upload_populations.argtypes = [ctx, dp]
+    lib.lbm_hip_synchronize.argtypes = [ctx]
+    lib.lbm_hip_get_info.argtypes = [ctx, C.POINTER(Info)]
+    lib.lbm_hip_time_run.argtypes = [ctx, C.c_int, C.c_int, dp, dp]
+    lib.lbm_hip_last_error.restype = C.c_char_p
+    lib.lbm_hip_last_error.argtypes = []
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def _check(lib, rc: int):
+    if rc != 0:
+        raise LbmHipError(rc, lib.lbm_hip_last_error().decode())
+
+
+def make_params(**kv) -> Params:
+    lib = load_library()
+    p = Params()
+    lib.lbm_hip_default_params(C.byref(p))
+    for k, v in kv.items():
+        k = k.lower()
+        if k not in KEY_ALIASES:
+            raise ValueError(f"unknown parameter '{k}'")
+        if k == "precision" and isinstance(v, str):
+            v = PRECISIONS[v.lower()]
+        setattr(p, KEY_ALIASES[k], v)
+    return p
+
+
+def validate(**kv) -> None:
+    """Params::validate (params.cpp:120-140); raises LbmHipError(EINVAL)."""
+    lib = load_library()
+    p = make_params(**kv)
+    _check(lib, lib.lbm_hip_validate(C.byref(p)))
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double)) if a is not None else None
+
+
+class Lattice:
+    """One lattice on one GPU: the reference's class LBM over the C ABI."""
+
+    def __init__(self, **kv):
+        self.lib = load_library()
+        self.params = make_params(**kv)
+        h = C.c_void_p()
+        _check(self.lib, self.lib.lbm_hip_create(C.byref(self.params), C.byref(h)))
+        self.h = h
+        info = self.info()
+        self.NX, self.NY, self.NZ, self.N = info.nx, info.ny, info.nz, int(info.n)
+        self.nu, self.omPlus, self.omMinus = info.nu, info.om_plus, info.om_minus
+        self.has_squirmer = bool(info.has_squirmer)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.lbm_hip_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- reference entry points ----
+    def step(self, flags: int = STEP_ALL):
+        _check(self.lib, self.lib.lbm_hip_step(self.h, flags))
+
+    def advance_squirmer(self):
+        _check(self.lib, self.lib.lbm_hip_advance_squirmer(self.h))
+
+    def correct_mass(self):
+        _check(self.lib, self.lib.lbm_hip_correct_mass(self.h))
+
+    def run(self, steps: int, mass_every: int = 10, last_flags: int = STEP_ALL):
+        _check(self.lib, self.lib.lbm_hip_run(self.h, steps, mass_every, last_flags))
+
+    def synchronize(self):
+        _check(self.lib, self.lib.lbm_hip_synchronize(self.h))
+
+    # ---- data ----
+    def info(self) -> Info:
+        o = Info()
+        _check(self.lib, self.lib.lbm_hip_get_info(self.h, C.byref(o)))
+        return o
+
+    def squirmer(self) -> dict:
+        s = SquirmerState()
+        _check(self.lib, self.lib.lbm_hip_get_squirmer(self.h, C.byref(s)))
+        out = {k: np.array(getattr(s, k)[:]) for k in ("X", "e", "U", "W", "Fh", "Th", "Fc", "Tc", "Fw")}
+        for k in ("R", "B1", "beta", "M", "I", "FS", "eps", "hc", "hmin"):
+            out[k] = getattr(s, k)
+        out["contacts"] = int(s.contacts)
+        return out
+
+    def set_squirmer(self, **changes):
+        s = SquirmerState()
+        _check(self.lib, self.lib.lbm_hip_get_squirmer(self.h, C.byref(s)))
+        for k, v in changes.items():
+            if isinstance(getattr(s, k), C.Array):
+                getattr(s, k)[:] = list(v)
+            else:
+                setattr(s, k, v)
+        _check(self.lib, self.lib.lbm_hip_set_squirmer(self.h, C.byref(s)))
+
+    def wall_force(self) -> np.ndarray:
+        o = np.zeros(3)
+        _check(self.lib, self.lib.lbm_hip_get_wall_force(self.h, _dp(o)))
+        return o
+
+    def diagnostics(self) -> dict:
+        d = Diag()
+        _check(self.lib, self.lib.lbm_hip_diagnostics(self.h, C.byref(d)))
+        return dict(count_fluid=int(d.count_fluid), mean_density=d.mean_density, max_speed=d.max_speed,
+                    sum_ux=d.sum_ux)
+
+    def populations(self) -> np.ndarray:
+        f = np.empty(Q * self.N)
+        _check(self.lib, self.lib.lbm_hip_download(self.h, _dp(f), None, None, None, None, None))
+        return f.reshape(Q, self.N)
+
+    def fields(self) -> dict:
+        N = self.N
+        rho, ux, uy, uz = (np.empty(N) for _ in range(4))
+        flag = np.empty(N, np.uint8)
+        _check(self.lib, self.lib.lbm_hip_download(self.h, None, _dp(rho), _dp(ux), _dp(uy), _dp(uz),
+                                                   flag.ctypes.data_as(C.POINTER(C.c_uint8))))
+        return dict(rho=rho, ux=ux, uy=uy, uz=uz, flag=flag)
+
+    def flags(self) -> np.ndarray:
+        flag = np.empty(self.N, np.uint8)
+        _check(self.lib, self.lib.lbm_hip_download(self.h, None, None, None, None, None,
+                                                   flag.ctypes.data_as(C.POINTER(C.c_uint8))))
+        return flag
+
+    def upload_populations(self, f: np.ndarray):
+        f = np.ascontiguousarray(f, dtype=np.float64).reshape(-1)
+        if f.size != Q * self.N:
+            raise ValueError("populations must have 19*N entries")
+        _check(self.lib, self.lib.lbm_hip_upload_populations(self.h, _dp(f)))
+
+    def time_run(self, steps: int, mass_every: int = 10) -> tuple[float, float]:
+        tot, k = C.c_double(), C.c_double()
+        _check(self.lib, self.lib.lbm_hip_time_run(self.h, steps, mass_every, C.byref(tot), C.byref(k)))
+        return tot.value, k.value

@@ -1,0 +1,68 @@
+// device_state.hpp -- device-resident squirmer state, coupling accumulators
+// and the argument blocks of the kernels.
+#pragma once
+#include <cstdint>
+
+#include "lattice.hpp"
+
+namespace lbmhip {
+
+// struct Squirmer (reference squirmer.hpp:14-37), kept on the device so a
+// whole time step (fluid update + rigid-body update + remap) runs without a
+// host round trip.
+struct DevSquirmer {
+    double R, B1, beta, M, I, FS;
+    double X[3], e[3], U[3], W[3];
+    double Fh[3], Th[3], Fc[3], Tc[3], Fw[3];
+    double eps, hc, hmin;
+    long long contacts;
+    int planar, plane_xz;
+};
+
+// Everything the kernels exchange through device memory besides the fields.
+// Buffers indexed [t & 1] are double-buffered by step parity: step t
+// accumulates into slot t&1 and clears slot (t+1)&1, so no kernel clears what
+// another block of the same launch still reads, and no memset is needed
+// (the step can be captured into a graph).
+struct DevState {
+    DevSquirmer sq;
+    // Momentum exchange of step t: [0..2] wall force, [3..5] particle force,
+    // [6..8] particle torque (lbm.cpp:131-132, 209-215).
+    double acc[2][9];
+    // Pending mass-correction shift d read by step t from slot t&1
+    // (lbm.cpp:333, 337: f_i += w_i d).
+    double shift[2];
+    // Mass change made by remaps since the last step, as excess over one
+    // unit per fluid node (cover removes sum_i f_i - 1, uncover adds
+    // sum_i feq_i - 1).
+    double remap_dmass;
+    long long nfluid;   // number of FLUID nodes, maintained by the remap
+    int band[6];        // x0, y0, z0 (wrapped) and widths of the rho band
+    int err;            // device error bits (ERR_*)
+    int changed;        // nodes changed by the last remap
+};
+
+constexpr int ERR_REMAP_OVERFLOW = 1;
+
+// Arguments of the fused collide-stream kernel.
+template <class S>
+struct StepArgs {
+    S* f;                 // 19 * N populations, AA in-place layout
+    uint8_t* flag;        // N flag bytes (NodeType | BOUNDARY)
+    void* rho;            // N (double in FP64 mode, float otherwise)
+    void* ux;
+    void* uy;
+    void* uz;
+    DevState* st;
+    double* mass_part;    // one excess-mass partial per block
+    int NX, NY, NZ;
+    uint32_t N;
+    int walls_y, walls_z, has_squirmer;
+    int gx, gy, gz;       // grid in tiles
+    int tx_shift;         // log2 of the tile width in x
+    int ty;               // tile height in y
+    double omPlus, omMinus, Fx, Fy, Fz;
+    long long t;          // step index (selects double-buffer slots)
+};
+
+}  // namespace lbmhip

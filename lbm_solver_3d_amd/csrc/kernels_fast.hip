@@ -1,0 +1,9 @@
+// kernels_fast.hip -- FP32-arithmetic step kernels for FP32 / FP16 / BF16
+// population storage (compiled with floating-point contraction allowed).
+#include "launch_step_impl.hpp"
+
+namespace lbmhip {
+template hipError_t launch_step<float>(const StepArgs<float>&, const FastCoef&, bool, int, hipStream_t);
+template hipError_t launch_step<_Float16>(const StepArgs<_Float16>&, const FastCoef&, bool, int, hipStream_t);
+template hipError_t launch_step<bf16_t>(const StepArgs<bf16_t>&, const FastCoef&, bool, int, hipStream_t);
+}  // namespace lbmhip

@@ -1,0 +1,213 @@
+// kernels_fp64.hip -- FP64 step kernels (reference operation order), the
+// squirmer / remap kernel for every storage format, and the auxiliary
+// kernels.  Compiled with -ffp-contract=off: no multiply-add is fused, so the
+// FP64 results round exactly like the reference's scalar code.
+#include "launch_step_impl.hpp"
+
+namespace lbmhip {
+
+template <class S>
+hipError_t launch_squirmer(const RemapArgs<S>& a, hipStream_t s) {
+    hipLaunchKernelGGL((squirmer_kernel<S>), dim3(1), dim3(SQ_THREADS), 0, s, a);
+    return hipGetLastError();
+}
+
+template <class S>
+hipError_t launch_gather(const S* f, const uint8_t* flag, double* out, int NX, int NY, int NZ, uint32_t N, int shifted,
+                         const DevState* st, int shift_slot, hipStream_t s) {
+    hipLaunchKernelGGL((gather_kernel<S>), dim3((N + 255) / 256), dim3(256), 0, s, f, flag, out, NX, NY, NZ, N,
+                       shifted, st, shift_slot);
+    return hipGetLastError();
+}
+
+template <class S>
+hipError_t launch_scatter(S* f, const uint8_t* flag, const double* in, int NX, int NY, int NZ, uint32_t N, int shifted,
+                          hipStream_t s) {
+    hipLaunchKernelGGL((scatter_kernel<S>), dim3((N + 255) / 256), dim3(256), 0, s, f, flag, in, NX, NY, NZ, N,
+                       shifted);
+    return hipGetLastError();
+}
+
+template <class S>
+hipError_t launch_mass(const S* f, const uint8_t* flag, double* part, int NX, int NY, int NZ, uint32_t N, int shifted,
+                       const DevState* st, int shift_slot, hipStream_t s) {
+    hipLaunchKernelGGL((mass_kernel<S>), dim3((N + 255) / 256), dim3(256), 0, s, f, flag, part, NX, NY, NZ, N, shifted,
+                       st, shift_slot);
+    return hipGetLastError();
+}
+
+// f_i = w_i everywhere (lbm.cpp:82-84), in storage format S.
+template <class S>
+__global__ __launch_bounds__(256) void fill_populations_kernel(S* f, uint32_t N) {
+    const size_t e = (size_t)blockIdx.x * 256u + threadIdx.x;
+    if (e >= (size_t)Q * N) return;
+    const int i = (int)(e / N);
+    f[e] = from_real<S>(W[i], i);
+}
+
+template <class S>
+hipError_t launch_fill_populations(S* f, uint32_t N, hipStream_t s) {
+    const size_t tot = (size_t)Q * N;
+    hipLaunchKernelGGL((fill_populations_kernel<S>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, f, N);
+    return hipGetLastError();
+}
+
+#define LBMHIP_INSTANTIATE(S)                                                                                     \
+    template hipError_t launch_squirmer<S>(const RemapArgs<S>&, hipStream_t);                                    \
+    template hipError_t launch_gather<S>(const S*, const uint8_t*, double*, int, int, int, uint32_t, int,        \
+                                         const DevState*, int, hipStream_t);                                     \
+    template hipError_t launch_scatter<S>(S*, const uint8_t*, const double*, int, int, int, uint32_t, int,       \
+                                          hipStream_t);                                                          \
+    template hipError_t launch_mass<S>(const S*, const uint8_t*, double*, int, int, int, uint32_t, int,          \
+                                       const DevState*, int, hipStream_t);                                       \
+    template hipError_t launch_fill_populations<S>(S*, uint32_t, hipStream_t);
+
+LBMHIP_INSTANTIATE(double)
+LBMHIP_INSTANTIATE(float)
+LBMHIP_INSTANTIATE(_Float16)
+LBMHIP_INSTANTIATE(bf16_t)
+#undef LBMHIP_INSTANTIATE
+
+template hipError_t launch_step<double>(const StepArgs<double>&, const FastCoef&, bool, int, hipStream_t);
+
+// ---------------------------------------------------------------------------
+// Mass correction (lbm.cpp:324-339): d = 1 - m / n_fluid with
+// m = n_fluid + excess, so d = -excess / n_fluid; the shift is added to the
+// slot the next step reads.  One block, fixed summation order.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void mass_finalize_kernel(const double* part, int nparts, DevState* st,
+                                                             int use_remap_dmass, int shift_slot) {
+    __shared__ double lds[16];
+    double s = 0.0;
+    for (int k = threadIdx.x; k < nparts; k += 1024) s += part[k];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double e = 0.0;
+        for (int w = 0; w < 16; ++w) e += lds[w];
+        if (use_remap_dmass) e += st->remap_dmass;
+        const long long nf = st->nfluid;
+        if (nf > 0) st->shift[shift_slot] += -e / (double)nf;
+    }
+}
+
+hipError_t launch_mass_finalize(const double* part, int nparts, DevState* st, int use_remap_dmass, int shift_slot,
+                                hipStream_t s) {
+    hipLaunchKernelGGL(mass_finalize_kernel, dim3(1), dim3(1024), 0, s, part, nparts, st, use_remap_dmass, shift_slot);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Diagnostics (lbm.cpp:342-364): count of FLUID nodes, sum of rho, maximum
+// speed and sum of ux over FLUID nodes; per-block partials, then one block.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void diag_kernel(const uint8_t* flag, const void* rho, const void* ux,
+                                                   const void* uy, const void* uz, int is_double, uint32_t N,
+                                                   double* part) {
+    __shared__ double lds[4 * 4];
+    const uint32_t n = blockIdx.x * 256u + threadIdx.x;
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    if (n < N && (flag[n] & TYPE_MASK) == FLUID) {
+        double r, a, b, c;
+        if (is_double) {
+            r = ((const double*)rho)[n]; a = ((const double*)ux)[n];
+            b = ((const double*)uy)[n]; c = ((const double*)uz)[n];
+        } else {
+            r = ((const float*)rho)[n]; a = ((const float*)ux)[n];
+            b = ((const float*)uy)[n]; c = ((const float*)uz)[n];
+        }
+        v[0] = 1.0; v[1] = r; v[2] = sqrt(a * a + b * b + c * c); v[3] = a;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        v[0] += __shfl_xor(v[0], off, 64);
+        v[1] += __shfl_xor(v[1], off, 64);
+        const double m = __shfl_xor(v[2], off, 64);
+        v[2] = v[2] < m ? m : v[2];
+        v[3] += __shfl_xor(v[3], off, 64);
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < 4; ++k) lds[w * 4 + k] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double o[4] = {lds[0], lds[1], lds[2], lds[3]};
+        for (int ww = 1; ww < 4; ++ww) {
+            o[0] += lds[ww * 4 + 0];
+            o[1] += lds[ww * 4 + 1];
+            o[2] = o[2] < lds[ww * 4 + 2] ? lds[ww * 4 + 2] : o[2];
+            o[3] += lds[ww * 4 + 3];
+        }
+        for (int k = 0; k < 4; ++k) part[(size_t)blockIdx.x * 4 + k] = o[k];
+    }
+}
+
+__global__ __launch_bounds__(1024) void diag_finalize_kernel(const double* part, int nparts, double* out4) {
+    __shared__ double lds[16 * 4];
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int k = threadIdx.x; k < nparts; k += 1024) {
+        v[0] += part[(size_t)k * 4 + 0];
+        v[1] += part[(size_t)k * 4 + 1];
+        v[2] = v[2] < part[(size_t)k * 4 + 2] ? part[(size_t)k * 4 + 2] : v[2];
+        v[3] += part[(size_t)k * 4 + 3];
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        v[0] += __shfl_xor(v[0], off, 64);
+        v[1] += __shfl_xor(v[1], off, 64);
+        const double m = __shfl_xor(v[2], off, 64);
+        v[2] = v[2] < m ? m : v[2];
+        v[3] += __shfl_xor(v[3], off, 64);
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < 4; ++k) lds[w * 4 + k] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double o[4] = {lds[0], lds[1], lds[2], lds[3]};
+        for (int ww = 1; ww < 16; ++ww) {
+            o[0] += lds[ww * 4 + 0];
+            o[1] += lds[ww * 4 + 1];
+            o[2] = o[2] < lds[ww * 4 + 2] ? lds[ww * 4 + 2] : o[2];
+            o[3] += lds[ww * 4 + 3];
+        }
+        for (int k = 0; k < 4; ++k) out4[k] = o[k];
+    }
+}
+
+hipError_t launch_diag(const uint8_t* flag, const void* rho, const void* ux, const void* uy, const void* uz,
+                       int is_double, uint32_t N, double* part, int nparts, double* out4, hipStream_t s) {
+    hipLaunchKernelGGL(diag_kernel, dim3(nparts), dim3(256), 0, s, flag, rho, ux, uy, uz, is_double, N, part);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(diag_finalize_kernel, dim3(1), dim3(1024), 0, s, part, nparts, out4);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void fill_macro_kernel(void* rho, void* ux, void* uy, void* uz, int is_double,
+                                                         uint32_t N) {
+    const uint32_t n = blockIdx.x * 256u + threadIdx.x;
+    if (n >= N) return;
+    if (is_double) {
+        ((double*)rho)[n] = 1.0; ((double*)ux)[n] = 0.0; ((double*)uy)[n] = 0.0; ((double*)uz)[n] = 0.0;
+    } else {
+        ((float*)rho)[n] = 1.0f; ((float*)ux)[n] = 0.0f; ((float*)uy)[n] = 0.0f; ((float*)uz)[n] = 0.0f;
+    }
+}
+
+hipError_t launch_fill_macro(void* rho, void* ux, void* uy, void* uz, int is_double, uint32_t N, hipStream_t s) {
+    hipLaunchKernelGGL(fill_macro_kernel, dim3((N + 255) / 256), dim3(256), 0, s, rho, ux, uy, uz, is_double, N);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void macro_to_double_kernel(const float* src, double* dst, uint32_t N) {
+    const uint32_t n = blockIdx.x * 256u + threadIdx.x;
+    if (n < N) dst[n] = (double)src[n];
+}
+
+hipError_t launch_macro_to_double(const void* src, int is_double, double* dst, uint32_t N, hipStream_t s) {
+    if (is_double) return hipMemcpyAsync(dst, src, sizeof(double) * N, hipMemcpyDeviceToDevice, s);
+    hipLaunchKernelGGL(macro_to_double_kernel, dim3((N + 255) / 256), dim3(256), 0, s, (const float*)src, dst, N);
+    return hipGetLastError();
+}
+
+}  // namespace lbmhip

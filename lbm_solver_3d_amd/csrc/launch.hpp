@@ -1,0 +1,61 @@
+// launch.hpp -- host-side launchers of the device kernels.  The FP64 kernels
+// and the squirmer/remap kernels are compiled in kernels_fp64.hip with
+// floating-point contraction off (reference operation order); the FP32 /
+// FP16 / BF16 step kernels in kernels_fast.hip with contraction on.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "device_state.hpp"
+#include "squirmer_kernel.hpp"
+#include "step_kernel.hpp"
+#include "storage.hpp"
+
+namespace lbmhip {
+
+template <class S>
+hipError_t launch_step(const StepArgs<S>& a, const FastCoef& k, bool odd, int mode, hipStream_t s);
+
+template <class S>
+hipError_t launch_squirmer(const RemapArgs<S>& a, hipStream_t s);
+
+template <class S>
+hipError_t launch_gather(const S* f, const uint8_t* flag, double* out, int NX, int NY, int NZ, uint32_t N,
+                         int shifted, const DevState* st, int shift_slot, hipStream_t s);
+
+template <class S>
+hipError_t launch_scatter(S* f, const uint8_t* flag, const double* in, int NX, int NY, int NZ, uint32_t N,
+                          int shifted, hipStream_t s);
+
+template <class S>
+hipError_t launch_mass(const S* f, const uint8_t* flag, double* part, int NX, int NY, int NZ, uint32_t N,
+                       int shifted, const DevState* st, int shift_slot, hipStream_t s);
+
+template <class S>
+hipError_t launch_fill_populations(S* f, uint32_t N, hipStream_t s);
+
+// Storage-independent kernels (kernels_fp64.hip).
+hipError_t launch_mass_finalize(const double* part, int nparts, DevState* st, int use_remap_dmass, int shift_slot,
+                                hipStream_t s);
+hipError_t launch_diag(const uint8_t* flag, const void* rho, const void* ux, const void* uy, const void* uz,
+                       int is_double, uint32_t N, double* part, int nparts, double* out4, hipStream_t s);
+hipError_t launch_fill_macro(void* rho, void* ux, void* uy, void* uz, int is_double, uint32_t N, hipStream_t s);
+hipError_t launch_macro_to_double(const void* src, int is_double, double* dst, uint32_t N, hipStream_t s);
+
+#define LBMHIP_DECLARE(S)                                                                                          \
+    extern template hipError_t launch_step<S>(const StepArgs<S>&, const FastCoef&, bool, int, hipStream_t);     \
+    extern template hipError_t launch_squirmer<S>(const RemapArgs<S>&, hipStream_t);                            \
+    extern template hipError_t launch_gather<S>(const S*, const uint8_t*, double*, int, int, int, uint32_t, int, \
+                                                const DevState*, int, hipStream_t);                             \
+    extern template hipError_t launch_scatter<S>(S*, const uint8_t*, const double*, int, int, int, uint32_t, int, \
+                                                 hipStream_t);                                                  \
+    extern template hipError_t launch_mass<S>(const S*, const uint8_t*, double*, int, int, int, uint32_t, int,   \
+                                              const DevState*, int, hipStream_t);                               \
+    extern template hipError_t launch_fill_populations<S>(S*, uint32_t, hipStream_t);
+
+LBMHIP_DECLARE(double)
+LBMHIP_DECLARE(float)
+LBMHIP_DECLARE(_Float16)
+LBMHIP_DECLARE(bf16_t)
+#undef LBMHIP_DECLARE
+
+}  // namespace lbmhip

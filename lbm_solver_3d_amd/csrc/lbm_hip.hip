@@ -1,0 +1,647 @@
+// lbm_hip.hip -- the C ABI (include/lbm_hip.h): contexts, device memory,
+// launch sequencing and host transfers.  All compute runs in the kernels of
+// kernels_fp64.hip / kernels_fast.hip.
+#include "lbm_hip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "launch.hpp"
+
+using namespace lbmhip;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                                  \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess)                                                                          \
+            return fail(LBM_HIP_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));          \
+    } while (0)
+
+size_t storage_bytes(int prec) {
+    switch (prec) {
+        case LBM_HIP_FP64: return 8;
+        case LBM_HIP_FP32: return 4;
+        default: return 2;
+    }
+}
+
+}  // namespace
+
+struct lbm_hip_ctx {
+    lbm_hip_params p{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int NX = 0, NY = 0, NZ = 0;
+    uint32_t N = 0;
+    int prec = 0;
+    size_t ssize = 8, msize = 8;
+    void* f = nullptr;
+    uint8_t* flag = nullptr;
+    void* rho = nullptr;
+    void* ux = nullptr;
+    void* uy = nullptr;
+    void* uz = nullptr;
+    DevState* st = nullptr;
+    double* mass_part = nullptr;  // >= max(G, ceil(N/256)) doubles
+    double* diag_part = nullptr;  // 4 * ceil(N/256) doubles
+    double* diag_out = nullptr;   // 4 doubles
+    int gx = 0, gy = 0, gz = 0, tx_shift = 8, ty = 1, G = 0, nb256 = 0;
+    double omPlus = 0, omMinus = 0, nu = 0;
+    FastCoef K{};
+    long long t = 0;  // steps done
+    bool has_squirmer = false;
+    bool macro_valid = true;  // initial rho = 1, u = 0 are the reference's values
+    bool wallf_valid = false;
+    bool mass_valid = false;
+    size_t device_bytes = 0;
+    std::vector<hipEvent_t> events;
+};
+
+extern "C" {
+
+const char* lbm_hip_last_error(void) { return g_err.c_str(); }
+
+void lbm_hip_default_params(lbm_hip_params* p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof *p);
+    // struct Params defaults, params.hpp:16-56
+    p->nx = 64; p->ny = 34; p->nz = 32;
+    p->tau = 1.0; p->lambda = 3.0 / 16.0;
+    p->walls_y = 1; p->walls_z = 0;
+    p->squirmer = 0;
+    p->radius = 4.0; p->b1 = 0.015; p->beta = 0.0; p->rho_p = 1.0;
+    p->x0 = -1; p->y0 = -1; p->z0 = -1;
+    p->angle = 30.0; p->planar = 1;
+    p->eps = 0.2; p->hc = 2.0; p->hmin = 0.5;
+    p->plane_xz = 0;
+    p->precision = LBM_HIP_FP64;
+    p->device = 0;
+}
+
+// Params::validate, params.cpp:120-140 (same rules and messages), plus the
+// 32-bit indexing limit of this implementation and a precision check.
+int lbm_hip_validate(const lbm_hip_params* p) {
+    if (!p) return fail(LBM_HIP_EINVAL, "null parameters");
+    auto bad = [](const char* m) { return fail(LBM_HIP_EINVAL, m); };
+    if (p->nx < 1 || p->ny < 1 || p->nz < 1) return bad("grid dimensions must be positive");
+    if (p->walls_y && p->ny < 4) return bad("ny must be >= 4 with walls in y");
+    if (p->walls_z && p->nz < 4) return bad("nz must be >= 4 with walls in z");
+    if (p->tau <= 0.5) return bad("tau must be > 0.5");
+    if (p->lambda <= 0.0) return bad("lambda must be > 0");
+    if (p->precision < LBM_HIP_FP64 || p->precision > LBM_HIP_BF16) return bad("unknown precision");
+    const double N = (double)p->nx * p->ny * p->nz;
+    if (N * 19.0 >= 4294967296.0) return bad("grid too large for 32-bit population indexing (19*N must be < 2^32)");
+    if (!p->squirmer) return LBM_HIP_OK;
+    if (p->radius <= 1.0) return bad("squirmer radius must be > 1 lattice unit");
+    if (std::fabs(p->angle) >= 45.0) return bad("initial angle must satisfy |angle| < 45 deg from the x-axis");
+    if (p->rho_p <= 0.0) return bad("rho_p must be > 0");
+    if (p->eps < 0.0 || p->hc <= 0.0 || p->hmin < 0.0) return bad("eps >= 0, hc > 0, hmin >= 0 required");
+    const double R = p->radius;
+    const double y = p->y0 >= 0 ? p->y0 : 0.5 * (p->ny - 1);
+    if (p->walls_y && (y - R - 0.5 < p->hmin || (p->ny - 1.5) - y - R < p->hmin))
+        return bad("squirmer initially overlaps (or is too close to) a y-wall");
+    if (2.0 * R + 2.0 > p->nx || (!p->walls_y && 2.0 * R + 2.0 > p->ny) || 2.0 * R + 2.0 > p->nz)
+        return bad("domain too small for the squirmer");
+    if (2.0 * p->b1 / 3.0 > 0.05) return bad("swimming speed U0 = 2 B1 / 3 too large (Mach number); keep U0 < 0.05");
+    return LBM_HIP_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+template <class F>
+int dispatch(int prec, F&& fn) {
+    switch (prec) {
+        case LBM_HIP_FP64: return fn((double*)nullptr);
+        case LBM_HIP_FP32: return fn((float*)nullptr);
+        case LBM_HIP_FP16: return fn((_Float16*)nullptr);
+        default: return fn((bf16_t*)nullptr);
+    }
+}
+
+// Squirmer::init (squirmer.cpp:20-30) on the host, in FP64.
+void init_squirmer(DevSquirmer& s, const lbm_hip_params& p, double nu, double xc, double yc, double zc) {
+    std::memset(&s, 0, sizeof s);
+    s.R = p.radius; s.B1 = p.b1; s.beta = p.beta; s.planar = p.planar; s.plane_xz = p.plane_xz;
+    s.M = p.rho_p * 4.0 / 3.0 * kPi * s.R * s.R * s.R;
+    s.I = 0.4 * s.M * s.R * s.R;
+    s.X[0] = xc; s.X[1] = yc; s.X[2] = zc;
+    const double a = p.angle * kPi / 180.0;
+    if (p.plane_xz) { s.e[0] = std::cos(a); s.e[1] = 0.0; s.e[2] = std::sin(a); }
+    else { s.e[0] = std::cos(a); s.e[1] = std::sin(a); s.e[2] = 0.0; }
+    s.eps = p.eps; s.hc = p.hc; s.hmin = p.hmin;
+    s.FS = 6.0 * kPi * nu * s.R * (2.0 * s.B1 / 3.0);
+}
+
+void band_of(const DevSquirmer& s, int NX, int NY, int NZ, int band[6]) {
+    const int Nd[3] = {NX, NY, NZ};
+    for (int d = 0; d < 3; ++d) {
+        const int a = (int)std::floor(s.X[d] - s.R) - 4, b = (int)std::ceil(s.X[d] + s.R) + 4;
+        int w = b - a + 1;
+        if (w > Nd[d]) w = Nd[d];
+        band[d] = ((a % Nd[d]) + Nd[d]) % Nd[d];
+        band[3 + d] = w;
+    }
+}
+
+FastCoef make_coef(double omP, double omM, double Fx, double Fy, double Fz, double scale) {
+    FastCoef k{};
+    const double gP = 1.0 - 0.5 * omP, gM = 1.0 - 0.5 * omM;
+    const double w1 = 1.0 / 18, w2 = 1.0 / 36, w0 = 1.0 / 3.0;
+    k.inv_s = (float)(1.0 / scale);
+    k.s = (float)scale;
+    k.a = (float)(0.5 * omP);
+    k.b = (float)(0.5 * omM);
+    k.one_m_omp = (float)(1.0 - omP);
+    k.kp_axis = (float)(omP * w1); k.kp_diag = (float)(omP * w2); k.kp_rest = (float)(omP * w0);
+    k.km_axis = (float)(3.0 * omM * w1); k.km_diag = (float)(3.0 * omM * w2);
+    k.hFx = (float)(0.5 * scale * Fx); k.hFy = (float)(0.5 * scale * Fy); k.hFz = (float)(0.5 * scale * Fz);
+    k.gPw9[0] = (float)(9.0 * gP * w1); k.gPw9[1] = (float)(9.0 * gP * w2); k.gPw9[2] = (float)(9.0 * gP * w0);
+    k.gPw3[0] = (float)(3.0 * gP * w1); k.gPw3[1] = (float)(3.0 * gP * w2); k.gPw3[2] = (float)(3.0 * gP * w0);
+    k.sm_axis = (float)(3.0 * scale * gM * w1); k.sm_diag = (float)(3.0 * scale * gM * w2);
+    k.Fx = (float)Fx; k.Fy = (float)Fy; k.Fz = (float)Fz;
+    return k;
+}
+
+template <class S>
+StepArgs<S> step_args(const lbm_hip_ctx* c) {
+    StepArgs<S> a{};
+    a.f = (S*)c->f; a.flag = c->flag;
+    a.rho = c->rho; a.ux = c->ux; a.uy = c->uy; a.uz = c->uz;
+    a.st = c->st; a.mass_part = c->mass_part;
+    a.NX = c->NX; a.NY = c->NY; a.NZ = c->NZ; a.N = c->N;
+    a.walls_y = c->p.walls_y; a.walls_z = c->p.walls_z; a.has_squirmer = c->has_squirmer;
+    a.gx = c->gx; a.gy = c->gy; a.gz = c->gz; a.tx_shift = c->tx_shift; a.ty = c->ty;
+    a.omPlus = c->omPlus; a.omMinus = c->omMinus;
+    a.Fx = c->p.fx; a.Fy = c->p.fy; a.Fz = c->p.fz;
+    a.t = c->t;
+    return a;
+}
+
+template <class S>
+RemapArgs<S> remap_args(const lbm_hip_ctx* c) {
+    RemapArgs<S> a{};
+    a.f = (S*)c->f; a.flag = c->flag;
+    a.rho = c->rho; a.ux = c->ux; a.uy = c->uy; a.uz = c->uz;
+    a.st = c->st;
+    a.NX = c->NX; a.NY = c->NY; a.NZ = c->NZ; a.N = c->N;
+    a.walls_y = c->p.walls_y; a.walls_z = c->p.walls_z;
+    a.shifted = (c->t > 0 && ((c->t - 1) & 1) == 0) ? 1 : 0;
+    a.acc_slot = (int)((c->t - 1) & 1);
+    a.shift_slot = (int)(c->t & 1);
+    return a;
+}
+
+bool layout_shifted(const lbm_hip_ctx* c) { return c->t > 0 && ((c->t - 1) & 1) == 0; }
+
+void free_ctx(lbm_hip_ctx* c) {
+    if (!c) return;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (void* q : {c->f, (void*)c->flag, c->rho, c->ux, c->uy, c->uz, (void*)c->st, (void*)c->mass_part,
+                    (void*)c->diag_part, (void*)c->diag_out})
+        if (q) hipFree(q);
+    for (hipEvent_t e : c->events) hipEventDestroy(e);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int check_device_error(lbm_hip_ctx* c) {
+    int err = 0;
+    HIP_TRY(hipMemcpyAsync(&err, &c->st->err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (err & ERR_REMAP_OVERFLOW)
+        return fail(LBM_HIP_EOVERFLOW, "remap changed more than " + std::to_string(REMAP_CAP) +
+                                           " nodes in one step; the lattice state is no longer valid");
+    return LBM_HIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lbm_hip_create(const lbm_hip_params* pin, lbm_hip_ctx** out) {
+    if (!out) return fail(LBM_HIP_EINVAL, "null output pointer");
+    *out = nullptr;
+    int rc = lbm_hip_validate(pin);
+    if (rc) return rc;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+        return fail(LBM_HIP_EDEVICE, "no HIP device available (liblbm_hip needs an AMD GPU)");
+    if (pin->device < 0 || pin->device >= ndev) return fail(LBM_HIP_EINVAL, "device ordinal out of range");
+
+    auto* c = new lbm_hip_ctx();
+    c->p = *pin;
+    c->device = pin->device;
+    c->prec = pin->precision;
+    c->NX = pin->nx; c->NY = pin->ny; c->NZ = pin->nz;
+    c->N = (uint32_t)((size_t)pin->nx * pin->ny * pin->nz);
+    c->ssize = storage_bytes(c->prec);
+    c->msize = c->prec == LBM_HIP_FP64 ? 8 : 4;
+    // lbm.cpp:67-69
+    c->omPlus = 1.0 / pin->tau;
+    c->omMinus = 1.0 / (0.5 + pin->lambda / (pin->tau - 0.5));
+    c->nu = (pin->tau - 0.5) / 3.0;
+    const double scale = c->prec == LBM_HIP_FP16 ? 32768.0 : 1.0;
+    c->K = make_coef(c->omPlus, c->omMinus, pin->fx, pin->fy, pin->fz, scale);
+
+    // tiles: one wavefront row-chunk along x, block of 256 threads
+    c->tx_shift = pin->nx <= 64 ? 6 : (pin->nx <= 128 ? 7 : 8);
+    c->ty = BLOCK >> c->tx_shift;
+    c->gx = (pin->nx + (1 << c->tx_shift) - 1) >> c->tx_shift;
+    c->gy = (pin->ny + c->ty - 1) / c->ty;
+    c->gz = pin->nz;
+    c->G = c->gx * c->gy * c->gz;
+    c->nb256 = (int)((c->N + 255) / 256);
+
+    auto cleanup = [&](int code) {
+        std::string m = g_err;
+        free_ctx(c);
+        g_err = m;
+        return code;
+    };
+    if (hipSetDevice(c->device) != hipSuccess) return cleanup(fail(LBM_HIP_EDEVICE, "hipSetDevice failed"));
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+        return cleanup(fail(LBM_HIP_EDEVICE, "hipStreamCreate failed"));
+
+    const size_t N = c->N;
+    auto dmalloc = [&](void** q, size_t bytes) -> bool {
+        if (hipMalloc(q, bytes) != hipSuccess) { *q = nullptr; return false; }
+        c->device_bytes += bytes;
+        return true;
+    };
+    const int nparts = c->G > c->nb256 ? c->G : c->nb256;
+    if (!dmalloc(&c->f, (size_t)Q * N * c->ssize) || !dmalloc((void**)&c->flag, N) ||
+        !dmalloc(&c->rho, N * c->msize) || !dmalloc(&c->ux, N * c->msize) || !dmalloc(&c->uy, N * c->msize) ||
+        !dmalloc(&c->uz, N * c->msize) || !dmalloc((void**)&c->st, sizeof(DevState)) ||
+        !dmalloc((void**)&c->mass_part, sizeof(double) * nparts) ||
+        !dmalloc((void**)&c->diag_part, sizeof(double) * 4 * c->nb256) ||
+        !dmalloc((void**)&c->diag_out, sizeof(double) * 4))
+        return cleanup(fail(LBM_HIP_ENOMEM, "device allocation failed"));
+
+    // flags: wall layers (lbm.cpp:72-78)
+    std::vector<uint8_t> flag(N);
+    for (int z = 0; z < c->NZ; ++z)
+        for (int y = 0; y < c->NY; ++y)
+            for (int x = 0; x < c->NX; ++x) {
+                const bool wall = (pin->walls_y && (y == 0 || y == c->NY - 1)) ||
+                                  (pin->walls_z && (z == 0 || z == c->NZ - 1));
+                flag[(size_t)x + (size_t)c->NX * ((size_t)y + (size_t)c->NY * z)] = wall ? WALL : FLUID;
+            }
+    DevState hs;
+    std::memset(&hs, 0, sizeof hs);
+    c->has_squirmer = pin->squirmer != 0;
+    if (c->has_squirmer) {
+        init_squirmer(hs.sq, *pin, c->nu, pin->x0 >= 0 ? pin->x0 : 0.5 * (c->NX - 1),
+                      pin->y0 >= 0 ? pin->y0 : 0.5 * (c->NY - 1), pin->z0 >= 0 ? pin->z0 : 0.5 * (c->NZ - 1));
+        // updateParticleFlags(true) (lbm.cpp:271-291)
+        const DevSquirmer& s = hs.sq;
+        const double R = s.R;
+        const int x0 = (int)std::floor(s.X[0] - R) - 2, x1 = (int)std::ceil(s.X[0] + R) + 2;
+        int y0 = (int)std::floor(s.X[1] - R) - 2, y1 = (int)std::ceil(s.X[1] + R) + 2;
+        const int z0 = (int)std::floor(s.X[2] - R) - 2, z1 = (int)std::ceil(s.X[2] + R) + 2;
+        if (pin->walls_y) { y0 = std::max(0, y0); y1 = std::min(c->NY - 1, y1); }
+        for (int zz = z0; zz <= z1; ++zz)
+            for (int yy = y0; yy <= y1; ++yy)
+                for (int xx = x0; xx <= x1; ++xx) {
+                    const int x = ((xx % c->NX) + c->NX) % c->NX;
+                    const int y = ((yy % c->NY) + c->NY) % c->NY;
+                    const int z = ((zz % c->NZ) + c->NZ) % c->NZ;
+                    const size_t n = (size_t)x + (size_t)c->NX * ((size_t)y + (size_t)c->NY * z);
+                    if (flag[n] == WALL) continue;
+                    double r[3];
+                    r[0] = x - s.X[0]; r[0] -= c->NX * std::round(r[0] / c->NX);
+                    r[1] = y - s.X[1]; if (!pin->walls_y) r[1] -= c->NY * std::round(r[1] / c->NY);
+                    r[2] = z - s.X[2]; if (!pin->walls_z) r[2] -= c->NZ * std::round(r[2] / c->NZ);
+                    const bool inside = r[0] * r[0] + r[1] * r[1] + r[2] * r[2] < R * R;
+                    flag[n] = inside ? PARTICLE : FLUID;
+                }
+        band_of(hs.sq, c->NX, c->NY, c->NZ, hs.band);
+    } else {
+        hs.sq.R = 4.0; hs.sq.B1 = 0.015; hs.sq.planar = 1; hs.sq.e[0] = 1.0;
+        hs.sq.eps = 0.2; hs.sq.hc = 2.0; hs.sq.hmin = 0.5;
+    }
+    // BOUNDARY bits and fluid count
+    long long nf = 0;
+    std::vector<uint8_t> dflag(flag);
+    for (int z = 0; z < c->NZ; ++z)
+        for (int y = 0; y < c->NY; ++y)
+            for (int x = 0; x < c->NX; ++x) {
+                const size_t n = (size_t)x + (size_t)c->NX * ((size_t)y + (size_t)c->NY * z);
+                if (flag[n] != FLUID) continue;
+                ++nf;
+                bool b = false;
+                for (int i = 1; i < Q && !b; ++i) {
+                    const int xn = (x + CX[i] + c->NX) % c->NX, yn = (y + CY[i] + c->NY) % c->NY,
+                              zn = (z + CZ[i] + c->NZ) % c->NZ;
+                    b = flag[(size_t)xn + (size_t)c->NX * ((size_t)yn + (size_t)c->NY * zn)] != FLUID;
+                }
+                if (b) dflag[n] |= BOUNDARY;
+            }
+    hs.nfluid = nf;
+
+    if (hipMemcpyAsync(c->flag, dflag.data(), N, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+        hipMemcpyAsync(c->st, &hs, sizeof hs, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+        launch_fill_macro(c->rho, c->ux, c->uy, c->uz, c->prec == LBM_HIP_FP64, c->N, c->stream) != hipSuccess ||
+        dispatch(c->prec, [&](auto* tag) -> int {
+            using S = std::remove_pointer_t<decltype(tag)>;
+            return launch_fill_populations<S>((S*)c->f, c->N, c->stream) == hipSuccess ? 0 : -1;
+        }) != 0 ||
+        hipMemsetAsync(c->mass_part, 0, sizeof(double) * nparts, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+        return cleanup(fail(LBM_HIP_EDEVICE, "context initialisation failed"));
+    *out = c;
+    return LBM_HIP_OK;
+}
+
+void lbm_hip_destroy(lbm_hip_ctx* ctx) { free_ctx(ctx); }
+
+int lbm_hip_step(lbm_hip_ctx* c, int flags) {
+    if (!c) return fail(LBM_HIP_EINVAL, "null context");
+    int mode = 0;
+    if (flags & LBM_STEP_MACRO) mode |= STEP_MACRO;
+    if (flags & LBM_STEP_WALL_FORCE) mode |= STEP_WALLF;
+    if (flags & LBM_STEP_MASS) mode |= STEP_MASS;
+    const bool odd = (c->t & 1) != 0;
+    hipError_t e = hipSuccess;
+    dispatch(c->prec, [&](auto* tag) -> int {
+        using S = std::remove_pointer_t<decltype(tag)>;
+        e = launch_step<S>(step_args<S>(c), c->K, odd, mode, c->stream);
+        return 0;
+    });
+    if (e != hipSuccess) return fail(LBM_HIP_EDEVICE, std::string("step launch: ") + hipGetErrorString(e));
+    ++c->t;
+    c->macro_valid = (flags & LBM_STEP_MACRO) != 0;
+    c->wallf_valid = (flags & LBM_STEP_WALL_FORCE) != 0;
+    c->mass_valid = (flags & LBM_STEP_MASS) != 0;
+    return LBM_HIP_OK;
+}
+
+int lbm_hip_advance_squirmer(lbm_hip_ctx* c) {
+    if (!c) return fail(LBM_HIP_EINVAL, "null context");
+    if (!c->has_squirmer) return LBM_HIP_OK;
+    hipError_t e = hipSuccess;
+    dispatch(c->prec, [&](auto* tag) -> int {
+        using S = std::remove_pointer_t<decltype(tag)>;
+        e = launch_squirmer<S>(remap_args<S>(c), c->stream);
+        return 0;
+    });
+    if (e != hipSuccess) return fail(LBM_HIP_EDEVICE, std::string("squirmer launch: ") + hipGetErrorString(e));
+    return LBM_HIP_OK;
+}
+
+int lbm_hip_correct_mass(lbm_hip_ctx* c) {
+    if (!c) return fail(LBM_HIP_EINVAL, "null context");
+    hipError_t e;
+    const int shift_slot = (int)(c->t & 1);
+    if (c->mass_valid) {
+        e = launch_mass_finalize(c->mass_part, c->G, c->st, 1, shift_slot, c->stream);
+    } else {
+        const int shifted = layout_shifted(c) ? 1 : 0;
+        e = hipSuccess;
+        dispatch(c->prec, [&](auto* tag) -> int {
+            using S = std::remove_pointer_t<decltype(tag)>;
+            e = launch_mass<S>((const S*)c->f, c->flag, c->mass_part, c->NX, c->NY, c->NZ, c->N, shifted, c->st,
+                               shift_slot, c->stream);
+            return 0;
+        });
+        if (e == hipSuccess) e = launch_mass_finalize(c->mass_part, c->nb256, c->st, 0, shift_slot, c->stream);
+    }
+    if (e != hipSuccess) return fail(LBM_HIP_EDEVICE, std::string("mass correction: ") + hipGetErrorString(e));
+    c->mass_valid = false;
+    return LBM_HIP_OK;
+}
+
+int lbm_hip_run(lbm_hip_ctx* c, int steps, int mass_every, int last_flags) {
+    if (!c) return fail(LBM_HIP_EINVAL, "null context");
+    if (steps < 0) return fail(LBM_HIP_EINVAL, "steps must be >= 0");
+    for (int s = 1; s <= steps; ++s) {
+        const long long tnext = c->t + 1;  // 1-based step number (main.cpp:60)
+        const bool mass = c->has_squirmer && mass_every > 0 && tnext % mass_every == 0;
+        int fl = (s == steps ? last_flags : 0) | (mass ? LBM_STEP_MASS : 0);
+        int rc = lbm_hip_step(c, fl);
+        if (rc) return rc;
+        if (c->has_squirmer) {
+            rc = lbm_hip_advance_squirmer(c);
+            if (rc) return rc;
+            if (mass) {
+                rc = lbm_hip_correct_mass(c);
+                if (rc) return rc;
+            }
+        }
+    }
+    return LBM_HIP_OK;
+}
+
+int lbm_hip_synchronize(lbm_hip_ctx* c) {
+    if (!c) return fail(LBM_HIP_EINVAL, "null context");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return check_device_error(c);
+}
+
+int lbm_hip_get_squirmer(lbm_hip_ctx* c, lbm_squirmer_state* o) {
+    if (!c || !o) return fail(LBM_HIP_EINVAL, "null argument");
+    DevState hs;
+    HIP_TRY(hipMemcpyAsync(&hs, c->st, sizeof hs, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (hs.err & ERR_REMAP_OVERFLOW) return check_device_error(c);
+    const DevSquirmer& s = hs.sq;
+    o->R = s.R; o->B1 = s.B1; o->beta = s.beta; o->M = s.M; o->I = s.I; o->FS = s.FS;
+    for (int d = 0; d < 3; ++d) {
+        o->X[d] = s.X[d]; o->e[d] = s.e[d]; o->U[d] = s.U[d]; o->W[d] = s.W[d];
+        o->Fh[d] = s.Fh[d]; o->Th[d] = s.Th[d]; o->Fc[d] = s.Fc[d]; o->Tc[d] = s.Tc[d]; o->Fw[d] = s.Fw[d];
+    }
+    if (c->has_squirmer && c->t > 0) {
+        // LBM::step publishes Fh/Th at its end (lbm.cpp:220-223)
+        const double* acc = hs.acc[(c->t - 1) & 1];
+        for (int d = 0; d < 3; ++d) { o->Fh[d] = acc[3 + d]; o->Th[d] = acc[6 + d]; }
+    }
+    o->eps = s.eps; o->hc = s.hc; o->hmin = s.hmin;
+    o->contacts = s.contacts;
+    o->planar = s.planar; o->plane_xz = s.plane_xz;
+    return LBM_HIP_OK;
+}
+
+int lbm_hip_set_squirmer(lbm_hip_ctx* c, const lbm_squirmer_state* in) {
+    if (!c || !in) return fail(LBM_HIP_EINVAL, "null argument");
+    DevSquirmer s;
+    std::memset(&s, 0, sizeof s);
+    s.R = in->R; s.B1 = in->B1; s.beta = in->beta; s.M = in->M; s.I = in->I; s.FS = in->FS;
+    for (int d = 0; d < 3; ++d) {
+        s.X[d] = in->X[d]; s.e[d] = in->e[d]; s.U[d] = in->U[d]; s.W[d] = in->W[d];
+        s.Fh[d] = in->Fh[d]; s.Th[d] = in->Th[d]; s.Fc[d] = in->Fc[d]; s.Tc[d] = in->Tc[d]; s.Fw[d] = in->Fw[d];
+    }
+    s.eps = in->eps; s.hc = in->hc; s.hmin = in->hmin;
+    s.contacts = in->contacts;
+    s.planar = in->planar; s.plane_xz = in->plane_xz;
+    HIP_TRY(hipMemcpyAsync(&c->st->sq, &s, sizeof s, hipMemcpyHostToDevice, c->stream));
+    int band[6];
+    band_of(s, c->NX, c->NY, c->NZ, band);
+    HIP_TRY(hipMemcpyAsync(c->st->band, band, sizeof band, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return LBM_HIP_OK;
+}
+
+int lbm_hip_get_wall_force(lbm_hip_ctx* c, double out[3]) {
+    if (!c || !out) return fail(LBM_HIP_EINVAL, "null argument");
+    if (c->t == 0) { out[0] = out[1] = out[2] = 0.0; return LBM_HIP_OK; }
+    if (!c->wallf_valid) return fail(LBM_HIP_ESTATE, "the last step did not accumulate the wall force");
+    HIP_TRY(hipMemcpyAsync(out, c->st->acc[(c->t - 1) & 1], 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return LBM_HIP_OK;
+}
+
+int lbm_hip_diagnostics(lbm_hip_ctx* c, lbm_hip_diag* o) {
+    if (!c || !o) return fail(LBM_HIP_EINVAL, "null argument");
+    if (!c->macro_valid) return fail(LBM_HIP_ESTATE, "the last step did not write the macroscopic fields");
+    HIP_TRY(launch_diag(c->flag, c->rho, c->ux, c->uy, c->uz, c->prec == LBM_HIP_FP64, c->N, c->diag_part, c->nb256,
+                        c->diag_out, c->stream));
+    double h[4];
+    HIP_TRY(hipMemcpyAsync(h, c->diag_out, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    o->count_fluid = (int64_t)h[0];
+    o->mean_density = h[0] > 0 ? h[1] / h[0] : 0.0;
+    o->max_speed = h[2];
+    o->sum_ux = h[3];
+    return LBM_HIP_OK;
+}
+
+int lbm_hip_download(lbm_hip_ctx* c, double* f, double* rho, double* ux, double* uy, double* uz, uint8_t* flag) {
+    if (!c) return fail(LBM_HIP_EINVAL, "null context");
+    const size_t N = c->N;
+    if ((rho || ux || uy || uz) && !c->macro_valid)
+        return fail(LBM_HIP_ESTATE, "the last step did not write the macroscopic fields");
+    if (f) {
+        double* tmp = nullptr;
+        if (hipMalloc((void**)&tmp, sizeof(double) * Q * N) != hipSuccess)
+            return fail(LBM_HIP_ENOMEM, "download staging allocation failed");
+        hipError_t e = hipSuccess;
+        const int shifted = layout_shifted(c) ? 1 : 0;
+        dispatch(c->prec, [&](auto* tag) -> int {
+            using S = std::remove_pointer_t<decltype(tag)>;
+            e = launch_gather<S>((const S*)c->f, c->flag, tmp, c->NX, c->NY, c->NZ, c->N, shifted, c->st,
+                                 (int)(c->t & 1), c->stream);
+            return 0;
+        });
+        if (e == hipSuccess) e = hipMemcpyAsync(f, tmp, sizeof(double) * Q * N, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        hipFree(tmp);
+        if (e != hipSuccess) return fail(LBM_HIP_EDEVICE, std::string("download f: ") + hipGetErrorString(e));
+    }
+    double* outs[4] = {rho, ux, uy, uz};
+    void* srcs[4] = {c->rho, c->ux, c->uy, c->uz};
+    if (rho || ux || uy || uz) {
+        double* tmp = nullptr;
+        if (hipMalloc((void**)&tmp, sizeof(double) * N) != hipSuccess)
+            return fail(LBM_HIP_ENOMEM, "download staging allocation failed");
+        for (int k = 0; k < 4; ++k) {
+            if (!outs[k]) continue;
+            hipError_t e = launch_macro_to_double(srcs[k], c->prec == LBM_HIP_FP64, tmp, c->N, c->stream);
+            if (e == hipSuccess) e = hipMemcpyAsync(outs[k], tmp, sizeof(double) * N, hipMemcpyDeviceToHost, c->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+            if (e != hipSuccess) {
+                hipFree(tmp);
+                return fail(LBM_HIP_EDEVICE, std::string("download fields: ") + hipGetErrorString(e));
+            }
+        }
+        hipFree(tmp);
+    }
+    if (flag) {
+        HIP_TRY(hipMemcpyAsync(flag, c->flag, N, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        for (size_t n = 0; n < N; ++n) flag[n] &= TYPE_MASK;
+    }
+    return check_device_error(c);
+}
+
+int lbm_hip_upload_populations(lbm_hip_ctx* c, const double* f) {
+    if (!c || !f) return fail(LBM_HIP_EINVAL, "null argument");
+    const size_t N = c->N;
+    double* tmp = nullptr;
+    if (hipMalloc((void**)&tmp, sizeof(double) * Q * N) != hipSuccess)
+        return fail(LBM_HIP_ENOMEM, "upload staging allocation failed");
+    hipError_t e = hipMemcpyAsync(tmp, f, sizeof(double) * Q * N, hipMemcpyHostToDevice, c->stream);
+    const int shifted = layout_shifted(c) ? 1 : 0;
+    if (e == hipSuccess)
+        dispatch(c->prec, [&](auto* tag) -> int {
+            using S = std::remove_pointer_t<decltype(tag)>;
+            e = launch_scatter<S>((S*)c->f, c->flag, tmp, c->NX, c->NY, c->NZ, c->N, shifted, c->stream);
+            return 0;
+        });
+    const double zero = 0.0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&c->st->shift[c->t & 1], &zero, sizeof zero, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    hipFree(tmp);
+    if (e != hipSuccess) return fail(LBM_HIP_EDEVICE, std::string("upload: ") + hipGetErrorString(e));
+    c->mass_valid = false;
+    return LBM_HIP_OK;
+}
+
+int lbm_hip_get_info(lbm_hip_ctx* c, lbm_hip_info* o) {
+    if (!c || !o) return fail(LBM_HIP_EINVAL, "null argument");
+    o->nx = c->NX; o->ny = c->NY; o->nz = c->NZ; o->precision = c->prec;
+    o->n = c->N;
+    o->steps_done = c->t;
+    o->tau = c->p.tau; o->om_plus = c->omPlus; o->om_minus = c->omMinus; o->nu = c->nu;
+    o->bytes_per_lup = 2.0 * Q * (double)c->ssize;
+    o->device_bytes = c->device_bytes;
+    o->has_squirmer = c->has_squirmer; o->walls_y = c->p.walls_y; o->walls_z = c->p.walls_z;
+    return LBM_HIP_OK;
+}
+
+int lbm_hip_time_run(lbm_hip_ctx* c, int steps, int mass_every, double* total_ms, double* step_kernel_ms) {
+    if (!c) return fail(LBM_HIP_EINVAL, "null context");
+    if (steps <= 0) return fail(LBM_HIP_EINVAL, "steps must be > 0");
+    const size_t need = 2 * (size_t)steps + 2;
+    while (c->events.size() < need) {
+        hipEvent_t e;
+        HIP_TRY(hipEventCreate(&e));
+        c->events.push_back(e);
+    }
+    hipEvent_t* ev = c->events.data();
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipEventRecord(ev[0], c->stream));
+    for (int s = 0; s < steps; ++s) {
+        const long long tnext = c->t + 1;
+        const bool mass = c->has_squirmer && mass_every > 0 && tnext % mass_every == 0;
+        HIP_TRY(hipEventRecord(ev[2 + 2 * s], c->stream));
+        int rc = lbm_hip_step(c, mass ? LBM_STEP_MASS : 0);
+        if (rc) return rc;
+        HIP_TRY(hipEventRecord(ev[3 + 2 * s], c->stream));
+        if (c->has_squirmer) {
+            rc = lbm_hip_advance_squirmer(c);
+            if (rc) return rc;
+            if (mass && (rc = lbm_hip_correct_mass(c))) return rc;
+        }
+    }
+    HIP_TRY(hipEventRecord(ev[1], c->stream));
+    HIP_TRY(hipEventSynchronize(ev[1]));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[1]));
+    double ksum = 0.0;
+    for (int s = 0; s < steps; ++s) {
+        float k = 0.f;
+        HIP_TRY(hipEventElapsedTime(&k, ev[2 + 2 * s], ev[3 + 2 * s]));
+        ksum += k;
+    }
+    if (total_ms) *total_ms = ms;
+    if (step_kernel_ms) *step_kernel_ms = ksum;
+    return check_device_error(c);
+}
+
+}  // extern "C"

@@ -1,0 +1,455 @@
+// squirmer_kernel.hpp -- LBM::advanceSquirmer on the device (one block):
+// soft wall potential, Newton-Euler update and hard gap in FP64 by one
+// thread, then the remap of the sphere onto the lattice.
+//
+// The reference remap (src/utils/lbm.cpp:271-320) is a sequential sweep over
+// the sphere's bounding box in z, y, x order, in place: an uncovered node's
+// refill density averages the FLUID neighbours as they are at that point of
+// the sweep, and the cover/uncover momentum sums are order dependent.  Here
+//   B. all threads classify the box in parallel and compact the changed
+//      nodes in sweep order (per-thread chunks + one block scan);
+//   C. one wavefront walks the changed list in that order, one lane per
+//      direction, summing in the reference's order on every lane, so the
+//      FP64 results are the reference's bit for bit;
+//   D. all threads move populations whose AA storage slot depends on the
+//      flags that just changed, write the equilibrium of uncovered nodes
+//      into the slots the next step reads, and refresh BOUNDARY bits.
+//
+// Storage slot of the population f_i(n) that the next step reads
+// ("loc(n, i)"): after an odd step (natural layout) slot i at n; after an
+// even step (shifted layout) slot opp(i) at n - c_i when that node is FLUID,
+// else slot i at n.  In the shifted layout a flag change therefore moves
+// the slots of the links between the changed node and its unchanged FLUID
+// neighbours; phase D applies exactly those moves.  Every such link owns a
+// pair of slots no other link touches, so phase D runs without ordering.
+#pragma once
+#include "device_state.hpp"
+#include "step_kernel.hpp"
+#include "storage.hpp"
+
+namespace lbmhip {
+
+constexpr int SQ_THREADS = 1024;
+constexpr int REMAP_CAP = 1024;  // changed nodes per remap held in LDS
+
+template <class S>
+struct RemapArgs {
+    S* f;
+    uint8_t* flag;
+    void* rho;
+    void* ux;
+    void* uy;
+    void* uz;
+    DevState* st;
+    int NX, NY, NZ;
+    uint32_t N;
+    int walls_y, walls_z;
+    int shifted;     // layout after the last step: 1 after an even step
+    int acc_slot;    // accumulator slot of the last step
+    int shift_slot;  // mass-shift slot the next step reads
+};
+
+// squirmer.cpp:73-77
+__device__ inline double soft_wall_force(double h, double eps, double FS, double hc) {
+    if (h >= hc) return 0.0;
+    const double hp = (h < 0.0) ? 0.0 : h;
+    const double s = 1.0 - hp / hc;
+    return eps * FS * s * s;
+}
+
+// lattice.hpp:42-47
+__device__ inline double equilibrium(int i, double rho, double ux, double uy, double uz) {
+    const double cu = CX[i] * ux + CY[i] * uy + CZ[i] * uz;
+    const double usq = ux * ux + uy * uy + uz * uz;
+    return W[i] * rho * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * usq);
+}
+
+__device__ inline int wrapi(int a, int n) { return ((a % n) + n) % n; }
+
+// lbm.cpp:100-104 at integer node coordinates
+__device__ inline void node_offset(const DevSquirmer& sq, int NX, int NY, int NZ, int wy, int wz, int x, int y, int z,
+                                   double r[3]) {
+    r[0] = x - sq.X[0]; r[0] -= NX * round(r[0] / NX);
+    r[1] = y - sq.X[1]; if (!wy) r[1] -= NY * round(r[1] / NY);
+    r[2] = z - sq.X[2]; if (!wz) r[2] -= NZ * round(r[2] / NZ);
+}
+
+// Read a device-side macroscopic value as double.
+template <class S>
+__device__ inline double load_macro(const void* p, uint32_t n) {
+    if constexpr (std::is_same<S, double>::value) return static_cast<const double*>(p)[n];
+    else return (double)static_cast<const float*>(p)[n];
+}
+template <class S>
+__device__ inline void store_macro(void* p, uint32_t n, double v) {
+    if constexpr (std::is_same<S, double>::value) static_cast<double*>(p)[n] = v;
+    else static_cast<float*>(p)[n] = (float)v;
+}
+
+template <class S>
+__global__ __launch_bounds__(SQ_THREADS) void squirmer_kernel(const RemapArgs<S> A) {
+    __shared__ uint32_t list[REMAP_CAP];
+    __shared__ double lrr[REMAP_CAP], lub[REMAP_CAP][3];
+    __shared__ int sbox[6];  // x0, y0, z0, wx, wy, wz
+    __shared__ int scount[SQ_THREADS / 64];
+    __shared__ int sK, sOverflow;
+
+    DevState* st = A.st;
+    DevSquirmer& sq = st->sq;
+    const int tid = threadIdx.x;
+    const int NX = A.NX, NY = A.NY, NZ = A.NZ;
+    const uint32_t N = A.N;
+
+    // ---------------- A: rigid body (lbm.cpp:257-262) ----------------
+    if (tid == 0) {
+        const double* acc = st->acc[A.acc_slot];
+        for (int d = 0; d < 3; ++d) { sq.Fh[d] = acc[3 + d]; sq.Th[d] = acc[6 + d]; }
+        // applyWallPotential (lbm.cpp:231-241)
+        sq.Fw[0] = sq.Fw[1] = sq.Fw[2] = 0.0;
+        if (A.walls_y) {
+            sq.Fw[1] += soft_wall_force(sq.X[1] - sq.R - 0.5, sq.eps, sq.FS, sq.hc);
+            sq.Fw[1] -= soft_wall_force((NY - 1.5) - sq.X[1] - sq.R, sq.eps, sq.FS, sq.hc);
+        }
+        if (A.walls_z) {
+            sq.Fw[2] += soft_wall_force(sq.X[2] - sq.R - 0.5, sq.eps, sq.FS, sq.hc);
+            sq.Fw[2] -= soft_wall_force((NZ - 1.5) - sq.X[2] - sq.R, sq.eps, sq.FS, sq.hc);
+        }
+        // Squirmer::integrate (squirmer.cpp:52-68)
+        for (int d = 0; d < 3; ++d) {
+            sq.U[d] += (sq.Fh[d] + sq.Fc[d] + sq.Fw[d]) / sq.M;
+            sq.W[d] += (sq.Th[d] + sq.Tc[d]) / sq.I;
+            sq.Fc[d] = sq.Tc[d] = 0.0;
+        }
+        if (sq.planar) {
+            if (sq.plane_xz) { sq.U[1] = 0.0; sq.W[0] = 0.0; sq.W[2] = 0.0; }
+            else { sq.U[2] = 0.0; sq.W[0] = 0.0; sq.W[1] = 0.0; }
+        }
+        for (int d = 0; d < 3; ++d) sq.X[d] += sq.U[d];
+        double we[3];
+        we[0] = sq.W[1] * sq.e[2] - sq.W[2] * sq.e[1];
+        we[1] = sq.W[2] * sq.e[0] - sq.W[0] * sq.e[2];
+        we[2] = sq.W[0] * sq.e[1] - sq.W[1] * sq.e[0];
+        for (int d = 0; d < 3; ++d) sq.e[d] += we[d];
+        if (sq.planar) {
+            if (sq.plane_xz) sq.e[1] = 0.0;
+            else sq.e[2] = 0.0;
+        }
+        const double nrm = sqrt(sq.e[0] * sq.e[0] + sq.e[1] * sq.e[1] + sq.e[2] * sq.e[2]);
+        for (int d = 0; d < 3; ++d) sq.e[d] /= nrm;
+        // enforceMinimumGap (lbm.cpp:245-253)
+        auto clamp_axis = [&](int d, int Nd) {
+            const double lo = 0.5 + sq.R + sq.hmin, hi = (Nd - 1.5) - sq.R - sq.hmin;
+            if (sq.X[d] < lo) { sq.X[d] = lo; if (sq.U[d] < 0) sq.U[d] = 0; ++sq.contacts; }
+            if (sq.X[d] > hi) { sq.X[d] = hi; if (sq.U[d] > 0) sq.U[d] = 0; ++sq.contacts; }
+        };
+        if (A.walls_y) clamp_axis(1, NY);
+        if (A.walls_z) clamp_axis(2, NZ);
+
+        // remap box (lbm.cpp:273-276); an axis longer than the domain is cut
+        // to one period, since later periodic copies of a node are no-ops in
+        // the reference's in-place sweep.
+        const double R = sq.R;
+        const int x0 = (int)floor(sq.X[0] - R) - 2, x1 = (int)ceil(sq.X[0] + R) + 2;
+        int y0 = (int)floor(sq.X[1] - R) - 2, y1 = (int)ceil(sq.X[1] + R) + 2;
+        const int z0 = (int)floor(sq.X[2] - R) - 2, z1 = (int)ceil(sq.X[2] + R) + 2;
+        if (A.walls_y) { y0 = y0 < 0 ? 0 : y0; y1 = y1 > NY - 1 ? NY - 1 : y1; }
+        int wx = x1 - x0 + 1, wy = y1 - y0 + 1, wz = z1 - z0 + 1;
+        wx = wx > NX ? NX : wx; wy = wy > NY ? NY : wy; wz = wz > NZ ? NZ : wz;
+        wy = wy < 0 ? 0 : wy;
+        sbox[0] = x0; sbox[1] = y0; sbox[2] = z0; sbox[3] = wx; sbox[4] = wy; sbox[5] = wz;
+        sOverflow = 0;
+    }
+    __syncthreads();
+
+    // ---------------- B: classify the box in sweep order ----------------
+    const int bx0 = sbox[0], by0 = sbox[1], bz0 = sbox[2], bwx = sbox[3], bwy = sbox[4], bwz = sbox[5];
+    const int M = bwx * bwy * bwz;
+    const int chunk = (M + SQ_THREADS - 1) / SQ_THREADS;
+    const int k0 = tid * chunk, k1 = (k0 + chunk < M) ? k0 + chunk : M;
+    const double R2 = sq.R * sq.R;
+    auto classify = [&](int k, uint32_t& node) -> int {
+        const int xx = bx0 + k % bwx, yy = by0 + (k / bwx) % bwy, zz = bz0 + k / (bwx * bwy);
+        const int x = wrapi(xx, NX), y = wrapi(yy, NY), z = wrapi(zz, NZ);
+        const uint32_t n = (uint32_t)x + (uint32_t)NX * ((uint32_t)y + (uint32_t)NY * (uint32_t)z);
+        const uint8_t t = A.flag[n] & TYPE_MASK;
+        if (t == WALL) return 0;
+        double r[3];
+        node_offset(sq, NX, NY, NZ, A.walls_y, A.walls_z, x, y, z, r);
+        const bool inside = r[0] * r[0] + r[1] * r[1] + r[2] * r[2] < R2;
+        const bool was = t == PARTICLE;
+        if (inside == was) return 0;
+        node = n;
+        return inside ? 2 : 1;  // 2: cover, 1: uncover
+    };
+    int cnt = 0;
+    for (int k = k0; k < k1; ++k) {
+        uint32_t nn;
+        cnt += classify(k, nn) != 0;
+    }
+    // block exclusive scan of cnt
+    const int lane = tid & 63, wave = tid >> 6;
+    int incl = cnt;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int v = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += v;
+    }
+    if (lane == 63) scount[wave] = incl;
+    __syncthreads();
+    if (tid == 0) {
+        int s = 0;
+        for (int w = 0; w < SQ_THREADS / 64; ++w) { const int c = scount[w]; scount[w] = s; s += c; }
+        sK = s;
+        if (s > REMAP_CAP) { sOverflow = 1; atomicOr(&st->err, ERR_REMAP_OVERFLOW); }
+    }
+    __syncthreads();
+    const int K = sK;
+    if (sOverflow) return;  // uniform: the host reports the error
+    {
+        int pos = scount[wave] + incl - cnt;
+        for (int k = k0; k < k1 && cnt > 0; ++k) {
+            uint32_t nn;
+            const int c = classify(k, nn);
+            if (c) { list[pos++] = nn | (c == 2 ? 0x80000000u : 0u); --cnt; }
+        }
+    }
+    __syncthreads();
+
+    // ---------------- C: sequential cover / uncover (wave 0) ----------------
+    if (wave == 0) {
+        double Fc[3] = {0, 0, 0}, Tc[3] = {0, 0, 0}, dmass = 0.0;
+        long long dnf = 0;
+        const double dsh = st->shift[A.shift_slot];
+        const int i = lane;  // direction handled by this lane (lanes >= Q idle)
+        const int cxi = i < Q ? CX[i] : 0, cyi = i < Q ? CY[i] : 0, czi = i < Q ? CZ[i] : 0;
+        for (int k = 0; k < K; ++k) {
+            const uint32_t e = list[k];
+            const uint32_t n = e & 0x7FFFFFFFu;
+            const bool cover = (e >> 31) != 0;
+            const int x = (int)(n % (uint32_t)NX), y = (int)((n / (uint32_t)NX) % (uint32_t)NY),
+                      z = (int)(n / ((uint32_t)NX * (uint32_t)NY));
+            double r[3];
+            node_offset(sq, NX, NY, NZ, A.walls_y, A.walls_z, x, y, z, r);
+            if (cover) {
+                // population f_i(n) after the last step (old flags), lbm.cpp:295-298
+                double fv = 0.0;
+                if (i < Q) {
+                    size_t slot = (size_t)i * N + n;
+                    if (A.shifted && i > 0) {
+                        const uint32_t m = (uint32_t)wrapi(x - cxi, NX) +
+                                           (uint32_t)NX * ((uint32_t)wrapi(y - cyi, NY) +
+                                                           (uint32_t)NY * (uint32_t)wrapi(z - czi, NZ));
+                        const uint8_t fm = A.flag[m];
+                        uint8_t tm = fm & TYPE_MASK;
+                        if (fm & FLIPPED) tm = (tm == FLUID) ? PARTICLE : FLUID;
+                        if (tm == FLUID) slot = (size_t)OPP[i] * N + m;
+                    }
+                    fv = to_real<S>(A.f[slot], i);
+                    if (dsh != 0.0) fv += W[i] * dsh;
+                }
+                double p[3] = {0, 0, 0}, msum = 0.0;
+                for (int j = 0; j < Q; ++j) {
+                    const double v = __shfl(fv, j, 64);
+                    p[0] += v * CX[j]; p[1] += v * CY[j]; p[2] += v * CZ[j];
+                    msum += v;
+                }
+                const double t[3] = {r[1] * p[2] - r[2] * p[1], r[2] * p[0] - r[0] * p[2], r[0] * p[1] - r[1] * p[0]};
+                for (int d = 0; d < 3; ++d) { Fc[d] += p[d]; Tc[d] += t[d]; }
+                dmass -= msum - 1.0;
+                --dnf;
+                if (lane == 0) A.flag[n] = (uint8_t)(PARTICLE | FLIPPED);
+            } else {
+                // refill density from FLUID neighbours as of this point of the sweep
+                double rv = 0.0;
+                int isf = 0;
+                if (i >= 1 && i < Q) {
+                    const uint32_t m = (uint32_t)wrapi(x + cxi, NX) +
+                                       (uint32_t)NX * ((uint32_t)wrapi(y + cyi, NY) +
+                                                       (uint32_t)NY * (uint32_t)wrapi(z + czi, NZ));
+                    if ((A.flag[m] & TYPE_MASK) == FLUID) { rv = load_macro<S>(A.rho, m); isf = 1; }
+                }
+                double rs = 0.0;
+                int c = 0;
+                for (int j = 1; j < Q; ++j) {
+                    const double v = __shfl(rv, j, 64);
+                    if (__shfl(isf, j, 64)) { rs += v; ++c; }
+                }
+                const double rr = c ? rs / c : 1.0;
+                double ub[3];  // Squirmer::surfaceVelocity(r, ub, false)
+                ub[0] = sq.U[0] + (sq.W[1] * r[2] - sq.W[2] * r[1]);
+                ub[1] = sq.U[1] + (sq.W[2] * r[0] - sq.W[0] * r[2]);
+                ub[2] = sq.U[2] + (sq.W[0] * r[1] - sq.W[1] * r[0]);
+                double fsum = 0.0;
+                for (int j = 0; j < Q; ++j) fsum += equilibrium(j, rr, ub[0], ub[1], ub[2]);
+                const double p[3] = {rr * ub[0], rr * ub[1], rr * ub[2]};
+                const double t[3] = {r[1] * p[2] - r[2] * p[1], r[2] * p[0] - r[0] * p[2], r[0] * p[1] - r[1] * p[0]};
+                for (int d = 0; d < 3; ++d) { Fc[d] -= p[d]; Tc[d] -= t[d]; }
+                dmass += fsum - 1.0;
+                ++dnf;
+                if (lane == 0) {
+                    store_macro<S>(A.rho, n, rr);
+                    store_macro<S>(A.ux, n, ub[0]);
+                    store_macro<S>(A.uy, n, ub[1]);
+                    store_macro<S>(A.uz, n, ub[2]);
+                    A.flag[n] = (uint8_t)(FLUID | FLIPPED);
+                    lrr[k] = rr; lub[k][0] = ub[0]; lub[k][1] = ub[1]; lub[k][2] = ub[2];
+                }
+            }
+            // make this node's flag / rho visible to the next iteration's loads
+            __threadfence_block();
+        }
+        if (lane == 0) {
+            for (int d = 0; d < 3; ++d) { sq.Fc[d] += Fc[d]; sq.Tc[d] += Tc[d]; }
+            st->remap_dmass += dmass;
+            st->nfluid += dnf;
+            st->changed = K;
+        }
+    }
+    __syncthreads();
+
+    // ---------------- D: storage moves and equilibrium refill ----------------
+    S* __restrict__ f = A.f;
+    for (int w = tid; w < K * Q; w += SQ_THREADS) {
+        const int k = w / Q, i = w - k * Q;
+        const uint32_t e = list[k];
+        const uint32_t n = e & 0x7FFFFFFFu;
+        const bool cover = (e >> 31) != 0;
+        if (!A.shifted || i == 0) {
+            if (!cover) f[(size_t)i * N + n] = from_real<S>(equilibrium(i, lrr[k], lub[k][0], lub[k][1], lub[k][2]), i);
+            continue;
+        }
+        const int x = (int)(n % (uint32_t)NX), y = (int)((n / (uint32_t)NX) % (uint32_t)NY),
+                  z = (int)(n / ((uint32_t)NX * (uint32_t)NY));
+        const uint32_t m = (uint32_t)wrapi(x - CX[i], NX) +
+                           (uint32_t)NX * ((uint32_t)wrapi(y - CY[i], NY) + (uint32_t)NY * (uint32_t)wrapi(z - CZ[i], NZ));
+        const uint8_t fm = A.flag[m];
+        const bool mfluid = (fm & TYPE_MASK) == FLUID, mflip = (fm & FLIPPED) != 0;
+        const size_t sin_ = (size_t)i * N + n, sout = (size_t)OPP[i] * N + m;
+        if (!cover) {
+            const S fe = from_real<S>(equilibrium(i, lrr[k], lub[k][0], lub[k][1], lub[k][2]), i);
+            if (mfluid && !mflip) { f[sin_] = f[sout]; f[sout] = fe; }
+            else if (mfluid) f[sout] = fe;
+            else f[sin_] = fe;
+        } else {
+            if (mfluid && !mflip) f[sout] = f[sin_];
+        }
+    }
+    __syncthreads();
+    for (int k = tid; k < K; k += SQ_THREADS) {
+        const uint32_t n = list[k] & 0x7FFFFFFFu;
+        A.flag[n] = A.flag[n] & TYPE_MASK;
+    }
+    __syncthreads();
+    // BOUNDARY bits of the changed nodes and their neighbours
+    for (int w = tid; w < K * Q; w += SQ_THREADS) {
+        const int k = w / Q, i = w - k * Q;
+        const uint32_t n0 = list[k] & 0x7FFFFFFFu;
+        const int x0 = (int)(n0 % (uint32_t)NX), y0 = (int)((n0 / (uint32_t)NX) % (uint32_t)NY),
+                  z0 = (int)(n0 / ((uint32_t)NX * (uint32_t)NY));
+        const int x = wrapi(x0 + CX[i], NX), y = wrapi(y0 + CY[i], NY), z = wrapi(z0 + CZ[i], NZ);
+        const uint32_t m = (uint32_t)x + (uint32_t)NX * ((uint32_t)y + (uint32_t)NY * (uint32_t)z);
+        const uint8_t t = A.flag[m] & TYPE_MASK;
+        bool b = false;
+        if (t == FLUID) {
+            for (int j = 1; j < Q; ++j) {
+                const uint32_t mm = (uint32_t)wrapi(x + CX[j], NX) +
+                                    (uint32_t)NX * ((uint32_t)wrapi(y + CY[j], NY) + (uint32_t)NY * (uint32_t)wrapi(z + CZ[j], NZ));
+                b |= (A.flag[mm] & TYPE_MASK) != FLUID;
+            }
+        }
+        A.flag[m] = (uint8_t)(t | (b ? BOUNDARY : 0));
+    }
+    if (tid == 0) {
+        // rho band the next step writes: the next remap box plus neighbours,
+        // with a margin for one step of motion (|U| < 1)
+        const double R = sq.R;
+        int lo[3], wd[3];
+        const int Nd[3] = {NX, NY, NZ};
+        for (int d = 0; d < 3; ++d) {
+            const int a = (int)floor(sq.X[d] - R) - 4, b = (int)ceil(sq.X[d] + R) + 4;
+            int wdt = b - a + 1;
+            wdt = wdt > Nd[d] ? Nd[d] : wdt;
+            lo[d] = wrapi(a, Nd[d]);
+            wd[d] = wdt;
+        }
+        for (int d = 0; d < 3; ++d) { st->band[d] = lo[d]; st->band[3 + d] = wd[d]; }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Logical population f_i(n) for download / upload (any layout).
+// ---------------------------------------------------------------------------
+template <class S>
+__global__ __launch_bounds__(256) void gather_kernel(const S* __restrict__ f, const uint8_t* __restrict__ flag,
+                                                     double* __restrict__ out, int NX, int NY, int NZ, uint32_t N,
+                                                     int shifted, const DevState* st, int shift_slot) {
+    const uint32_t n = blockIdx.x * 256u + threadIdx.x;
+    if (n >= N) return;
+    const int x = (int)(n % (uint32_t)NX), y = (int)((n / (uint32_t)NX) % (uint32_t)NY),
+              z = (int)(n / ((uint32_t)NX * (uint32_t)NY));
+    const bool fluid = (flag[n] & TYPE_MASK) == FLUID;
+    const double dsh = st->shift[shift_slot];
+    for (int i = 0; i < Q; ++i) {
+        size_t slot = (size_t)i * N + n;
+        if (fluid && shifted && i > 0) {
+            const uint32_t m = (uint32_t)wrapi(x - CX[i], NX) +
+                               (uint32_t)NX * ((uint32_t)wrapi(y - CY[i], NY) + (uint32_t)NY * (uint32_t)wrapi(z - CZ[i], NZ));
+            if ((flag[m] & TYPE_MASK) == FLUID) slot = (size_t)OPP[i] * N + m;
+        }
+        double v = to_real<S>(f[slot], i);
+        if (fluid && dsh != 0.0) v += W[i] * dsh;
+        out[(size_t)i * N + n] = v;
+    }
+}
+
+template <class S>
+__global__ __launch_bounds__(256) void scatter_kernel(S* __restrict__ f, const uint8_t* __restrict__ flag,
+                                                      const double* __restrict__ in, int NX, int NY, int NZ, uint32_t N,
+                                                      int shifted) {
+    const uint32_t n = blockIdx.x * 256u + threadIdx.x;
+    if (n >= N) return;
+    const int x = (int)(n % (uint32_t)NX), y = (int)((n / (uint32_t)NX) % (uint32_t)NY),
+              z = (int)(n / ((uint32_t)NX * (uint32_t)NY));
+    const bool fluid = (flag[n] & TYPE_MASK) == FLUID;
+    for (int i = 0; i < Q; ++i) {
+        size_t slot = (size_t)i * N + n;
+        if (fluid && shifted && i > 0) {
+            const uint32_t m = (uint32_t)wrapi(x - CX[i], NX) +
+                               (uint32_t)NX * ((uint32_t)wrapi(y - CY[i], NY) + (uint32_t)NY * (uint32_t)wrapi(z - CZ[i], NZ));
+            if ((flag[m] & TYPE_MASK) == FLUID) slot = (size_t)OPP[i] * N + m;
+        }
+        f[slot] = from_real<S>(in[(size_t)i * N + n], i);
+    }
+}
+
+// Excess mass sum_i f_i - 1 of every FLUID node from the logical populations,
+// one partial per block (used when the last step emitted no mass partials).
+template <class S>
+__global__ __launch_bounds__(256) void mass_kernel(const S* __restrict__ f, const uint8_t* __restrict__ flag,
+                                                   double* __restrict__ part, int NX, int NY, int NZ, uint32_t N,
+                                                   int shifted, const DevState* st, int shift_slot) {
+    __shared__ double lds[4];
+    const uint32_t n = blockIdx.x * 256u + threadIdx.x;
+    double ex[1] = {0.0};
+    if (n < N && (flag[n] & TYPE_MASK) == FLUID) {
+        const int x = (int)(n % (uint32_t)NX), y = (int)((n / (uint32_t)NX) % (uint32_t)NY),
+                  z = (int)(n / ((uint32_t)NX * (uint32_t)NY));
+        const double dsh = st->shift[shift_slot];
+        double s = 0.0;
+        for (int i = 0; i < Q; ++i) {
+            size_t slot = (size_t)i * N + n;
+            if (shifted && i > 0) {
+                const uint32_t m = (uint32_t)wrapi(x - CX[i], NX) +
+                                   (uint32_t)NX * ((uint32_t)wrapi(y - CY[i], NY) + (uint32_t)NY * (uint32_t)wrapi(z - CZ[i], NZ));
+                if ((flag[m] & TYPE_MASK) == FLUID) slot = (size_t)OPP[i] * N + m;
+            }
+            double v = to_real<S>(f[slot], i);
+            if (dsh != 0.0) v += W[i] * dsh;
+            s += v;
+        }
+        ex[0] = s - 1.0;
+    }
+    block_reduce<1>(ex, lds);
+    if (threadIdx.x == 0) part[blockIdx.x] = ex[0];
+}
+
+}  // namespace lbmhip

@@ -1,0 +1,452 @@
+// step_kernel.hpp -- the fused D3Q19 TRT collide + stream kernel (gfx950).
+//
+// One thread per lattice node, one population array updated in place with
+// the AA access pattern (P. Bailey et al., ICPP 2009):
+//
+//   even step t: node n reads f_i from slot i at n, collides and writes
+//                f*_i into slot opp(i) at n            (all accesses local)
+//   odd step t:  node n reads f_i from slot opp(i) at n - c_i, collides and
+//                writes f*_i into slot i at n + c_i    (all accesses remote)
+//
+// After an odd step the array is in the natural layout (slot i at n holds
+// the f_i that node n collides next), after an even step it is "shifted".
+// Each slot is read and written by one thread only, so the update is race
+// free without a second array.  This is the reference's push step
+// (src/utils/lbm.cpp:128-224) with the same post-collision values; only the
+// places the values are kept differ.
+//
+// Halfway bounce-back is folded into the same stores.  A link (n, i) with a
+// non-fluid neighbour n + c_i returns f*_i - 6 w_i rho (c_i . u_b) as
+// f_opp(i)(n) of the next step (lbm.cpp:189-199); the value goes to slot
+// opp(i) at n in both parities, which is exactly where the next step's node
+// n reads f_opp(i) when its upstream neighbour is solid.  Node n only looks
+// at neighbour flags when its own flag byte carries the BOUNDARY bit, so
+// bulk wavefronts run the plain stream code.
+//
+// Momentum exchange (lbm.cpp:202-215): PARTICLE links accumulate force and
+// torque in FP64 per thread, then per block through wavefront reductions and
+// LDS, then one FP64 atomic per component per block into the step's
+// accumulator.  WALL links do the same on request (the wall force is only a
+// diagnostic).
+#pragma once
+#include <type_traits>
+
+#include "device_state.hpp"
+#include "storage.hpp"
+
+namespace lbmhip {
+
+constexpr int STEP_MACRO = 1;
+constexpr int STEP_WALLF = 2;
+constexpr int STEP_MASS = 4;
+
+constexpr int BLOCK = 256;
+
+template <int B, int E, class Fn>
+__device__ __forceinline__ void static_for(Fn&& fn) {
+    if constexpr (B < E) {
+        fn(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(fn);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 collision in the reference's operation order (lbm.cpp:143-172).
+// g holds f_i; on return g holds f*_i.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void collide_exact(double (&g)[Q], double omP, double omM, double Fx, double Fy,
+                                              double Fz, double& r, double& u, double& v, double& wz) {
+    r = 0.0;
+    double mx = 0.0, my = 0.0, mz = 0.0;
+    static_for<0, Q>([&](auto I) {
+        constexpr int i = decltype(I)::value;
+        r += g[i];
+        if constexpr (CX[i] != 0) mx += (double)CX[i] * g[i];
+        if constexpr (CY[i] != 0) my += (double)CY[i] * g[i];
+        if constexpr (CZ[i] != 0) mz += (double)CZ[i] * g[i];
+    });
+    u = (mx + 0.5 * Fx) / r;
+    v = (my + 0.5 * Fy) / r;
+    wz = (mz + 0.5 * Fz) / r;
+    const double usq = u * u + v * v + wz * wz, uF = u * Fx + v * Fy + wz * Fz;
+    const double gP = 1.0 - 0.5 * omP, gM = 1.0 - 0.5 * omM;
+
+    {  // rest population: c_0 = 0
+        const double cu = 0.0, cF = 0.0;
+        const double feqP = W[0] * r * (1.0 + 4.5 * cu * cu - 1.5 * usq);
+        const double feqM = W[0] * r * 3.0 * cu;
+        const double fP = 0.5 * (g[0] + g[0]), fM = 0.5 * (g[0] - g[0]);
+        const double SP = W[0] * (9.0 * cu * cF - 3.0 * uF), SM = W[0] * 3.0 * cF;
+        g[0] = g[0] - omP * (fP - feqP) - omM * (fM - feqM) + gP * SP + gM * SM;
+    }
+    // Opposite pairs (i, i+1): every term of direction i+1 is the exact
+    // negation or copy of the one of direction i, so the four products are
+    // formed once and combined in the reference's order for both.
+    static_for<0, 9>([&](auto P) {
+        constexpr int i = 2 * decltype(P)::value + 1, io = i + 1;
+        double cu, cF;
+        {
+            bool first = true;
+            auto addc = [&](int c, double a, double b) {
+                const double ta = c > 0 ? a : -a, tb = c > 0 ? b : -b;
+                if (first) { cu = ta; cF = tb; first = false; }
+                else { cu = cu + ta; cF = cF + tb; }
+            };
+            if (CX[i] != 0) addc(CX[i], u, Fx);
+            if (CY[i] != 0) addc(CY[i], v, Fy);
+            if (CZ[i] != 0) addc(CZ[i], wz, Fz);
+        }
+        const double feqP = W[i] * r * (1.0 + 4.5 * cu * cu - 1.5 * usq);
+        const double feqM = W[i] * r * 3.0 * cu;
+        const double fP = 0.5 * (g[i] + g[io]), fM = 0.5 * (g[i] - g[io]);
+        const double SP = W[i] * (9.0 * cu * cF - 3.0 * uF), SM = W[i] * 3.0 * cF;
+        const double A = omP * (fP - feqP), B = omM * (fM - feqM), C = gP * SP, D = gM * SM;
+        const double gi = g[i], gio = g[io];
+        g[i] = gi - A - B + C + D;
+        g[io] = gio - A + B + C - D;
+    });
+}
+
+// ---------------------------------------------------------------------------
+// FP32 collision on shifted, scaled populations gs_i = s (f_i - w_i).
+// Same TRT + Guo update; constants fold the scale.  On return gs holds
+// s (f*_i - w_i); rho is the density and (jx, jy, jz) = s * rho * u.
+// ---------------------------------------------------------------------------
+struct FastCoef {
+    float inv_s, s, a, b, one_m_omp;  // a = omP/2, b = omM/2
+    float kp_axis, kp_diag, kp_rest;  // omP * w_i
+    float km_axis, km_diag;           // 3 * omM * w_i
+    float hFx, hFy, hFz;              // s * F / 2
+    float gPw9[3], gPw3[3];           // 9 gP w (axis/diag/rest), 3 gP w
+    float sm_axis, sm_diag;           // 3 s gM w
+    float Fx, Fy, Fz;
+};
+
+template <bool FORCE>
+__device__ __forceinline__ void collide_fast(float (&g)[Q], const FastCoef& K, float& rho, float& jx, float& jy,
+                                             float& jz, float& drho_s) {
+    // pair sums and differences feed both the moments and the collision
+    float sp[9], df[9];
+    static_for<0, 9>([&](auto P) {
+        constexpr int p = decltype(P)::value, i = 2 * p + 1;
+        sp[p] = g[i] + g[i + 1];
+        df[p] = g[i] - g[i + 1];
+    });
+    // directions: 1:+x 3:+y 5:+z 7:(+x+y) 9:(+x-y) 11:(+x+z) 13:(+x-z) 15:(+y+z) 17:(+y-z)
+    float d = g[0];
+    static_for<0, 9>([&](auto P) { d += sp[decltype(P)::value]; });
+    drho_s = d;
+    jx = ((df[0] + df[3]) + (df[4] + df[5])) + df[6];
+    jy = ((df[1] + df[3]) - (df[4] - df[7])) + df[8];
+    jz = ((df[2] + df[5]) - (df[6] - df[7])) - df[8];
+    if constexpr (FORCE) { jx += K.hFx; jy += K.hFy; jz += K.hFz; }
+    rho = 1.0f + d * K.inv_s;
+    const float isr = K.inv_s / rho;  // 1 / (s rho)
+    const float q = 4.5f * isr;
+    const float jj = jx * jx + jy * jy + jz * jz;
+    const float t0 = d - 1.5f * isr * jj;
+    float uF = 0.f, irho = 0.f;
+    if constexpr (FORCE) {
+        irho = 1.0f / rho;
+        uF = (jx * K.Fx + jy * K.Fy + jz * K.Fz);  // = s rho (u . F)
+    }
+    // rest population
+    {
+        float r0 = K.one_m_omp * g[0] + K.kp_rest * t0;
+        if constexpr (FORCE) r0 -= K.gPw3[2] * uF * irho;
+        g[0] = r0;
+    }
+    static_for<0, 9>([&](auto P) {
+        constexpr int p = decltype(P)::value, i = 2 * p + 1, io = i + 1;
+        constexpr bool diag = p >= 3;
+        float pcu;  // s rho (c_i . u)
+        if constexpr (p == 0) pcu = jx;
+        else if constexpr (p == 1) pcu = jy;
+        else if constexpr (p == 2) pcu = jz;
+        else if constexpr (p == 3) pcu = jx + jy;
+        else if constexpr (p == 4) pcu = jx - jy;
+        else if constexpr (p == 5) pcu = jx + jz;
+        else if constexpr (p == 6) pcu = jx - jz;
+        else if constexpr (p == 7) pcu = jy + jz;
+        else pcu = jy - jz;
+        const float kp = diag ? K.kp_diag : K.kp_axis;
+        const float km = diag ? K.km_diag : K.km_axis;
+        const float e = (q * pcu) * pcu + t0;
+        float Pp = kp * e - K.a * sp[p];
+        float Mm = km * pcu - K.b * df[p];
+        if constexpr (FORCE) {
+            // c_i . F for this direction
+            float cF;
+            if constexpr (p == 0) cF = K.Fx;
+            else if constexpr (p == 1) cF = K.Fy;
+            else if constexpr (p == 2) cF = K.Fz;
+            else if constexpr (p == 3) cF = K.Fx + K.Fy;
+            else if constexpr (p == 4) cF = K.Fx - K.Fy;
+            else if constexpr (p == 5) cF = K.Fx + K.Fz;
+            else if constexpr (p == 6) cF = K.Fx - K.Fz;
+            else if constexpr (p == 7) cF = K.Fy + K.Fz;
+            else cF = K.Fy - K.Fz;
+            const int k = diag ? 1 : 0;
+            Pp += (K.gPw9[k] * cF * pcu - K.gPw3[k] * uF) * irho;
+            Mm += (diag ? K.sm_diag : K.sm_axis) * cF;
+        }
+        const float gi = g[i], gio = g[io];
+        g[i] = gi + Pp + Mm;
+        g[io] = gio + Pp - Mm;
+    });
+}
+
+// ---------------------------------------------------------------------------
+// Squirmer surface velocity at a link midpoint (lbm.cpp:100-104 minimum
+// image, squirmer.cpp:35-46 rigid-body motion + two-mode slip), FP64.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void link_velocity(const DevSquirmer& sq, int NX, int NY, int NZ, int wallsY, int wallsZ,
+                                              double px, double py, double pz, double rb[3], double ub[3]) {
+    rb[0] = px - sq.X[0]; rb[0] -= NX * round(rb[0] / NX);
+    rb[1] = py - sq.X[1]; if (!wallsY) rb[1] -= NY * round(rb[1] / NY);
+    rb[2] = pz - sq.X[2]; if (!wallsZ) rb[2] -= NZ * round(rb[2] / NZ);
+    ub[0] = sq.U[0] + (sq.W[1] * rb[2] - sq.W[2] * rb[1]);
+    ub[1] = sq.U[1] + (sq.W[2] * rb[0] - sq.W[0] * rb[2]);
+    ub[2] = sq.U[2] + (sq.W[0] * rb[1] - sq.W[1] * rb[0]);
+    const double rn = sqrt(rb[0] * rb[0] + rb[1] * rb[1] + rb[2] * rb[2]);
+    if (rn < 1e-12) return;
+    const double rh[3] = {rb[0] / rn, rb[1] / rn, rb[2] / rn};
+    const double ct = sq.e[0] * rh[0] + sq.e[1] * rh[1] + sq.e[2] * rh[2];
+    const double amp = sq.B1 * (1.0 + sq.beta * ct);
+    ub[0] += amp * (ct * rh[0] - sq.e[0]);
+    ub[1] += amp * (ct * rh[1] - sq.e[1]);
+    ub[2] += amp * (ct * rh[2] - sq.e[2]);
+}
+
+// Wavefront + block reduction of K doubles; thread 0 of the block gets the sums.
+template <int K>
+__device__ __forceinline__ void block_reduce(double (&v)[K], double* lds) {
+    static_for<0, K>([&](auto J) {
+        constexpr int j = decltype(J)::value;
+        double x = v[j];
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+        v[j] = x;
+    });
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0)
+        static_for<0, K>([&](auto J) { lds[wave * K + decltype(J)::value] = v[decltype(J)::value]; });
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        static_for<0, K>([&](auto J) {
+            constexpr int j = decltype(J)::value;
+            double s = lds[j];
+            for (int w = 1; w < BLOCK / 64; ++w) s += lds[w * K + j];
+            v[j] = s;
+        });
+    }
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// The kernel.
+// ---------------------------------------------------------------------------
+template <class S, bool ODD, int MODE>
+__global__ __launch_bounds__(BLOCK) void aa_step_kernel(const StepArgs<S> A, const FastCoef K) {
+    using St = Store<S>;
+    using T = typename St::C;
+    constexpr bool EXACT = std::is_same<S, double>::value;
+    constexpr bool MACRO = (MODE & STEP_MACRO) != 0;
+    constexpr bool WALLF = (MODE & STEP_WALLF) != 0;
+    constexpr bool MASS = (MODE & STEP_MASS) != 0;
+
+    // ---- tile: the hardware deals blocks to the 8 XCDs round robin; give
+    // each XCD a contiguous range of tiles so neighbouring rows share an L2.
+    const int G = A.gx * A.gy * A.gz;
+    const int lin = blockIdx.x;
+    const int q8 = G >> 3, r8 = G & 7, xcd = lin & 7;
+    const int tile = xcd * q8 + (xcd < r8 ? xcd : r8) + (lin >> 3);
+    const int tgx = tile % A.gx, trest = tile / A.gx;
+    const int tgy = trest % A.gy, tgz = trest / A.gy;
+    const int lx = threadIdx.x & ((1 << A.tx_shift) - 1), ly = threadIdx.x >> A.tx_shift;
+    const int x = (tgx << A.tx_shift) + lx, y = tgy * A.ty + ly, z = tgz;
+
+    const uint32_t NX = (uint32_t)A.NX, NY = (uint32_t)A.NY, NZ = (uint32_t)A.NZ, N = A.N;
+    const uint32_t NXY = NX * NY;
+    const bool inb = (uint32_t)x < NX && (uint32_t)y < NY;
+    const uint32_t n = inb ? (uint32_t)x + NX * ((uint32_t)y + NY * (uint32_t)z) : 0u;
+    const uint8_t fl = inb ? A.flag[n] : (uint8_t)WALL;
+    const bool active = (fl & TYPE_MASK) == FLUID;
+    const bool bnd = active && (fl & BOUNDARY);
+
+    if (tile == 0 && threadIdx.x == 0) {
+        // clear the other parity's accumulators for step t+1
+        const int o = (int)((A.t + 1) & 1);
+        for (int k = 0; k < 9; ++k) A.st->acc[o][k] = 0.0;
+        A.st->shift[o] = 0.0;
+        A.st->remap_dmass = 0.0;
+    }
+
+    // periodic neighbour coordinates (walls are solid node layers)
+    const uint32_t ux_ = (uint32_t)x, uy_ = (uint32_t)y, uz_ = (uint32_t)z;
+    const uint32_t xo[3] = {ux_ == 0 ? NX - 1 : ux_ - 1, ux_, ux_ + 1 == NX ? 0u : ux_ + 1};
+    const uint32_t yo[3] = {(uy_ == 0 ? NY - 1 : uy_ - 1) * NX, uy_ * NX, (uy_ + 1 == NY ? 0u : uy_ + 1) * NX};
+    const uint32_t zo[3] = {(uz_ == 0 ? NZ - 1 : uz_ - 1) * NXY, uz_ * NXY, (uz_ + 1 == NZ ? 0u : uz_ + 1) * NXY};
+
+    // solid / particle neighbour masks (bit i: n + c_i)
+    uint32_t solid = 0, part = 0;
+    if (bnd) {
+        static_for<1, Q>([&](auto I) {
+            constexpr int i = decltype(I)::value;
+            const uint32_t m = xo[1 + CX[i]] + yo[1 + CY[i]] + zo[1 + CZ[i]];
+            const uint8_t tt = A.flag[m] & TYPE_MASK;
+            solid |= (uint32_t)(tt != FLUID) << i;
+            part |= (uint32_t)(tt == PARTICLE) << i;
+        });
+    }
+
+    double facc[9];
+    for (int k = 0; k < 9; ++k) facc[k] = 0.0;
+    double excess = 0.0;
+
+    if (active) {
+        S* __restrict__ f = A.f;
+        // ---- gather the 19 populations node n collides ----
+        T g[Q];
+        static_for<0, Q>([&](auto I) {
+            constexpr int i = decltype(I)::value;
+            S raw;
+            if constexpr (!ODD || i == 0) {
+                raw = f[(size_t)i * N + n];
+            } else {
+                const uint32_t up = xo[1 - CX[i]] + yo[1 - CY[i]] + zo[1 - CZ[i]];
+                size_t e = (size_t)OPP[i] * N + up;
+                if ((solid >> OPP[i]) & 1u) e = (size_t)i * N + n;
+                raw = f[e];
+            }
+            g[i] = St::load(raw);
+        });
+
+        // pending mass-correction shift (lbm.cpp:337), applied on load
+        const double dsh = A.st->shift[A.t & 1];
+        if (dsh != 0.0) {
+            static_for<0, Q>([&](auto I) {
+                constexpr int i = decltype(I)::value;
+                if constexpr (EXACT) g[i] = g[i] + W[i] * dsh;
+                else g[i] += (T)(W[i] * dsh * St::scale);
+            });
+        }
+
+        // ---- collide ----
+        double rho_d;
+        T rho_t, m0, m1, m2;  // macroscopic outputs (real units)
+        if constexpr (EXACT) {
+            double r, u, v, wz;
+            collide_exact(g, A.omPlus, A.omMinus, A.Fx, A.Fy, A.Fz, r, u, v, wz);
+            rho_d = r; rho_t = r; m0 = u; m1 = v; m2 = wz;
+            if constexpr (MASS) {
+                double s = 0.0;
+                static_for<0, Q>([&](auto I) { s += g[decltype(I)::value]; });
+                excess = s - 1.0;
+            }
+        } else {
+            float rho, jx, jy, jz, drs;
+            if (A.Fx != 0.0 || A.Fy != 0.0 || A.Fz != 0.0) collide_fast<true>(g, K, rho, jx, jy, jz, drs);
+            else collide_fast<false>(g, K, rho, jx, jy, jz, drs);
+            rho_d = rho; rho_t = rho;
+            const float isr = K.inv_s / rho;
+            m0 = jx * isr; m1 = jy * isr; m2 = jz * isr;
+            if constexpr (MASS) excess = (double)drs * (double)K.inv_s;
+        }
+
+        // ---- macroscopic fields: all fluid nodes on request, else the
+        // band around the squirmer that the remap reads (lbm.cpp:152, 307)
+        if constexpr (MACRO) {
+            static_cast<T*>(A.rho)[n] = rho_t;
+            static_cast<T*>(A.ux)[n] = m0;
+            static_cast<T*>(A.uy)[n] = m1;
+            static_cast<T*>(A.uz)[n] = m2;
+        } else {
+            if (A.has_squirmer) {
+                const int* b = A.st->band;
+                int dx = x - b[0]; dx += dx < 0 ? (int)NX : 0;
+                int dy = y - b[1]; dy += dy < 0 ? (int)NY : 0;
+                int dz = z - b[2]; dz += dz < 0 ? (int)NZ : 0;
+                if (dx < b[3] && dy < b[4] && dz < b[5]) static_cast<T*>(A.rho)[n] = rho_t;
+            }
+        }
+
+        // ---- moving-boundary bounce-back on particle links (FP64) ----
+        if (bnd && part) {
+            const DevSquirmer& sq = A.st->sq;
+            static_for<1, Q>([&](auto I) {
+                constexpr int i = decltype(I)::value;
+                if ((part >> i) & 1u) {
+                    double rb[3], ub[3];
+                    link_velocity(sq, A.NX, A.NY, A.NZ, A.walls_y, A.walls_z, x + 0.5 * CX[i], y + 0.5 * CY[i],
+                                  z + 0.5 * CZ[i], rb, ub);
+                    double fpost;
+                    if constexpr (EXACT) fpost = g[i];
+                    else fpost = W[i] + (double)g[i] * (1.0 / St::scale);
+                    const double cub = CX[i] * ub[0] + CY[i] * ub[1] + CZ[i] * ub[2];
+                    const double fb = fpost - 6.0 * W[i] * rho_d * cub;
+                    const double s = fpost + fb, dl = fpost - fb;
+                    const double dpx = s * CX[i] - dl * ub[0];
+                    const double dpy = s * CY[i] - dl * ub[1];
+                    const double dpz = s * CZ[i] - dl * ub[2];
+                    facc[3] += dpx; facc[4] += dpy; facc[5] += dpz;
+                    facc[6] += rb[1] * dpz - rb[2] * dpy;
+                    facc[7] += rb[2] * dpx - rb[0] * dpz;
+                    facc[8] += rb[0] * dpy - rb[1] * dpx;
+                    if constexpr (MASS) excess += fb - fpost;
+                    if constexpr (EXACT) g[i] = fb;
+                    else g[i] = (T)((fb - W[i]) * St::scale);
+                }
+            });
+        }
+        if constexpr (WALLF) {
+            if (bnd) {
+                const uint32_t wl = solid & ~part;
+                static_for<1, Q>([&](auto I) {
+                    constexpr int i = decltype(I)::value;
+                    if ((wl >> i) & 1u) {
+                        double fpost;
+                        if constexpr (EXACT) fpost = g[i];
+                        else fpost = W[i] + (double)g[i] * (1.0 / St::scale);
+                        const double s = fpost + fpost;  // u_b = 0: f_b = f*
+                        if constexpr (CX[i] != 0) facc[0] += s * CX[i];
+                        if constexpr (CY[i] != 0) facc[1] += s * CY[i];
+                        if constexpr (CZ[i] != 0) facc[2] += s * CZ[i];
+                    }
+                });
+            }
+        }
+
+        // ---- stream ----
+        static_for<0, Q>([&](auto I) {
+            constexpr int i = decltype(I)::value;
+            const S val = St::store(g[i]);
+            if constexpr (!ODD || i == 0) {
+                f[(size_t)OPP[i] * N + n] = val;
+            } else {
+                const uint32_t dn = xo[1 + CX[i]] + yo[1 + CY[i]] + zo[1 + CZ[i]];
+                size_t e = (size_t)i * N + dn;
+                if ((solid >> i) & 1u) e = (size_t)OPP[i] * N + n;
+                f[e] = val;
+            }
+        });
+    }
+
+    // ---- block reductions ----
+    __shared__ double lds[(BLOCK / 64) * 9];
+    const bool links = (part != 0u) || (WALLF && solid != 0u);
+    if (__syncthreads_or(links)) {
+        block_reduce<9>(facc, lds);
+        if (threadIdx.x == 0) {
+            double* acc = A.st->acc[A.t & 1];
+            for (int k = 0; k < 9; ++k)
+                if (facc[k] != 0.0) atomicAdd(&acc[k], facc[k]);
+        }
+    }
+    if constexpr (MASS) {
+        double m[1] = {excess};
+        block_reduce<1>(m, lds);
+        if (threadIdx.x == 0) A.mass_part[tile] = m[0];
+    }
+}
+
+}  // namespace lbmhip
