@@ -57,6 +57,7 @@ struct StepArgs {
     double* mass_part;    // one excess-mass partial per block
     int NX, NY, NZ;
     uint32_t N;
+    uint32_t SS;          // slot stride: population i of node n is f[i * SS + n]
     int walls_y, walls_z, has_squirmer;
     int gx, gy, gz;       // grid in tiles
     int tx_shift;         // log2 of the tile width in x

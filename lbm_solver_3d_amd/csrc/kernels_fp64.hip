@@ -13,53 +13,53 @@ hipError_t launch_squirmer(const RemapArgs<S>& a, hipStream_t s) {
 }
 
 template <class S>
-hipError_t launch_gather(const S* f, const uint8_t* flag, double* out, int NX, int NY, int NZ, uint32_t N, int shifted,
-                         const DevState* st, int shift_slot, hipStream_t s) {
-    hipLaunchKernelGGL((gather_kernel<S>), dim3((N + 255) / 256), dim3(256), 0, s, f, flag, out, NX, NY, NZ, N,
+hipError_t launch_gather(const S* f, const uint8_t* flag, double* out, int NX, int NY, int NZ, uint32_t N,
+                         uint32_t SS, int shifted, const DevState* st, int shift_slot, hipStream_t s) {
+    hipLaunchKernelGGL((gather_kernel<S>), dim3((N + 255) / 256), dim3(256), 0, s, f, flag, out, NX, NY, NZ, N, SS,
                        shifted, st, shift_slot);
     return hipGetLastError();
 }
 
 template <class S>
-hipError_t launch_scatter(S* f, const uint8_t* flag, const double* in, int NX, int NY, int NZ, uint32_t N, int shifted,
-                          hipStream_t s) {
-    hipLaunchKernelGGL((scatter_kernel<S>), dim3((N + 255) / 256), dim3(256), 0, s, f, flag, in, NX, NY, NZ, N,
+hipError_t launch_scatter(S* f, const uint8_t* flag, const double* in, int NX, int NY, int NZ, uint32_t N,
+                          uint32_t SS, int shifted, hipStream_t s) {
+    hipLaunchKernelGGL((scatter_kernel<S>), dim3((N + 255) / 256), dim3(256), 0, s, f, flag, in, NX, NY, NZ, N, SS,
                        shifted);
     return hipGetLastError();
 }
 
 template <class S>
-hipError_t launch_mass(const S* f, const uint8_t* flag, double* part, int NX, int NY, int NZ, uint32_t N, int shifted,
-                       const DevState* st, int shift_slot, hipStream_t s) {
-    hipLaunchKernelGGL((mass_kernel<S>), dim3((N + 255) / 256), dim3(256), 0, s, f, flag, part, NX, NY, NZ, N, shifted,
-                       st, shift_slot);
+hipError_t launch_mass(const S* f, const uint8_t* flag, double* part, int NX, int NY, int NZ, uint32_t N,
+                       uint32_t SS, int shifted, const DevState* st, int shift_slot, hipStream_t s) {
+    hipLaunchKernelGGL((mass_kernel<S>), dim3((N + 255) / 256), dim3(256), 0, s, f, flag, part, NX, NY, NZ, N, SS,
+                       shifted, st, shift_slot);
     return hipGetLastError();
 }
 
-// f_i = w_i everywhere (lbm.cpp:82-84), in storage format S.
+// f_i = w_i everywhere (lbm.cpp:82-84), in storage format S (slot padding too).
 template <class S>
-__global__ __launch_bounds__(256) void fill_populations_kernel(S* f, uint32_t N) {
+__global__ __launch_bounds__(256) void fill_populations_kernel(S* f, uint32_t SS) {
     const size_t e = (size_t)blockIdx.x * 256u + threadIdx.x;
-    if (e >= (size_t)Q * N) return;
-    const int i = (int)(e / N);
+    if (e >= (size_t)Q * SS) return;
+    const int i = (int)(e / SS);
     f[e] = from_real<S>(W[i], i);
 }
 
 template <class S>
-hipError_t launch_fill_populations(S* f, uint32_t N, hipStream_t s) {
-    const size_t tot = (size_t)Q * N;
-    hipLaunchKernelGGL((fill_populations_kernel<S>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, f, N);
+hipError_t launch_fill_populations(S* f, uint32_t SS, hipStream_t s) {
+    const size_t tot = (size_t)Q * SS;
+    hipLaunchKernelGGL((fill_populations_kernel<S>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, f, SS);
     return hipGetLastError();
 }
 
 #define LBMHIP_INSTANTIATE(S)                                                                                     \
     template hipError_t launch_squirmer<S>(const RemapArgs<S>&, hipStream_t);                                    \
-    template hipError_t launch_gather<S>(const S*, const uint8_t*, double*, int, int, int, uint32_t, int,        \
-                                         const DevState*, int, hipStream_t);                                     \
-    template hipError_t launch_scatter<S>(S*, const uint8_t*, const double*, int, int, int, uint32_t, int,       \
-                                          hipStream_t);                                                          \
-    template hipError_t launch_mass<S>(const S*, const uint8_t*, double*, int, int, int, uint32_t, int,          \
-                                       const DevState*, int, hipStream_t);                                       \
+    template hipError_t launch_gather<S>(const S*, const uint8_t*, double*, int, int, int, uint32_t, uint32_t,   \
+                                         int, const DevState*, int, hipStream_t);                                \
+    template hipError_t launch_scatter<S>(S*, const uint8_t*, const double*, int, int, int, uint32_t, uint32_t,  \
+                                          int, hipStream_t);                                                     \
+    template hipError_t launch_mass<S>(const S*, const uint8_t*, double*, int, int, int, uint32_t, uint32_t,     \
+                                       int, const DevState*, int, hipStream_t);                                  \
     template hipError_t launch_fill_populations<S>(S*, uint32_t, hipStream_t);
 
 LBMHIP_INSTANTIATE(double)

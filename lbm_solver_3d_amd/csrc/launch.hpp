@@ -20,18 +20,18 @@ hipError_t launch_squirmer(const RemapArgs<S>& a, hipStream_t s);
 
 template <class S>
 hipError_t launch_gather(const S* f, const uint8_t* flag, double* out, int NX, int NY, int NZ, uint32_t N,
-                         int shifted, const DevState* st, int shift_slot, hipStream_t s);
+                         uint32_t SS, int shifted, const DevState* st, int shift_slot, hipStream_t s);
 
 template <class S>
 hipError_t launch_scatter(S* f, const uint8_t* flag, const double* in, int NX, int NY, int NZ, uint32_t N,
-                          int shifted, hipStream_t s);
+                          uint32_t SS, int shifted, hipStream_t s);
 
 template <class S>
 hipError_t launch_mass(const S* f, const uint8_t* flag, double* part, int NX, int NY, int NZ, uint32_t N,
-                       int shifted, const DevState* st, int shift_slot, hipStream_t s);
+                       uint32_t SS, int shifted, const DevState* st, int shift_slot, hipStream_t s);
 
 template <class S>
-hipError_t launch_fill_populations(S* f, uint32_t N, hipStream_t s);
+hipError_t launch_fill_populations(S* f, uint32_t SS, hipStream_t s);
 
 // Storage-independent kernels (kernels_fp64.hip).
 hipError_t launch_mass_finalize(const double* part, int nparts, DevState* st, int use_remap_dmass, int shift_slot,
@@ -44,12 +44,12 @@ hipError_t launch_macro_to_double(const void* src, int is_double, double* dst, u
 #define LBMHIP_DECLARE(S)                                                                                          \
     extern template hipError_t launch_step<S>(const StepArgs<S>&, const FastCoef&, bool, int, hipStream_t);     \
     extern template hipError_t launch_squirmer<S>(const RemapArgs<S>&, hipStream_t);                            \
-    extern template hipError_t launch_gather<S>(const S*, const uint8_t*, double*, int, int, int, uint32_t, int, \
-                                                const DevState*, int, hipStream_t);                             \
-    extern template hipError_t launch_scatter<S>(S*, const uint8_t*, const double*, int, int, int, uint32_t, int, \
-                                                 hipStream_t);                                                  \
-    extern template hipError_t launch_mass<S>(const S*, const uint8_t*, double*, int, int, int, uint32_t, int,   \
-                                              const DevState*, int, hipStream_t);                               \
+    extern template hipError_t launch_gather<S>(const S*, const uint8_t*, double*, int, int, int, uint32_t,      \
+                                                uint32_t, int, const DevState*, int, hipStream_t);              \
+    extern template hipError_t launch_scatter<S>(S*, const uint8_t*, const double*, int, int, int, uint32_t,     \
+                                                 uint32_t, int, hipStream_t);                                   \
+    extern template hipError_t launch_mass<S>(const S*, const uint8_t*, double*, int, int, int, uint32_t,        \
+                                              uint32_t, int, const DevState*, int, hipStream_t);                \
     extern template hipError_t launch_fill_populations<S>(S*, uint32_t, hipStream_t);
 
 LBMHIP_DECLARE(double)

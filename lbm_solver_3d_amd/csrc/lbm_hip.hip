@@ -7,6 +7,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -47,6 +48,7 @@ struct lbm_hip_ctx {
     hipStream_t stream = nullptr;
     int NX = 0, NY = 0, NZ = 0;
     uint32_t N = 0;
+    uint32_t SS = 0;  // slot stride in elements (N plus padding)
     int prec = 0;
     size_t ssize = 8, msize = 8;
     void* f = nullptr;
@@ -103,7 +105,7 @@ int lbm_hip_validate(const lbm_hip_params* p) {
     if (p->tau <= 0.5) return bad("tau must be > 0.5");
     if (p->lambda <= 0.0) return bad("lambda must be > 0");
     if (p->precision < LBM_HIP_FP64 || p->precision > LBM_HIP_BF16) return bad("unknown precision");
-    const double N = (double)p->nx * p->ny * p->nz;
+    const double N = (double)p->nx * p->ny * p->nz + 65536.0;
     if (N * 19.0 >= 4294967296.0) return bad("grid too large for 32-bit population indexing (19*N must be < 2^32)");
     if (!p->squirmer) return LBM_HIP_OK;
     if (p->radius <= 1.0) return bad("squirmer radius must be > 1 lattice unit");
@@ -184,7 +186,7 @@ StepArgs<S> step_args(const lbm_hip_ctx* c) {
     a.f = (S*)c->f; a.flag = c->flag;
     a.rho = c->rho; a.ux = c->ux; a.uy = c->uy; a.uz = c->uz;
     a.st = c->st; a.mass_part = c->mass_part;
-    a.NX = c->NX; a.NY = c->NY; a.NZ = c->NZ; a.N = c->N;
+    a.NX = c->NX; a.NY = c->NY; a.NZ = c->NZ; a.N = c->N; a.SS = c->SS;
     a.walls_y = c->p.walls_y; a.walls_z = c->p.walls_z; a.has_squirmer = c->has_squirmer;
     a.gx = c->gx; a.gy = c->gy; a.gz = c->gz; a.tx_shift = c->tx_shift; a.ty = c->ty;
     a.omPlus = c->omPlus; a.omMinus = c->omMinus;
@@ -199,12 +201,22 @@ RemapArgs<S> remap_args(const lbm_hip_ctx* c) {
     a.f = (S*)c->f; a.flag = c->flag;
     a.rho = c->rho; a.ux = c->ux; a.uy = c->uy; a.uz = c->uz;
     a.st = c->st;
-    a.NX = c->NX; a.NY = c->NY; a.NZ = c->NZ; a.N = c->N;
+    a.NX = c->NX; a.NY = c->NY; a.NZ = c->NZ; a.N = c->N; a.SS = c->SS;
     a.walls_y = c->p.walls_y; a.walls_z = c->p.walls_z;
     a.shifted = (c->t > 0 && ((c->t - 1) & 1) == 0) ? 1 : 0;
     a.acc_slot = (int)((c->t - 1) & 1);
     a.shift_slot = (int)(c->t & 1);
     return a;
+}
+
+// Slot stride: N rounded up to 256 bytes, plus a pad so that the 19
+// population streams do not sit at the same offset modulo large powers of
+// two (LBM_HIP_SLOT_PAD overrides the pad, in bytes).
+uint32_t slot_stride(uint32_t N, size_t ssize) {
+    size_t bytes = ((size_t)N * ssize + 255) / 256 * 256;
+    size_t pad = 33 * 256;
+    if (const char* e = std::getenv("LBM_HIP_SLOT_PAD")) pad = (size_t)std::strtoull(e, nullptr, 10) / 256 * 256;
+    return (uint32_t)((bytes + pad) / ssize);
 }
 
 bool layout_shifted(const lbm_hip_ctx* c) { return c->t > 0 && ((c->t - 1) & 1) == 0; }
@@ -252,6 +264,7 @@ int lbm_hip_create(const lbm_hip_params* pin, lbm_hip_ctx** out) {
     c->NX = pin->nx; c->NY = pin->ny; c->NZ = pin->nz;
     c->N = (uint32_t)((size_t)pin->nx * pin->ny * pin->nz);
     c->ssize = storage_bytes(c->prec);
+    c->SS = slot_stride(c->N, c->ssize);
     c->msize = c->prec == LBM_HIP_FP64 ? 8 : 4;
     // lbm.cpp:67-69
     c->omPlus = 1.0 / pin->tau;
@@ -286,7 +299,7 @@ int lbm_hip_create(const lbm_hip_params* pin, lbm_hip_ctx** out) {
         return true;
     };
     const int nparts = c->G > c->nb256 ? c->G : c->nb256;
-    if (!dmalloc(&c->f, (size_t)Q * N * c->ssize) || !dmalloc((void**)&c->flag, N) ||
+    if (!dmalloc(&c->f, (size_t)Q * c->SS * c->ssize) || !dmalloc((void**)&c->flag, N) ||
         !dmalloc(&c->rho, N * c->msize) || !dmalloc(&c->ux, N * c->msize) || !dmalloc(&c->uy, N * c->msize) ||
         !dmalloc(&c->uz, N * c->msize) || !dmalloc((void**)&c->st, sizeof(DevState)) ||
         !dmalloc((void**)&c->mass_part, sizeof(double) * nparts) ||
@@ -360,7 +373,7 @@ int lbm_hip_create(const lbm_hip_params* pin, lbm_hip_ctx** out) {
         launch_fill_macro(c->rho, c->ux, c->uy, c->uz, c->prec == LBM_HIP_FP64, c->N, c->stream) != hipSuccess ||
         dispatch(c->prec, [&](auto* tag) -> int {
             using S = std::remove_pointer_t<decltype(tag)>;
-            return launch_fill_populations<S>((S*)c->f, c->N, c->stream) == hipSuccess ? 0 : -1;
+            return launch_fill_populations<S>((S*)c->f, c->SS, c->stream) == hipSuccess ? 0 : -1;
         }) != 0 ||
         hipMemsetAsync(c->mass_part, 0, sizeof(double) * nparts, c->stream) != hipSuccess ||
         hipStreamSynchronize(c->stream) != hipSuccess)
@@ -416,8 +429,8 @@ int lbm_hip_correct_mass(lbm_hip_ctx* c) {
         e = hipSuccess;
         dispatch(c->prec, [&](auto* tag) -> int {
             using S = std::remove_pointer_t<decltype(tag)>;
-            e = launch_mass<S>((const S*)c->f, c->flag, c->mass_part, c->NX, c->NY, c->NZ, c->N, shifted, c->st,
-                               shift_slot, c->stream);
+            e = launch_mass<S>((const S*)c->f, c->flag, c->mass_part, c->NX, c->NY, c->NZ, c->N, c->SS, shifted,
+                               c->st, shift_slot, c->stream);
             return 0;
         });
         if (e == hipSuccess) e = launch_mass_finalize(c->mass_part, c->nb256, c->st, 0, shift_slot, c->stream);
@@ -534,7 +547,7 @@ int lbm_hip_download(lbm_hip_ctx* c, double* f, double* rho, double* ux, double*
         const int shifted = layout_shifted(c) ? 1 : 0;
         dispatch(c->prec, [&](auto* tag) -> int {
             using S = std::remove_pointer_t<decltype(tag)>;
-            e = launch_gather<S>((const S*)c->f, c->flag, tmp, c->NX, c->NY, c->NZ, c->N, shifted, c->st,
+            e = launch_gather<S>((const S*)c->f, c->flag, tmp, c->NX, c->NY, c->NZ, c->N, c->SS, shifted, c->st,
                                  (int)(c->t & 1), c->stream);
             return 0;
         });
@@ -580,7 +593,7 @@ int lbm_hip_upload_populations(lbm_hip_ctx* c, const double* f) {
     if (e == hipSuccess)
         dispatch(c->prec, [&](auto* tag) -> int {
             using S = std::remove_pointer_t<decltype(tag)>;
-            e = launch_scatter<S>((S*)c->f, c->flag, tmp, c->NX, c->NY, c->NZ, c->N, shifted, c->stream);
+            e = launch_scatter<S>((S*)c->f, c->flag, tmp, c->NX, c->NY, c->NZ, c->N, c->SS, shifted, c->stream);
             return 0;
         });
     const double zero = 0.0;

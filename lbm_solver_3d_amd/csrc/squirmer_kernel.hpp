@@ -43,6 +43,7 @@ struct RemapArgs {
     DevState* st;
     int NX, NY, NZ;
     uint32_t N;
+    uint32_t SS;     // slot stride (elements between populations i and i+1)
     int walls_y, walls_z;
     int shifted;     // layout after the last step: 1 after an even step
     int acc_slot;    // accumulator slot of the last step
@@ -98,7 +99,7 @@ __global__ __launch_bounds__(SQ_THREADS) void squirmer_kernel(const RemapArgs<S>
     DevSquirmer& sq = st->sq;
     const int tid = threadIdx.x;
     const int NX = A.NX, NY = A.NY, NZ = A.NZ;
-    const uint32_t N = A.N;
+    const uint32_t SS = A.SS;
 
     // ---------------- A: rigid body (lbm.cpp:257-262) ----------------
     if (tid == 0) {
@@ -233,7 +234,7 @@ __global__ __launch_bounds__(SQ_THREADS) void squirmer_kernel(const RemapArgs<S>
                 // population f_i(n) after the last step (old flags), lbm.cpp:295-298
                 double fv = 0.0;
                 if (i < Q) {
-                    size_t slot = (size_t)i * N + n;
+                    size_t slot = (size_t)i * SS + n;
                     if (A.shifted && i > 0) {
                         const uint32_t m = (uint32_t)wrapi(x - cxi, NX) +
                                            (uint32_t)NX * ((uint32_t)wrapi(y - cyi, NY) +
@@ -241,7 +242,7 @@ __global__ __launch_bounds__(SQ_THREADS) void squirmer_kernel(const RemapArgs<S>
                         const uint8_t fm = A.flag[m];
                         uint8_t tm = fm & TYPE_MASK;
                         if (fm & FLIPPED) tm = (tm == FLUID) ? PARTICLE : FLUID;
-                        if (tm == FLUID) slot = (size_t)OPP[i] * N + m;
+                        if (tm == FLUID) slot = (size_t)OPP[i] * SS + m;
                     }
                     fv = to_real<S>(A.f[slot], i);
                     if (dsh != 0.0) fv += W[i] * dsh;
@@ -314,7 +315,7 @@ __global__ __launch_bounds__(SQ_THREADS) void squirmer_kernel(const RemapArgs<S>
         const uint32_t n = e & 0x7FFFFFFFu;
         const bool cover = (e >> 31) != 0;
         if (!A.shifted || i == 0) {
-            if (!cover) f[(size_t)i * N + n] = from_real<S>(equilibrium(i, lrr[k], lub[k][0], lub[k][1], lub[k][2]), i);
+            if (!cover) f[(size_t)i * SS + n] = from_real<S>(equilibrium(i, lrr[k], lub[k][0], lub[k][1], lub[k][2]), i);
             continue;
         }
         const int x = (int)(n % (uint32_t)NX), y = (int)((n / (uint32_t)NX) % (uint32_t)NY),
@@ -323,7 +324,7 @@ __global__ __launch_bounds__(SQ_THREADS) void squirmer_kernel(const RemapArgs<S>
                            (uint32_t)NX * ((uint32_t)wrapi(y - CY[i], NY) + (uint32_t)NY * (uint32_t)wrapi(z - CZ[i], NZ));
         const uint8_t fm = A.flag[m];
         const bool mfluid = (fm & TYPE_MASK) == FLUID, mflip = (fm & FLIPPED) != 0;
-        const size_t sin_ = (size_t)i * N + n, sout = (size_t)OPP[i] * N + m;
+        const size_t sin_ = (size_t)i * SS + n, sout = (size_t)OPP[i] * SS + m;
         if (!cover) {
             const S fe = from_real<S>(equilibrium(i, lrr[k], lub[k][0], lub[k][1], lub[k][2]), i);
             if (mfluid && !mflip) { f[sin_] = f[sout]; f[sout] = fe; }
@@ -381,7 +382,7 @@ __global__ __launch_bounds__(SQ_THREADS) void squirmer_kernel(const RemapArgs<S>
 template <class S>
 __global__ __launch_bounds__(256) void gather_kernel(const S* __restrict__ f, const uint8_t* __restrict__ flag,
                                                      double* __restrict__ out, int NX, int NY, int NZ, uint32_t N,
-                                                     int shifted, const DevState* st, int shift_slot) {
+                                                     uint32_t SS, int shifted, const DevState* st, int shift_slot) {
     const uint32_t n = blockIdx.x * 256u + threadIdx.x;
     if (n >= N) return;
     const int x = (int)(n % (uint32_t)NX), y = (int)((n / (uint32_t)NX) % (uint32_t)NY),
@@ -389,11 +390,11 @@ __global__ __launch_bounds__(256) void gather_kernel(const S* __restrict__ f, co
     const bool fluid = (flag[n] & TYPE_MASK) == FLUID;
     const double dsh = st->shift[shift_slot];
     for (int i = 0; i < Q; ++i) {
-        size_t slot = (size_t)i * N + n;
+        size_t slot = (size_t)i * SS + n;
         if (fluid && shifted && i > 0) {
             const uint32_t m = (uint32_t)wrapi(x - CX[i], NX) +
                                (uint32_t)NX * ((uint32_t)wrapi(y - CY[i], NY) + (uint32_t)NY * (uint32_t)wrapi(z - CZ[i], NZ));
-            if ((flag[m] & TYPE_MASK) == FLUID) slot = (size_t)OPP[i] * N + m;
+            if ((flag[m] & TYPE_MASK) == FLUID) slot = (size_t)OPP[i] * SS + m;
         }
         double v = to_real<S>(f[slot], i);
         if (fluid && dsh != 0.0) v += W[i] * dsh;
@@ -404,18 +405,18 @@ __global__ __launch_bounds__(256) void gather_kernel(const S* __restrict__ f, co
 template <class S>
 __global__ __launch_bounds__(256) void scatter_kernel(S* __restrict__ f, const uint8_t* __restrict__ flag,
                                                       const double* __restrict__ in, int NX, int NY, int NZ, uint32_t N,
-                                                      int shifted) {
+                                                      uint32_t SS, int shifted) {
     const uint32_t n = blockIdx.x * 256u + threadIdx.x;
     if (n >= N) return;
     const int x = (int)(n % (uint32_t)NX), y = (int)((n / (uint32_t)NX) % (uint32_t)NY),
               z = (int)(n / ((uint32_t)NX * (uint32_t)NY));
     const bool fluid = (flag[n] & TYPE_MASK) == FLUID;
     for (int i = 0; i < Q; ++i) {
-        size_t slot = (size_t)i * N + n;
+        size_t slot = (size_t)i * SS + n;
         if (fluid && shifted && i > 0) {
             const uint32_t m = (uint32_t)wrapi(x - CX[i], NX) +
                                (uint32_t)NX * ((uint32_t)wrapi(y - CY[i], NY) + (uint32_t)NY * (uint32_t)wrapi(z - CZ[i], NZ));
-            if ((flag[m] & TYPE_MASK) == FLUID) slot = (size_t)OPP[i] * N + m;
+            if ((flag[m] & TYPE_MASK) == FLUID) slot = (size_t)OPP[i] * SS + m;
         }
         f[slot] = from_real<S>(in[(size_t)i * N + n], i);
     }
@@ -426,7 +427,7 @@ __global__ __launch_bounds__(256) void scatter_kernel(S* __restrict__ f, const u
 template <class S>
 __global__ __launch_bounds__(256) void mass_kernel(const S* __restrict__ f, const uint8_t* __restrict__ flag,
                                                    double* __restrict__ part, int NX, int NY, int NZ, uint32_t N,
-                                                   int shifted, const DevState* st, int shift_slot) {
+                                                   uint32_t SS, int shifted, const DevState* st, int shift_slot) {
     __shared__ double lds[4];
     const uint32_t n = blockIdx.x * 256u + threadIdx.x;
     double ex[1] = {0.0};
@@ -436,11 +437,11 @@ __global__ __launch_bounds__(256) void mass_kernel(const S* __restrict__ f, cons
         const double dsh = st->shift[shift_slot];
         double s = 0.0;
         for (int i = 0; i < Q; ++i) {
-            size_t slot = (size_t)i * N + n;
+            size_t slot = (size_t)i * SS + n;
             if (shifted && i > 0) {
                 const uint32_t m = (uint32_t)wrapi(x - CX[i], NX) +
                                    (uint32_t)NX * ((uint32_t)wrapi(y - CY[i], NY) + (uint32_t)NY * (uint32_t)wrapi(z - CZ[i], NZ));
-                if ((flag[m] & TYPE_MASK) == FLUID) slot = (size_t)OPP[i] * N + m;
+                if ((flag[m] & TYPE_MASK) == FLUID) slot = (size_t)OPP[i] * SS + m;
             }
             double v = to_real<S>(f[slot], i);
             if (dsh != 0.0) v += W[i] * dsh;

@@ -218,6 +218,32 @@ __device__ __forceinline__ void link_velocity(const DevSquirmer& sq, int NX, int
     ub[2] += amp * (ct * rh[2] - sq.e[2]);
 }
 
+// Moving-boundary bounce-back of one particle link (lbm.cpp:189-215), FP64:
+// f_b and the link's momentum exchange and torque.  Kept out of line:
+// particle links are a thin shell of nodes.
+struct LinkResult {
+    double fb, dp[3], tq[3];
+};
+__device__ __noinline__ LinkResult particle_link(const DevSquirmer& sq, int NX, int NY, int NZ, int wallsY,
+                                                 int wallsZ, int x, int y, int z, int cx, int cy, int cz, double w6,
+                                                 double fpost, double rho) {
+    double rb[3], ub[3];
+    link_velocity(sq, NX, NY, NZ, wallsY, wallsZ, x + 0.5 * cx, y + 0.5 * cy, z + 0.5 * cz, rb, ub);
+    const double cub = cx * ub[0] + cy * ub[1] + cz * ub[2];
+    const double fb = fpost - w6 * rho * cub;
+    const double s = fpost + fb, dl = fpost - fb;
+    const double dpx = s * cx - dl * ub[0];
+    const double dpy = s * cy - dl * ub[1];
+    const double dpz = s * cz - dl * ub[2];
+    LinkResult o;
+    o.fb = fb;
+    o.dp[0] = dpx; o.dp[1] = dpy; o.dp[2] = dpz;
+    o.tq[0] = rb[1] * dpz - rb[2] * dpy;
+    o.tq[1] = rb[2] * dpx - rb[0] * dpz;
+    o.tq[2] = rb[0] * dpy - rb[1] * dpx;
+    return o;
+}
+
 // Wavefront + block reduction of K doubles; thread 0 of the block gets the sums.
 template <int K>
 __device__ __forceinline__ void block_reduce(double (&v)[K], double* lds) {
@@ -244,6 +270,14 @@ __device__ __forceinline__ void block_reduce(double (&v)[K], double* lds) {
 
 // ---------------------------------------------------------------------------
 // The kernel.
+//
+// Addressing: a wavefront is 64 consecutive x of one (y, z) row (tiles are
+// at least 64 wide), so the row offsets of the 9 (dy, dz) neighbour rows are
+// wavefront-uniform and live in scalar registers together with the 19 slot
+// bases; a lane only holds its three x offsets (x-1, x, x+1, periodic).
+// Every load and store is then one global access with a scalar base and a
+// 32-bit lane offset.  Only odd-step wavefronts that contain BOUNDARY nodes
+// take the per-lane path that redirects solid links to the node's own slots.
 // ---------------------------------------------------------------------------
 template <class S, bool ODD, int MODE>
 __global__ __launch_bounds__(BLOCK) void aa_step_kernel(const StepArgs<S> A, const FastCoef K) {
@@ -262,13 +296,23 @@ __global__ __launch_bounds__(BLOCK) void aa_step_kernel(const StepArgs<S> A, con
     const int tile = xcd * q8 + (xcd < r8 ? xcd : r8) + (lin >> 3);
     const int tgx = tile % A.gx, trest = tile / A.gx;
     const int tgy = trest % A.gy, tgz = trest / A.gy;
-    const int lx = threadIdx.x & ((1 << A.tx_shift) - 1), ly = threadIdx.x >> A.tx_shift;
+    const int lx = threadIdx.x & ((1 << A.tx_shift) - 1);
+    const int ly = __builtin_amdgcn_readfirstlane(threadIdx.x >> A.tx_shift);  // one row per wavefront
     const int x = (tgx << A.tx_shift) + lx, y = tgy * A.ty + ly, z = tgz;
 
-    const uint32_t NX = (uint32_t)A.NX, NY = (uint32_t)A.NY, NZ = (uint32_t)A.NZ, N = A.N;
+    const uint32_t NX = (uint32_t)A.NX, NY = (uint32_t)A.NY, NZ = (uint32_t)A.NZ, SS = A.SS;
     const uint32_t NXY = NX * NY;
-    const bool inb = (uint32_t)x < NX && (uint32_t)y < NY;
-    const uint32_t n = inb ? (uint32_t)x + NX * ((uint32_t)y + NY * (uint32_t)z) : 0u;
+    const bool row_ok = (uint32_t)y < NY;  // wavefront-uniform
+    const bool inb = row_ok && (uint32_t)x < NX;
+    const uint32_t uy_ = row_ok ? (uint32_t)y : 0u, uz_ = (uint32_t)z;
+    // wavefront-uniform row offsets of the neighbour rows (dy, dz in -1..1)
+    const uint32_t yo[3] = {(uy_ == 0 ? NY - 1 : uy_ - 1) * NX, uy_ * NX, (uy_ + 1 == NY ? 0u : uy_ + 1) * NX};
+    const uint32_t zo[3] = {(uz_ == 0 ? NZ - 1 : uz_ - 1) * NXY, uz_ * NXY, (uz_ + 1 == NZ ? 0u : uz_ + 1) * NXY};
+    // lane x offsets (x-1, x, x+1 with periodic wrap)
+    const uint32_t ux_ = inb ? (uint32_t)x : 0u;
+    const uint32_t xo[3] = {ux_ == 0 ? NX - 1 : ux_ - 1, ux_, ux_ + 1 == NX ? 0u : ux_ + 1};
+    const uint32_t n = ux_ + yo[1] + zo[1];
+
     const uint8_t fl = inb ? A.flag[n] : (uint8_t)WALL;
     const bool active = (fl & TYPE_MASK) == FLUID;
     const bool bnd = active && (fl & BOUNDARY);
@@ -281,12 +325,6 @@ __global__ __launch_bounds__(BLOCK) void aa_step_kernel(const StepArgs<S> A, con
         A.st->remap_dmass = 0.0;
     }
 
-    // periodic neighbour coordinates (walls are solid node layers)
-    const uint32_t ux_ = (uint32_t)x, uy_ = (uint32_t)y, uz_ = (uint32_t)z;
-    const uint32_t xo[3] = {ux_ == 0 ? NX - 1 : ux_ - 1, ux_, ux_ + 1 == NX ? 0u : ux_ + 1};
-    const uint32_t yo[3] = {(uy_ == 0 ? NY - 1 : uy_ - 1) * NX, uy_ * NX, (uy_ + 1 == NY ? 0u : uy_ + 1) * NX};
-    const uint32_t zo[3] = {(uz_ == 0 ? NZ - 1 : uz_ - 1) * NXY, uz_ * NXY, (uz_ + 1 == NZ ? 0u : uz_ + 1) * NXY};
-
     // solid / particle neighbour masks (bit i: n + c_i)
     uint32_t solid = 0, part = 0;
     if (bnd) {
@@ -298,28 +336,38 @@ __global__ __launch_bounds__(BLOCK) void aa_step_kernel(const StepArgs<S> A, con
             part |= (uint32_t)(tt == PARTICLE) << i;
         });
     }
+    // odd steps redirect solid links per lane: wavefronts without boundary
+    // nodes keep scalar-base addressing throughout
+    const bool wave_bnd = ODD && __builtin_amdgcn_ballot_w64(bnd) != 0;
+
+    S* __restrict__ f = A.f;
+    // scalar base of population slot `s` shifted by a row offset, and lane
+    // byte offsets (32-bit, so every access is scalar base + lane offset)
+    auto base = [&](int s, uint32_t row) -> char* { return (char*)(f + ((size_t)s * SS + row)); };
+    const uint32_t xb[3] = {xo[0] * (uint32_t)sizeof(S), xo[1] * (uint32_t)sizeof(S), xo[2] * (uint32_t)sizeof(S)};
+    auto at = [](char* b, uint32_t off) -> S* { return (S*)(b + off); };
 
     double facc[9];
     for (int k = 0; k < 9; ++k) facc[k] = 0.0;
     double excess = 0.0;
 
     if (active) {
-        S* __restrict__ f = A.f;
         // ---- gather the 19 populations node n collides ----
         T g[Q];
-        static_for<0, Q>([&](auto I) {
-            constexpr int i = decltype(I)::value;
-            S raw;
-            if constexpr (!ODD || i == 0) {
-                raw = f[(size_t)i * N + n];
-            } else {
-                const uint32_t up = xo[1 - CX[i]] + yo[1 - CY[i]] + zo[1 - CZ[i]];
-                size_t e = (size_t)OPP[i] * N + up;
-                if ((solid >> OPP[i]) & 1u) e = (size_t)i * N + n;
-                raw = f[e];
-            }
-            g[i] = St::load(raw);
-        });
+        if (!wave_bnd) {
+            static_for<0, Q>([&](auto I) {
+                constexpr int i = decltype(I)::value;
+                if constexpr (!ODD) g[i] = St::load(*at(base(i, yo[1] + zo[1]), xb[1]));
+                else g[i] = St::load(*at(base(OPP[i], yo[1 - CY[i]] + zo[1 - CZ[i]]), xb[1 - CX[i]]));
+            });
+        } else {
+            static_for<0, Q>([&](auto I) {
+                constexpr int i = decltype(I)::value;
+                const S* p = at(base(OPP[i], yo[1 - CY[i]] + zo[1 - CZ[i]]), xb[1 - CX[i]]);
+                if ((solid >> OPP[i]) & 1u) p = at(base(i, yo[1] + zo[1]), xb[1]);
+                g[i] = St::load(*p);
+            });
+        }
 
         // pending mass-correction shift (lbm.cpp:337), applied on load
         const double dsh = A.st->shift[A.t & 1];
@@ -348,8 +396,12 @@ __global__ __launch_bounds__(BLOCK) void aa_step_kernel(const StepArgs<S> A, con
             if (A.Fx != 0.0 || A.Fy != 0.0 || A.Fz != 0.0) collide_fast<true>(g, K, rho, jx, jy, jz, drs);
             else collide_fast<false>(g, K, rho, jx, jy, jz, drs);
             rho_d = rho; rho_t = rho;
-            const float isr = K.inv_s / rho;
-            m0 = jx * isr; m1 = jy * isr; m2 = jz * isr;
+            if constexpr (MACRO) {
+                const float isr = K.inv_s / rho;
+                m0 = jx * isr; m1 = jy * isr; m2 = jz * isr;
+            } else {
+                m0 = m1 = m2 = 0.f;
+            }
             if constexpr (MASS) excess = (double)drs * (double)K.inv_s;
         }
 
@@ -376,22 +428,14 @@ __global__ __launch_bounds__(BLOCK) void aa_step_kernel(const StepArgs<S> A, con
             static_for<1, Q>([&](auto I) {
                 constexpr int i = decltype(I)::value;
                 if ((part >> i) & 1u) {
-                    double rb[3], ub[3];
-                    link_velocity(sq, A.NX, A.NY, A.NZ, A.walls_y, A.walls_z, x + 0.5 * CX[i], y + 0.5 * CY[i],
-                                  z + 0.5 * CZ[i], rb, ub);
                     double fpost;
                     if constexpr (EXACT) fpost = g[i];
                     else fpost = W[i] + (double)g[i] * (1.0 / St::scale);
-                    const double cub = CX[i] * ub[0] + CY[i] * ub[1] + CZ[i] * ub[2];
-                    const double fb = fpost - 6.0 * W[i] * rho_d * cub;
-                    const double s = fpost + fb, dl = fpost - fb;
-                    const double dpx = s * CX[i] - dl * ub[0];
-                    const double dpy = s * CY[i] - dl * ub[1];
-                    const double dpz = s * CZ[i] - dl * ub[2];
-                    facc[3] += dpx; facc[4] += dpy; facc[5] += dpz;
-                    facc[6] += rb[1] * dpz - rb[2] * dpy;
-                    facc[7] += rb[2] * dpx - rb[0] * dpz;
-                    facc[8] += rb[0] * dpy - rb[1] * dpx;
+                    const LinkResult lr = particle_link(sq, A.NX, A.NY, A.NZ, A.walls_y, A.walls_z, x, y, z, CX[i],
+                                                        CY[i], CZ[i], 6.0 * W[i], fpost, rho_d);
+                    const double fb = lr.fb;
+                    facc[3] += lr.dp[0]; facc[4] += lr.dp[1]; facc[5] += lr.dp[2];
+                    facc[6] += lr.tq[0]; facc[7] += lr.tq[1]; facc[8] += lr.tq[2];
                     if constexpr (MASS) excess += fb - fpost;
                     if constexpr (EXACT) g[i] = fb;
                     else g[i] = (T)((fb - W[i]) * St::scale);
@@ -417,18 +461,22 @@ __global__ __launch_bounds__(BLOCK) void aa_step_kernel(const StepArgs<S> A, con
         }
 
         // ---- stream ----
-        static_for<0, Q>([&](auto I) {
-            constexpr int i = decltype(I)::value;
-            const S val = St::store(g[i]);
-            if constexpr (!ODD || i == 0) {
-                f[(size_t)OPP[i] * N + n] = val;
-            } else {
-                const uint32_t dn = xo[1 + CX[i]] + yo[1 + CY[i]] + zo[1 + CZ[i]];
-                size_t e = (size_t)i * N + dn;
-                if ((solid >> i) & 1u) e = (size_t)OPP[i] * N + n;
-                f[e] = val;
-            }
-        });
+        if (!wave_bnd) {
+            static_for<0, Q>([&](auto I) {
+                constexpr int i = decltype(I)::value;
+                const S val = St::store(g[i]);
+                if constexpr (!ODD) *at(base(OPP[i], yo[1] + zo[1]), xb[1]) = val;
+                else *at(base(i, yo[1 + CY[i]] + zo[1 + CZ[i]]), xb[1 + CX[i]]) = val;
+            });
+        } else {
+            static_for<0, Q>([&](auto I) {
+                constexpr int i = decltype(I)::value;
+                const S val = St::store(g[i]);
+                S* p = at(base(i, yo[1 + CY[i]] + zo[1 + CZ[i]]), xb[1 + CX[i]]);
+                if ((solid >> i) & 1u) p = at(base(OPP[i], yo[1] + zo[1]), xb[1]);
+                *p = val;
+            });
+        }
     }
 
     // ---- block reductions ----

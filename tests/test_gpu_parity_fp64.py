@@ -39,8 +39,10 @@ def test_duct_fp64_bit_identical(oracle_mod, lbm, steps):
             o.step()
             L.step(lbm.STEP_ALL)
         _compare_exact(o, L)
-        wf = L.wall_force()
-        assert np.allclose(wf, o.L.wallForce[:], rtol=1e-12, atol=1e-15)
+        # the wall force is summed in another order (per-thread, tree, atomics):
+        # components agree to round-off of the total
+        wf, wo = L.wall_force(), np.array(o.L.wallForce[:])
+        assert np.abs(wf - wo).max() <= 1e-12 * np.abs(wo).max()
 
 
 def test_duct_fp64_non_cubic_tiles(oracle_mod, lbm):
