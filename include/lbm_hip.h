@@ -92,6 +92,7 @@ typedef struct lbm_hip_info {
     double bytes_per_lup;   /* algorithmic HBM bytes per lattice update: 2*19*sizeof(storage) */
     uint64_t device_bytes;  /* device memory held by the context */
     int has_squirmer, walls_y, walls_z;
+    int64_t count_fluid;    /* FLUID nodes now (LBM::countFluid, lbm.cpp:342) */
 } lbm_hip_info;
 
 typedef struct lbm_hip_ctx lbm_hip_ctx;
@@ -162,9 +163,10 @@ int lbm_hip_synchronize(lbm_hip_ctx* ctx);
 
 int lbm_hip_get_info(lbm_hip_ctx* ctx, lbm_hip_info* out);
 
-/* Bench support: run `steps` driver-loop steps (as lbm_hip_run) and return the
- * wall time and the summed duration of the fused step kernels, both measured
- * with HIP events on the context's stream. */
+/* Bench support: time `steps` driver-loop steps (as lbm_hip_run, graph
+ * replayed) with HIP events on the context's stream -> *total_ms; then run up
+ * to 64 further steps eagerly with an event pair around each fused step
+ * kernel and return their mean duration times `steps` -> *step_kernel_ms. */
 int lbm_hip_time_run(lbm_hip_ctx* ctx, int steps, int mass_every, double* total_ms,
                      double* step_kernel_ms);
 

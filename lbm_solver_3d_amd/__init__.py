@@ -81,6 +81,7 @@ class Info(C.Structure):
         ("tau", C.c_double), ("om_plus", C.c_double), ("om_minus", C.c_double), ("nu", C.c_double),
         ("bytes_per_lup", C.c_double), ("device_bytes", C.c_uint64),
         ("has_squirmer", C.c_int), ("walls_y", C.c_int), ("walls_z", C.c_int),
+        ("count_fluid", C.c_int64),
     ]
 
 

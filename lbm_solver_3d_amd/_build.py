@@ -51,6 +51,32 @@ def _compile(name: str, extra: list[str]) -> Path:
     return obj
 
 
+HOST = PKG / "host"
+FACADE = PKG / "liblbm_facade.so"
+DRIVER = PKG / "bin" / "lbm_squirmer"
+HOST_SRC = ["params.cpp", "squirmer.cpp", "lbm.cpp", "io.cpp", "analytic.cpp"]
+
+
+def build_host(verbose: bool = True) -> None:
+    """The drop-in C++ facade (class LBM & co. over the C ABI) and the
+    lbm_squirmer driver, compiled with the host compiler."""
+    cxx = os.environ.get("CXX", "g++")
+    flags = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", f"-I{ROOT / 'include'}", f"-I{HOST}"]
+    link = [f"-L{PKG}", "-llbm_hip", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,$ORIGIN/.."]
+    cmd = [cxx, *flags, "-shared", *[str(HOST / f) for f in HOST_SRC], "-o", str(FACADE), *link]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"facade build failed:\n{' '.join(cmd)}\n{r.stderr}")
+    DRIVER.parent.mkdir(exist_ok=True)
+    cmd = [cxx, *flags, str(HOST / "main.cpp"), "-o", str(DRIVER), f"-L{PKG}", "-llbm_facade", "-llbm_hip",
+           "-Wl,-rpath,$ORIGIN/.."]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"driver build failed:\n{' '.join(cmd)}\n{r.stderr}")
+    if verbose:
+        print(f"built {FACADE} and {DRIVER}")
+
+
 def build(verbose: bool = True) -> Path:
     BUILD.mkdir(exist_ok=True)
     with ThreadPoolExecutor(max_workers=3) as ex:
@@ -63,6 +89,7 @@ def build(verbose: bool = True) -> Path:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    build_host(verbose)
     return LIB
 
 

@@ -78,8 +78,16 @@ template hipError_t launch_step<double>(const StepArgs<double>&, const FastCoef&
 __global__ __launch_bounds__(1024) void mass_finalize_kernel(const double* part, int nparts, DevState* st,
                                                              int use_remap_dmass, int shift_slot) {
     __shared__ double lds[16];
-    double s = 0.0;
-    for (int k = threadIdx.x; k < nparts; k += 1024) s += part[k];
+    // eight independent partial sums per thread keep eight loads in flight;
+    // the combination order is fixed, so the result is deterministic
+    double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int k = threadIdx.x;
+    for (; k + 7 * 1024 < nparts; k += 8 * 1024) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) a[j] += part[k + j * 1024];
+    }
+    for (int j = 0; k < nparts; k += 1024, ++j) a[j & 7] += part[k];
+    double s = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
     if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = s;
     __syncthreads();

@@ -31,6 +31,7 @@ namespace lbmhip {
 
 constexpr int SQ_THREADS = 1024;
 constexpr int REMAP_CAP = 1024;  // changed nodes per remap held in LDS
+constexpr int BOX_MAX = 128;     // remap box columns per axis held in LDS
 
 template <class S>
 struct RemapArgs {
@@ -94,6 +95,8 @@ __global__ __launch_bounds__(SQ_THREADS) void squirmer_kernel(const RemapArgs<S>
     __shared__ int sbox[6];  // x0, y0, z0, wx, wy, wz
     __shared__ int scount[SQ_THREADS / 64];
     __shared__ int sK, sOverflow;
+    __shared__ double boxr[3][BOX_MAX];
+    __shared__ int boxc[3][BOX_MAX];
 
     DevState* st = A.st;
     DevSquirmer& sq = st->sq;
@@ -163,30 +166,69 @@ __global__ __launch_bounds__(SQ_THREADS) void squirmer_kernel(const RemapArgs<S>
     __syncthreads();
 
     // ---------------- B: classify the box in sweep order ----------------
+    // Per-axis minimum-image offsets (lbm.cpp:100-104) and wrapped
+    // coordinates of the box columns are computed once into LDS; a node's
+    // inside test is then (rx^2 + ry^2) + rz^2 < R^2 with the same doubles
+    // the reference forms.  Each thread owns a contiguous chunk of box
+    // positions (sweep order), issues its flag loads in batches, and keeps
+    // change / cover bits in registers for the compaction pass.
     const int bx0 = sbox[0], by0 = sbox[1], bz0 = sbox[2], bwx = sbox[3], bwy = sbox[4], bwz = sbox[5];
+    if (bwx > BOX_MAX || bwy > BOX_MAX || bwz > BOX_MAX) {
+        if (tid == 0) atomicOr(&st->err, ERR_REMAP_OVERFLOW);
+        return;  // uniform
+    }
+    for (int k = tid; k < bwx + bwy + bwz; k += SQ_THREADS) {
+        int d = 0, j = k;
+        if (j >= bwx) { j -= bwx; d = 1; if (j >= bwy) { j -= bwy; d = 2; } }
+        const int Nd = d == 0 ? NX : (d == 1 ? NY : NZ);
+        const int c = (d == 0 ? bx0 : (d == 1 ? by0 : bz0)) + j;
+        const int cw = wrapi(c, Nd);
+        double r = cw - sq.X[d];
+        const bool per = d == 0 || (d == 1 ? !A.walls_y : !A.walls_z);
+        if (per) r -= Nd * round(r / Nd);
+        boxr[d][j] = r;
+        boxc[d][j] = cw;
+    }
+    __syncthreads();
     const int M = bwx * bwy * bwz;
     const int chunk = (M + SQ_THREADS - 1) / SQ_THREADS;
     const int k0 = tid * chunk, k1 = (k0 + chunk < M) ? k0 + chunk : M;
     const double R2 = sq.R * sq.R;
-    auto classify = [&](int k, uint32_t& node) -> int {
-        const int xx = bx0 + k % bwx, yy = by0 + (k / bwx) % bwy, zz = bz0 + k / (bwx * bwy);
-        const int x = wrapi(xx, NX), y = wrapi(yy, NY), z = wrapi(zz, NZ);
-        const uint32_t n = (uint32_t)x + (uint32_t)NX * ((uint32_t)y + (uint32_t)NY * (uint32_t)z);
-        const uint8_t t = A.flag[n] & TYPE_MASK;
-        if (t == WALL) return 0;
-        double r[3];
-        node_offset(sq, NX, NY, NZ, A.walls_y, A.walls_z, x, y, z, r);
-        const bool inside = r[0] * r[0] + r[1] * r[1] + r[2] * r[2] < R2;
-        const bool was = t == PARTICLE;
-        if (inside == was) return 0;
-        node = n;
-        return inside ? 2 : 1;  // 2: cover, 1: uncover
+    const uint32_t uNX = (uint32_t)NX, uNXY = (uint32_t)NX * (uint32_t)NY;
+    auto node_at = [&](int ix, int iy, int iz) -> uint32_t {
+        return (uint32_t)boxc[0][ix] + uNX * (uint32_t)boxc[1][iy] + uNXY * (uint32_t)boxc[2][iz];
     };
+    uint64_t chg = 0, cov = 0;  // bit j: position k0 + j changes / is covered
     int cnt = 0;
-    for (int k = k0; k < k1; ++k) {
-        uint32_t nn;
-        cnt += classify(k, nn) != 0;
+    if (k0 < k1) {
+        int ix = k0 % bwx, iy = (k0 / bwx) % bwy, iz = k0 / (bwx * bwy);
+        for (int kb = k0; kb < k1; kb += 8) {
+            uint8_t fv[8];
+            int jx[8], jy[8], jz[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                jx[j] = ix; jy[j] = iy; jz[j] = iz;
+                fv[j] = (kb + j < k1) ? A.flag[node_at(ix, iy, iz)] : (uint8_t)WALL;
+                if (++ix == bwx) { ix = 0; if (++iy == bwy) { iy = 0; ++iz; } }
+            }
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const uint8_t t = fv[j] & TYPE_MASK;
+                if (t == WALL) continue;
+                const double rx = boxr[0][jx[j]], ry = boxr[1][jy[j]], rz = boxr[2][jz[j]];
+                const bool inside = rx * rx + ry * ry + rz * rz < R2;
+                if (inside != (t == PARTICLE)) {
+                    const int b = kb + j - k0;
+                    if (b < 64) {
+                        chg |= 1ull << b;
+                        if (inside) cov |= 1ull << b;
+                    }
+                    ++cnt;
+                }
+            }
+        }
     }
+    if (chunk > 64 && tid == 0) atomicOr(&st->err, ERR_REMAP_OVERFLOW);
     // block exclusive scan of cnt
     const int lane = tid & 63, wave = tid >> 6;
     int incl = cnt;
@@ -197,20 +239,22 @@ __global__ __launch_bounds__(SQ_THREADS) void squirmer_kernel(const RemapArgs<S>
     if (lane == 63) scount[wave] = incl;
     __syncthreads();
     if (tid == 0) {
-        int s = 0;
-        for (int w = 0; w < SQ_THREADS / 64; ++w) { const int c = scount[w]; scount[w] = s; s += c; }
-        sK = s;
-        if (s > REMAP_CAP) { sOverflow = 1; atomicOr(&st->err, ERR_REMAP_OVERFLOW); }
+        int tot = 0;
+        for (int w = 0; w < SQ_THREADS / 64; ++w) { const int c = scount[w]; scount[w] = tot; tot += c; }
+        sK = tot;
+        if (tot > REMAP_CAP || chunk > 64) { sOverflow = 1; atomicOr(&st->err, ERR_REMAP_OVERFLOW); }
     }
     __syncthreads();
     const int K = sK;
     if (sOverflow) return;  // uniform: the host reports the error
     {
         int pos = scount[wave] + incl - cnt;
-        for (int k = k0; k < k1 && cnt > 0; ++k) {
-            uint32_t nn;
-            const int c = classify(k, nn);
-            if (c) { list[pos++] = nn | (c == 2 ? 0x80000000u : 0u); --cnt; }
+        while (chg) {
+            const int b = __builtin_ctzll(chg);
+            chg &= chg - 1;
+            const int k = k0 + b;
+            const int ix = k % bwx, iy = (k / bwx) % bwy, iz = k / (bwx * bwy);
+            list[pos++] = node_at(ix, iy, iz) | (((cov >> b) & 1ull) ? 0x80000000u : 0u);
         }
     }
     __syncthreads();

@@ -1,0 +1,31 @@
+// io.hpp -- drop-in for the reference's src/utils/io.hpp: legacy-VTK field
+// output, trajectory rows and the final squirmer state, in the reference's
+// byte and text formats (io.cpp:44-97), so its fixtures and readers apply.
+#pragma once
+#include <cstdio>
+#include <string>
+
+#include "lbm.hpp"
+
+namespace lbm {
+
+// STRUCTURED_POINTS, binary big-endian: flag (uint8), density (float64),
+// velocity (3 x float64).  Calls L.syncHost(false) first.
+void writeVTK(LBM& L, const std::string& path);
+
+// One row every traj_every steps:  t X Y Z ex ey ez Ux Uy Uz Wx Wy Wz Fwall_y Fwall_z
+class TrajectoryWriter {
+public:
+    explicit TrajectoryWriter(const std::string& path);
+    ~TrajectoryWriter();
+    void write(int t, const Squirmer& s);
+    void flush();
+
+private:
+    std::FILE* fp_ = nullptr;
+};
+
+void writeSquirmerState(const LBM& L, int t, const std::string& path);
+void ensureDirectory(const std::string& dir);
+
+}  // namespace lbm

@@ -1,0 +1,108 @@
+"""The lbm_squirmer driver built on the C++ facade (lbm_solver_3d_amd/host):
+same CLI, input decks, error behaviour and output files as the reference's
+bin/lbm_squirmer (src/main.cpp, src/utils/params.cpp, io.cpp, analytic.cpp).
+
+CPU tests exercise parsing and validation (they exit before any GPU call);
+GPU tests run the reference's input decks and compare the outputs with the
+reference's own output fixtures."""
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import analytic as A
+
+ROOT = Path(__file__).resolve().parents[1]
+DRIVER = ROOT / "lbm_solver_3d_amd" / "bin" / "lbm_squirmer"
+REFOUT = ROOT / "tests" / "golden" / "reference_outputs"
+
+
+def run(args, cwd=None, timeout=600):
+    if not DRIVER.exists():
+        pytest.skip("driver not built")
+    return subprocess.run([str(DRIVER), *map(str, args)], capture_output=True, text=True, cwd=cwd, timeout=timeout)
+
+
+def read_kv(path):
+    out = {}
+    for line in Path(path).read_text().splitlines():
+        p = line.split()
+        if p:
+            vals = [float(v) for v in p[1:]]
+            out[p[0]] = vals[0] if len(vals) == 1 else np.array(vals)
+    return out
+
+
+# ------------------------------------------------------------------ CPU
+@pytest.mark.parametrize("args,msg", [
+    ([], "usage:"),
+    (["/nonexistent/deck.in"], "error: cannot open input file '/nonexistent/deck.in'"),
+    ([REFOUT / "duct.in", "tau=0.5"], "error: tau must be > 0.5"),
+    ([REFOUT / "duct.in", "bogus=1"], "error: unknown parameter 'bogus'"),
+    ([REFOUT / "duct.in", "nx=abc"], "error: invalid value 'abc' for parameter 'nx'"),
+    ([REFOUT / "duct.in", "walls_y=maybe"], "error: cannot interpret 'maybe' as a boolean"),
+    ([REFOUT / "channel_neutral.in", "angle=50"], "error: initial angle must satisfy |angle| < 45 deg"),
+    ([REFOUT / "channel_neutral.in", "y0=3"], "error: squirmer initially overlaps (or is too close to) a y-wall"),
+    ([REFOUT / "channel_neutral.in", "b1=0.1"], "swimming speed U0 = 2 B1 / 3 too large"),
+    ([REFOUT / "duct.in", "print_every=0"], "error: print_every and traj_every must be >= 1"),
+])
+def test_driver_rejects_like_reference(args, msg, tmp_path):
+    r = run(args, cwd=tmp_path)
+    assert r.returncode == 1
+    assert msg in r.stderr
+
+
+def test_driver_bad_deck_line(tmp_path):
+    deck = tmp_path / "bad.in"
+    deck.write_text("nx = 8\nthis line has no equals sign\n")
+    r = run([deck], cwd=tmp_path)
+    assert r.returncode == 1
+    assert f"error: {deck}:2: expected 'key = value'" in r.stderr
+
+
+# ------------------------------------------------------------------ GPU
+@pytest.mark.gpu
+def test_duct_deck_matches_reference_outputs(tmp_path):
+    """inputs/duct.in through the driver (FP64 storage): outputs/duct_summary.txt,
+    duct_profile_{y,z}.dat and duct_final.vtk."""
+    r = run([REFOUT / "duct.in", f"output_dir={tmp_path}", "precision=fp64"])
+    assert r.returncode == 0, r.stderr
+    s = read_kv(tmp_path / "duct_summary.txt")
+    ref = read_kv(REFOUT / "duct_summary.txt")
+    assert s["steps"] == ref["steps"] == 16000 and s["converged"] == 1
+    assert s["nu"] == ref["nu"]
+    assert s["l2_error"] == pytest.approx(ref["l2_error"], rel=1e-8)
+    assert s["max_speed"] == pytest.approx(ref["max_speed"], rel=1e-9)
+    assert s["mean_density"] == pytest.approx(ref["mean_density"], abs=1e-10)
+    assert s["wall_force"][0] == pytest.approx(ref["wall_force"][0], rel=1e-9)
+    assert np.array_equal(s["body_force_total"], ref["body_force_total"])
+    for ax in ("y", "z"):
+        a = np.loadtxt(tmp_path / f"duct_profile_{ax}.dat")
+        b = np.loadtxt(REFOUT / f"duct_profile_{ax}.dat")
+        assert np.array_equal(a[:, 0], b[:, 0])
+        assert np.allclose(a[:, 1:], b[:, 1:], rtol=1e-8)
+    v, w = A.read_vtk(tmp_path / "duct_final.vtk"), A.read_vtk(REFOUT / "duct_final.vtk")
+    assert v["dims"] == w["dims"] and np.array_equal(v["flag"], w["flag"])
+    assert np.allclose(v["density"], w["density"], rtol=1e-12)
+    assert np.allclose(v["velocity"], w["velocity"], rtol=1e-8, atol=1e-14)
+    # byte-level format: identical header and sizes
+    assert (tmp_path / "duct_final.vtk").stat().st_size == (REFOUT / "duct_final.vtk").stat().st_size
+
+
+@pytest.mark.gpu
+def test_puller_deck_trajectory_rows(tmp_path):
+    """inputs/channel_puller.in for 1000 steps: trajectory rows equal the first
+    rows of outputs/channel_puller_traj.dat to the printed digits."""
+    r = run([REFOUT / "channel_puller.in", f"output_dir={tmp_path}", "steps=1000", "print_every=500",
+             "precision=fp64"])
+    assert r.returncode == 0, r.stderr
+    mine = np.loadtxt(tmp_path / "channel_puller_traj.dat")
+    ref = np.loadtxt(REFOUT / "channel_puller_traj.dat")[:len(mine)]
+    assert len(mine) == 20 and np.array_equal(mine[:, 0], ref[:, 0])
+    assert np.allclose(mine[:, 1:7], ref[:, 1:7], atol=1.5e-6)
+    assert np.allclose(mine[:, 7:13], ref[:, 7:13], rtol=3e-6, atol=1e-12)
+    st = read_kv(tmp_path / "channel_puller_squirmer.dat")
+    assert st["t"] == 1000 and st["R"] == 4 and st["beta"] == 1
+    summ = read_kv(tmp_path / "channel_puller_summary.txt")
+    assert summ["steps"] == 1000 and "contacts" in summ

@@ -1,0 +1,212 @@
+"""The reference's own acceptance tests and the BASELINE validation cases,
+run on the GPU path in every storage precision.
+
+* duct flow: L2 error against the series solution, second-order convergence,
+  wall force = body force, mass (reference tests/test_duct_flow.py);
+* plane channel exact with Lambda = 3/16;
+* 64x32x32 duct with tau = 1, Fx = 1e-6, 20000 steps, second order over
+  cross-sections 16 / 32 / 64;
+* squirmer in a periodic box: swimming speed, straight swimming, pusher >
+  neutral > puller, mass, Blake near field (tests/test_squirmer_bulk.py);
+* 128^3 neutral squirmer (R = 6, B1 = 1.5e-3): speed against the FP64 run
+  and against Blake's 2 B1 / 3.
+Reduced precisions are compared with the FP64 mode, which itself matches the
+CPU oracle bit for bit (duct) or to 1e-12 (squirmer), see
+test_gpu_parity_fp64.py.  Tolerances are written next to each assertion.
+"""
+import numpy as np
+import pytest
+
+import analytic as A
+
+pytestmark = pytest.mark.gpu
+
+PRECS = ["fp64", "fp32", "fp16", "bf16"]
+
+
+def steady(L, lbm, steps=100000, print_every=1000, tol=1e-12):
+    """main.cpp:60-87 convergence loop on the GPU path."""
+    prev, t = 0.0, 0
+    while t < steps:
+        L.run(print_every - 1, 0, 0)
+        L.step(lbm.STEP_ALL)
+        t += print_every
+        sx = L.diagnostics()["sum_ux"]
+        change = abs(sx - prev) / (abs(sx) + 1e-30)
+        prev = sx
+        if tol > 0 and change < tol:
+            return t, True
+    return t, False
+
+
+# duct L2 errors of the reference at H = 8 / 16 / 32 (tau = 0.8, fx = 5e-5 (32/H)^2),
+# reproduced by the oracle (tests/test_oracle_cpu.py) and the notebook (tutorial.ipynb:331-334)
+DUCT_L2 = {8: 4.025e-3, 16: 8.803e-4, 32: 2.105e-4}
+# relative L2 floor each storage format adds (u_max ~ 0.04); see the FP16/BF16 tests below
+DUCT_TOL = {"fp64": 2e-3, "fp32": 2e-3, "fp16": 0.05, "bf16": 0.25}
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_duct_against_exact_solution(lbm, prec):
+    errs = []
+    for H in (8, 16, 32):
+        fx = 5e-5 * (32 / H) ** 2
+        with lbm.Lattice(nx=4, ny=H + 2, nz=H + 2, tau=0.8, fx=fx, walls_y=1, walls_z=1, precision=prec) as L:
+            done, conv = steady(L, lbm, tol=1e-12 if prec in ("fp64", "fp32") else 1e-6)
+            fd = L.fields()
+            err = A.poiseuille_error(fd["ux"], fd["flag"], 4, H + 2, H + 2, True, True, L.nu, fx)
+            errs.append(err)
+            d = L.diagnostics()
+            fw = L.wall_force()[0]
+            fb = fx * d["count_fluid"]
+            if prec in ("fp64", "fp32"):
+                assert conv
+                assert abs(fw - fb) / fb < 1e-6, (H, fw, fb)        # momentum balance (test_duct_flow.py:45-49)
+                rho_tol = 1e-10 if prec == "fp64" else 1e-6
+                assert abs(d["mean_density"] - 1.0) < rho_tol
+        # L2 against the exact solution at the reference's values
+        assert err == pytest.approx(DUCT_L2[H], rel=DUCT_TOL[prec]), (prec, H, err)
+    order = np.log(np.array(errs[:-1]) / np.array(errs[1:])) / np.log(2)
+    if prec in ("fp64", "fp32", "fp16"):
+        assert np.all(order > 1.8), order
+    assert errs[-1] < 5e-4 or prec == "bf16"
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_plane_channel_exact(lbm, prec):
+    with lbm.Lattice(nx=4, ny=34, nz=1, walls_y=1, walls_z=0, tau=0.8, fx=1e-5, precision=prec) as L:
+        done, conv = steady(L, lbm)
+        fd = L.fields()
+        err = A.poiseuille_error(fd["ux"], fd["flag"], 4, 34, 1, True, False, L.nu, 1e-5)
+    assert conv
+    assert err < 1e-6, err   # test_duct_flow.py:52-56
+
+
+def test_c1_duct_second_order_fp64(lbm):
+    """64x32x32-class duct, tau = 1, Fx = 1e-6, f = feq, 20000 steps; cross-sections 16/32/64."""
+    errs = {}
+    for H in (16, 32, 64):
+        with lbm.Lattice(nx=4, ny=H + 2, nz=H + 2, tau=1.0, fx=1e-6, walls_y=1, walls_z=1,
+                         precision="fp64") as L:
+            L.run(20000, 0, lbm.STEP_ALL)
+            fd = L.fields()
+            errs[H] = A.poiseuille_error(fd["ux"], fd["flag"], 4, H + 2, H + 2, True, True, L.nu, 1e-6)
+    e = np.array([errs[16], errs[32], errs[64]])
+    order = np.log(e[:-1] / e[1:]) / np.log(2)
+    assert np.all(order > 1.8), (errs, order)
+    assert errs[64] < 1e-4
+
+
+def test_c1_duct_64x32x32_precisions(lbm):
+    """The BASELINE C1 grid itself: FP32 / FP16 / BF16 against FP64 after 20000 steps."""
+    kv = dict(nx=64, ny=32, nz=32, tau=1.0, fx=1e-6, walls_y=1, walls_z=1)
+    ux = {}
+    for prec in PRECS:
+        with lbm.Lattice(**kv, precision=prec) as L:
+            L.run(20000, 0, lbm.STEP_ALL)
+            fd = L.fields()
+            m = fd["flag"] == 0
+            ux[prec] = fd["ux"][m]
+            if prec == "fp64":
+                err = A.poiseuille_error(fd["ux"], fd["flag"], 64, 32, 32, True, True, L.nu, 1e-6)
+                assert err < 1e-3
+    ref = ux["fp64"]
+    rel = {p: float(np.linalg.norm(ux[p] - ref) / np.linalg.norm(ref)) for p in PRECS}
+    # measured storage error against FP64 (relative L2 of u_x over the fluid)
+    assert rel["fp32"] < 1e-5, rel
+    assert rel["fp16"] < 2e-2, rel
+    assert rel["bf16"] < 1.5e-1, rel
+
+
+# ------------------------------------------------------------------ squirmer
+BULK = dict(nx=40, ny=40, nz=40, walls_y=0, walls_z=0, squirmer=1, radius=4.0, b1=0.015, angle=0, planar=1,
+            tau=1.0)
+U0 = 2 * 0.015 / 3
+
+
+def swim(L, steps, every=10):
+    """Trajectory samples of (t, U, W, e) every `every` steps (driver loop, mass correction every 10)."""
+    out = []
+    t = 0
+    while t < steps:
+        L.run(every, 10, 0)
+        t += every
+        s = L.squirmer()
+        out.append((t, *s["U"], *s["W"], *s["e"]))
+    return np.array(out)
+
+
+@pytest.fixture(scope="module")
+def bulk_runs(lbm):
+    runs = {}
+    for prec in ("fp64", "fp32", "fp16"):
+        for beta in (-3.0, 0.0, 3.0):
+            with lbm.Lattice(**BULK, beta=beta, precision=prec) as L:
+                tr = swim(L, 2000)
+                L.step(lbm.STEP_ALL)
+                fd = L.fields()
+                runs[prec, beta] = dict(tr=tr, fields=fd, state=L.squirmer(), diag=L.diagnostics())
+    return runs
+
+
+def late(tr, col):
+    return tr[tr[:, 0] > 1000, col]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "fp16"])
+def test_bulk_swimming_speed(bulk_runs, prec):
+    ratio = late(bulk_runs[prec, 0.0]["tr"], 1).mean() / U0
+    assert 0.93 < ratio < 1.03, ratio                                  # test_squirmer_bulk.py:47-49
+    assert ratio == pytest.approx(0.985, abs=0.003)                    # tutorial.ipynb:549-563
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "fp16"])
+def test_bulk_straight_and_ordered(bulk_runs, prec):
+    u = {}
+    for beta in (-3.0, 0.0, 3.0):
+        tr = bulk_runs[prec, beta]["tr"]
+        assert np.abs(late(tr, 2)).max() < 1e-3 * U0                   # test_squirmer_bulk.py:52-57
+        assert np.abs(late(tr, 6)).max() < 1e-6
+        assert np.abs(tr[:, 8]).max() < 1e-6                           # e_y: no rotation
+        u[beta] = late(tr, 1).mean()
+        assert abs(bulk_runs[prec, beta]["diag"]["mean_density"] - 1.0) < 1e-4
+    assert u[-3.0] > u[0.0] > u[3.0]                                   # test_squirmer_bulk.py:60-62
+    # notebook values 1.143 / 0.985 / 0.865 (tutorial.ipynb:549-563)
+    assert [u[b] / U0 for b in (-3.0, 0.0, 3.0)] == pytest.approx([1.143, 0.985, 0.865], abs=0.004)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp16"])
+def test_bulk_speed_vs_fp64(bulk_runs, prec):
+    for beta in (-3.0, 0.0, 3.0):
+        u = late(bulk_runs[prec, beta]["tr"], 1).mean()
+        u64 = late(bulk_runs["fp64", beta]["tr"], 1).mean()
+        assert abs(u / u64 - 1) < (1e-5 if prec == "fp32" else 5e-3), (beta, u / u64)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "fp16"])
+def test_bulk_near_field_blake(bulk_runs, prec):
+    for beta in (-3.0, 0.0, 3.0):
+        r = bulk_runs[prec, beta]
+        fd = r["fields"]
+        e = A.near_field_error(fd["ux"], fd["uy"], fd["flag"], 40, 40, 40, r["state"]["X"], 4.0, 0.015, beta)
+        assert e < 0.30, (beta, e)                                     # test_squirmer_bulk.py:70-75
+        # tutorial.ipynb:617-619: 0.21 / 0.16 / 0.22
+        assert e == pytest.approx({-3.0: 0.21, 0.0: 0.16, 3.0: 0.22}[beta], abs=0.02), (beta, e)
+
+
+def test_c2_bulk_128_speed(lbm):
+    """128^3 periodic box, R = 6, B1 = 1.5e-3 (U0 = 1e-3), beta = 0, 5000 steps:
+    FP32 / FP16 speed within 0.5 % of FP64; FP64 near Blake's 2 B1 / 3 up to
+    the finite-box correction."""
+    kv = dict(nx=128, ny=128, nz=128, walls_y=0, walls_z=0, squirmer=1, radius=6.0, b1=1.5e-3, beta=0.0, angle=0,
+              planar=1, tau=1.0)
+    u = {}
+    for prec in ("fp64", "fp32", "fp16", "bf16"):
+        with lbm.Lattice(**kv, precision=prec) as L:
+            tr = swim(L, 5000, every=50)
+        u[prec] = tr[tr[:, 0] > 2500, 1].mean()
+    u0 = 2 * 1.5e-3 / 3
+    assert 0.93 < u["fp64"] / u0 < 1.03, u["fp64"] / u0
+    assert abs(u["fp32"] / u["fp64"] - 1) < 1e-4
+    assert abs(u["fp16"] / u["fp64"] - 1) < 5e-3
+    assert abs(u["bf16"] / u["fp64"] - 1) < 5e-2
