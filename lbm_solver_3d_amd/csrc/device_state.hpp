@@ -20,18 +20,19 @@ struct DevSquirmer {
 };
 
 // Everything the kernels exchange through device memory besides the fields.
-// Buffers indexed [t & 1] are double-buffered by step parity: step t
-// accumulates into slot t&1 and clears slot (t+1)&1, so no kernel clears what
-// another block of the same launch still reads, and no memset is needed
-// (the step can be captured into a graph).
+// The mass shift is double-buffered by step parity: step t reads slot t&1
+// and clears slot (t+1)&1, so no kernel clears what another block of the
+// same launch still reads, and no memset is needed (the step can be
+// captured into a graph).
 struct DevState {
-    DevSquirmer sq;
-    // Momentum exchange of step t: [0..2] wall force, [3..5] particle force,
-    // [6..8] particle torque (lbm.cpp:131-132, 209-215).
-    double acc[2][9];
+    DevSquirmer sq;        // Fh, Th published by particle_links_kernel after each step
+    double wallForce[3];   // wall momentum exchange of the last step that computed it
     // Pending mass-correction shift d read by step t from slot t&1
     // (lbm.cpp:333, 337: f_i += w_i d).
     double shift[2];
+    double link_dmass;     // mass change of the moving-boundary corrections of the last step
+    unsigned link_ticket;  // last-block counters of the two link kernels
+    unsigned wall_ticket;
     // Mass change made by remaps since the last step, as excess over one
     // unit per fluid node (cover removes sum_i f_i - 1, uncover adds
     // sum_i feq_i - 1).

@@ -13,6 +13,20 @@ hipError_t launch_squirmer(const RemapArgs<S>& a, hipStream_t s) {
 }
 
 template <class S>
+hipError_t launch_links(const LinkArgs<S>& a, int blocks, hipStream_t s) {
+    hipLaunchKernelGGL((particle_links_kernel<S>), dim3(blocks), dim3(LINK_BLOCK), 0, s, a);
+    return hipGetLastError();
+}
+
+template <class S>
+hipError_t launch_wall_force(const S* f, const uint8_t* flag, double* part, DevState* st, int NX, int NY, int NZ,
+                             uint32_t N, uint32_t SS, hipStream_t s) {
+    hipLaunchKernelGGL((wall_force_kernel<S>), dim3((N + 255) / 256), dim3(256), 0, s, f, flag, part, st, NX, NY, NZ,
+                       N, SS);
+    return hipGetLastError();
+}
+
+template <class S>
 hipError_t launch_gather(const S* f, const uint8_t* flag, double* out, int NX, int NY, int NZ, uint32_t N,
                          uint32_t SS, int shifted, const DevState* st, int shift_slot, hipStream_t s) {
     hipLaunchKernelGGL((gather_kernel<S>), dim3((N + 255) / 256), dim3(256), 0, s, f, flag, out, NX, NY, NZ, N, SS,
@@ -54,6 +68,9 @@ hipError_t launch_fill_populations(S* f, uint32_t SS, hipStream_t s) {
 
 #define LBMHIP_INSTANTIATE(S)                                                                                     \
     template hipError_t launch_squirmer<S>(const RemapArgs<S>&, hipStream_t);                                    \
+    template hipError_t launch_links<S>(const LinkArgs<S>&, int, hipStream_t);                                   \
+    template hipError_t launch_wall_force<S>(const S*, const uint8_t*, double*, DevState*, int, int, int,       \
+                                             uint32_t, uint32_t, hipStream_t);                                  \
     template hipError_t launch_gather<S>(const S*, const uint8_t*, double*, int, int, int, uint32_t, uint32_t,   \
                                          int, const DevState*, int, hipStream_t);                                \
     template hipError_t launch_scatter<S>(S*, const uint8_t*, const double*, int, int, int, uint32_t, uint32_t,  \
@@ -94,7 +111,7 @@ __global__ __launch_bounds__(1024) void mass_finalize_kernel(const double* part,
     if (threadIdx.x == 0) {
         double e = 0.0;
         for (int w = 0; w < 16; ++w) e += lds[w];
-        if (use_remap_dmass) e += st->remap_dmass;
+        if (use_remap_dmass) e += st->remap_dmass + st->link_dmass;
         const long long nf = st->nfluid;
         if (nf > 0) st->shift[shift_slot] += -e / (double)nf;
     }

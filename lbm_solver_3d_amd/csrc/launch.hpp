@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include "device_state.hpp"
+#include "links_kernel.hpp"
 #include "squirmer_kernel.hpp"
 #include "step_kernel.hpp"
 #include "storage.hpp"
@@ -17,6 +18,13 @@ hipError_t launch_step(const StepArgs<S>& a, const FastCoef& k, bool odd, int mo
 
 template <class S>
 hipError_t launch_squirmer(const RemapArgs<S>& a, hipStream_t s);
+
+template <class S>
+hipError_t launch_links(const LinkArgs<S>& a, int blocks, hipStream_t s);
+
+template <class S>
+hipError_t launch_wall_force(const S* f, const uint8_t* flag, double* part, DevState* st, int NX, int NY, int NZ,
+                             uint32_t N, uint32_t SS, hipStream_t s);
 
 template <class S>
 hipError_t launch_gather(const S* f, const uint8_t* flag, double* out, int NX, int NY, int NZ, uint32_t N,
@@ -44,6 +52,9 @@ hipError_t launch_macro_to_double(const void* src, int is_double, double* dst, u
 #define LBMHIP_DECLARE(S)                                                                                          \
     extern template hipError_t launch_step<S>(const StepArgs<S>&, const FastCoef&, bool, int, hipStream_t);     \
     extern template hipError_t launch_squirmer<S>(const RemapArgs<S>&, hipStream_t);                            \
+    extern template hipError_t launch_links<S>(const LinkArgs<S>&, int, hipStream_t);                           \
+    extern template hipError_t launch_wall_force<S>(const S*, const uint8_t*, double*, DevState*, int, int, int, \
+                                                    uint32_t, uint32_t, hipStream_t);                           \
     extern template hipError_t launch_gather<S>(const S*, const uint8_t*, double*, int, int, int, uint32_t,      \
                                                 uint32_t, int, const DevState*, int, hipStream_t);              \
     extern template hipError_t launch_scatter<S>(S*, const uint8_t*, const double*, int, int, int, uint32_t,     \

@@ -14,15 +14,11 @@ static hipError_t launch_step_v(const StepArgs<S>& a, const FastCoef& k, hipStre
 
 template <class S, bool ODD>
 static hipError_t launch_step_p(const StepArgs<S>& a, const FastCoef& k, int mode, hipStream_t s) {
-    switch (mode & 7) {
+    switch (mode & (STEP_MACRO | STEP_MASS)) {
         case 0: return launch_step_v<S, ODD, 0>(a, k, s);
-        case 1: return launch_step_v<S, ODD, 1>(a, k, s);
-        case 2: return launch_step_v<S, ODD, 2>(a, k, s);
-        case 3: return launch_step_v<S, ODD, 3>(a, k, s);
-        case 4: return launch_step_v<S, ODD, 4>(a, k, s);
-        case 5: return launch_step_v<S, ODD, 5>(a, k, s);
-        case 6: return launch_step_v<S, ODD, 6>(a, k, s);
-        default: return launch_step_v<S, ODD, 7>(a, k, s);
+        case STEP_MACRO: return launch_step_v<S, ODD, STEP_MACRO>(a, k, s);
+        case STEP_MASS: return launch_step_v<S, ODD, STEP_MASS>(a, k, s);
+        default: return launch_step_v<S, ODD, STEP_MACRO | STEP_MASS>(a, k, s);
     }
 }
 

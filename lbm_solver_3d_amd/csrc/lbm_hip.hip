@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -61,6 +62,8 @@ struct lbm_hip_ctx {
     double* mass_part = nullptr;  // >= max(G, ceil(N/256)) doubles
     double* diag_part = nullptr;  // 4 * ceil(N/256) doubles
     double* diag_out = nullptr;   // 4 doubles
+    double* red_part = nullptr;   // 8 * max(LINK_MAX_BLOCKS, ceil(N/256)) doubles (link / wall kernels)
+    int link_blocks = 0;
     int gx = 0, gy = 0, gz = 0, tx_shift = 8, ty = 1, G = 0, nb256 = 0;
     double omPlus = 0, omMinus = 0, nu = 0;
     FastCoef K{};
@@ -207,9 +210,24 @@ RemapArgs<S> remap_args(const lbm_hip_ctx* c) {
     a.NX = c->NX; a.NY = c->NY; a.NZ = c->NZ; a.N = c->N; a.SS = c->SS;
     a.walls_y = c->p.walls_y; a.walls_z = c->p.walls_z;
     a.shifted = (c->t > 0 && ((c->t - 1) & 1) == 0) ? 1 : 0;
-    a.acc_slot = (int)((c->t - 1) & 1);
     a.shift_slot = (int)(c->t & 1);
     return a;
+}
+
+template <class S>
+LinkArgs<S> link_args(const lbm_hip_ctx* c) {
+    LinkArgs<S> a{};
+    a.f = (S*)c->f; a.flag = c->flag; a.rho = c->rho; a.st = c->st; a.part = c->red_part;
+    a.NX = c->NX; a.NY = c->NY; a.NZ = c->NZ; a.SS = c->SS;
+    a.walls_y = c->p.walls_y; a.walls_z = c->p.walls_z;
+    return a;
+}
+
+// Blocks of particle_links_kernel: its box is at most ceil(2R) + 5 wide per axis.
+int link_blocks_for(double R, int NX, int NY, int NZ) {
+    const int w = (int)std::ceil(2.0 * R) + 5;
+    const long long M = (long long)std::min(w, NX) * std::min(w, NY) * std::min(w, NZ);
+    return (int)((M + LINK_BLOCK - 1) / LINK_BLOCK);
 }
 
 // Slot stride: N rounded up to 256 bytes, plus a pad so that the 19
@@ -229,7 +247,7 @@ void free_ctx(lbm_hip_ctx* c) {
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
     for (void* q : {c->f, (void*)c->flag, c->rho, c->ux, c->uy, c->uz, (void*)c->st, (void*)c->mass_part,
-                    (void*)c->diag_part, (void*)c->diag_out})
+                    (void*)c->diag_part, (void*)c->diag_out, (void*)c->red_part})
         if (q) hipFree(q);
     for (hipEvent_t e : c->events) hipEventDestroy(e);
     if (c->graph) hipGraphExecDestroy(c->graph);
@@ -308,7 +326,8 @@ int lbm_hip_create(const lbm_hip_params* pin, lbm_hip_ctx** out) {
         !dmalloc(&c->uz, N * c->msize) || !dmalloc((void**)&c->st, sizeof(DevState)) ||
         !dmalloc((void**)&c->mass_part, sizeof(double) * nparts) ||
         !dmalloc((void**)&c->diag_part, sizeof(double) * 4 * c->nb256) ||
-        !dmalloc((void**)&c->diag_out, sizeof(double) * 4))
+        !dmalloc((void**)&c->diag_out, sizeof(double) * 4) ||
+        !dmalloc((void**)&c->red_part, sizeof(double) * 8 * std::max(LINK_MAX_BLOCKS, c->nb256)))
         return cleanup(fail(LBM_HIP_ENOMEM, "device allocation failed"));
 
     // flags: wall layers (lbm.cpp:72-78)
@@ -349,6 +368,8 @@ int lbm_hip_create(const lbm_hip_params* pin, lbm_hip_ctx** out) {
                     flag[n] = inside ? PARTICLE : FLUID;
                 }
         band_of(hs.sq, c->NX, c->NY, c->NZ, hs.band);
+        c->link_blocks = link_blocks_for(hs.sq.R, c->NX, c->NY, c->NZ);
+        if (c->link_blocks > LINK_MAX_BLOCKS) return cleanup(fail(LBM_HIP_EINVAL, "squirmer radius too large"));
     } else {
         hs.sq.R = 4.0; hs.sq.B1 = 0.015; hs.sq.planar = 1; hs.sq.e[0] = 1.0;
         hs.sq.eps = 0.2; hs.sq.hc = 2.0; hs.sq.hmin = 0.5;
@@ -392,13 +413,17 @@ int lbm_hip_step(lbm_hip_ctx* c, int flags) {
     if (!c) return fail(LBM_HIP_EINVAL, "null context");
     int mode = 0;
     if (flags & LBM_STEP_MACRO) mode |= STEP_MACRO;
-    if (flags & LBM_STEP_WALL_FORCE) mode |= STEP_WALLF;
     if (flags & LBM_STEP_MASS) mode |= STEP_MASS;
     const bool odd = (c->t & 1) != 0;
     hipError_t e = hipSuccess;
     dispatch(c->prec, [&](auto* tag) -> int {
         using S = std::remove_pointer_t<decltype(tag)>;
         e = launch_step<S>(step_args<S>(c), c->K, odd, mode, c->stream);
+        // moving-boundary bounce-back and force/torque on the squirmer
+        if (e == hipSuccess && c->has_squirmer) e = launch_links<S>(link_args<S>(c), c->link_blocks, c->stream);
+        if (e == hipSuccess && (flags & LBM_STEP_WALL_FORCE))
+            e = launch_wall_force<S>((const S*)c->f, c->flag, c->red_part, c->st, c->NX, c->NY, c->NZ, c->N, c->SS,
+                                     c->stream);
         return 0;
     });
     if (e != hipSuccess) return fail(LBM_HIP_EDEVICE, std::string("step launch: ") + hipGetErrorString(e));
@@ -536,11 +561,6 @@ int lbm_hip_get_squirmer(lbm_hip_ctx* c, lbm_squirmer_state* o) {
         o->X[d] = s.X[d]; o->e[d] = s.e[d]; o->U[d] = s.U[d]; o->W[d] = s.W[d];
         o->Fh[d] = s.Fh[d]; o->Th[d] = s.Th[d]; o->Fc[d] = s.Fc[d]; o->Tc[d] = s.Tc[d]; o->Fw[d] = s.Fw[d];
     }
-    if (c->has_squirmer && c->t > 0) {
-        // LBM::step publishes Fh/Th at its end (lbm.cpp:220-223)
-        const double* acc = hs.acc[(c->t - 1) & 1];
-        for (int d = 0; d < 3; ++d) { o->Fh[d] = acc[3 + d]; o->Th[d] = acc[6 + d]; }
-    }
     o->eps = s.eps; o->hc = s.hc; o->hmin = s.hmin;
     o->contacts = s.contacts;
     o->planar = s.planar; o->plane_xz = s.plane_xz;
@@ -559,6 +579,11 @@ int lbm_hip_set_squirmer(lbm_hip_ctx* c, const lbm_squirmer_state* in) {
     s.eps = in->eps; s.hc = in->hc; s.hmin = in->hmin;
     s.contacts = in->contacts;
     s.planar = in->planar; s.plane_xz = in->plane_xz;
+    if (c->has_squirmer) {
+        const int lb = link_blocks_for(s.R, c->NX, c->NY, c->NZ);
+        if (lb > LINK_MAX_BLOCKS) return fail(LBM_HIP_EINVAL, "squirmer radius too large");
+        c->link_blocks = lb;
+    }
     HIP_TRY(hipMemcpyAsync(&c->st->sq, &s, sizeof s, hipMemcpyHostToDevice, c->stream));
     int band[6];
     band_of(s, c->NX, c->NY, c->NZ, band);
@@ -571,7 +596,7 @@ int lbm_hip_get_wall_force(lbm_hip_ctx* c, double out[3]) {
     if (!c || !out) return fail(LBM_HIP_EINVAL, "null argument");
     if (c->t == 0) { out[0] = out[1] = out[2] = 0.0; return LBM_HIP_OK; }
     if (!c->wallf_valid) return fail(LBM_HIP_ESTATE, "the last step did not accumulate the wall force");
-    HIP_TRY(hipMemcpyAsync(out, c->st->acc[(c->t - 1) & 1], 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(out, c->st->wallForce, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     return LBM_HIP_OK;
 }

@@ -1,0 +1,178 @@
+// links_kernel.hpp -- the moving-boundary bounce-back and momentum exchange
+// of PARTICLE links, and the wall momentum exchange, applied after the fused
+// step to the slots where the step left the bounced populations.
+//
+// After either step parity the bounced population of a link (n, i) with a
+// solid node n + c_i is in slot i of n + c_i, holding the post-collision
+// value f*_i of node n (the wall bounce-back of lbm.cpp:197-199 with u_b = 0).
+// particle_links_kernel turns it into
+//     f_b = f*_i - 6 w_i rho_n (c_i . u_b)          (lbm.cpp:189-199)
+// with u_b the squirmer surface velocity (rigid body + two-mode slip) at the
+// link midpoint, and sums the momentum exchange (Wen et al. 2014)
+//     dp = (f* + f_b) c_i - (f* - f_b) u_b,   torque r_b x dp   (lbm.cpp:202-215)
+// in FP64.  The sums are deterministic: per thread in direction order, per
+// block in a fixed tree, then the last block to finish adds the block
+// partials in block order and publishes Fh, Th (lbm.cpp:220-223).
+#pragma once
+#include "device_state.hpp"
+#include "step_kernel.hpp"
+#include "storage.hpp"
+
+namespace lbmhip {
+
+constexpr int LINK_BLOCK = 256;
+constexpr int LINK_MAX_BLOCKS = 8192;
+
+template <class S>
+struct LinkArgs {
+    S* f;
+    const uint8_t* flag;
+    const void* rho;     // density of this step (double in FP64 mode, float otherwise)
+    DevState* st;
+    double* part;        // LINK_MAX_BLOCKS x 8 partial sums
+    int NX, NY, NZ;
+    uint32_t SS;
+    int walls_y, walls_z;
+};
+
+// Box of nodes that can own a particle link: the sphere's nodes lie within
+// R of X, their neighbours within R + 1 (lbm.cpp:273-275 without the margin).
+struct LinkBox {
+    int x0, y0, z0, wx, wy, wz;
+};
+__device__ inline LinkBox link_box(const DevSquirmer& sq, int NX, int NY, int NZ) {
+    LinkBox b;
+    const int Nd[3] = {NX, NY, NZ};
+    int lo[3], wd[3];
+    for (int d = 0; d < 3; ++d) {
+        lo[d] = (int)floor(sq.X[d] - sq.R) - 1;
+        const int hi = (int)ceil(sq.X[d] + sq.R) + 1;
+        int w = hi - lo[d] + 1;
+        wd[d] = w > Nd[d] ? Nd[d] : w;
+    }
+    b.x0 = lo[0]; b.y0 = lo[1]; b.z0 = lo[2];
+    b.wx = wd[0]; b.wy = wd[1]; b.wz = wd[2];
+    return b;
+}
+
+__device__ inline int wrap_once(int a, int n) {
+    a %= n;
+    return a < 0 ? a + n : a;
+}
+
+// Fixed-order combination of per-block partials by the last block to finish.
+// Returns true in the last block; `out` then holds the sums (thread 0).
+template <int K>
+__device__ bool last_block_sum(double* part, unsigned* ticket, const double (&mine)[K], double (&out)[K],
+                               double* lds) {
+    __shared__ bool is_last;
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < K; ++k) part[(size_t)blockIdx.x * 8 + k] = mine[k];
+        __threadfence();
+        const unsigned t = atomicAdd(ticket, 1u);
+        is_last = t == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!is_last) return false;
+    __threadfence();
+    double v[K];
+    for (int k = 0; k < K; ++k) v[k] = 0.0;
+    for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x)
+        for (int k = 0; k < K; ++k) v[k] += __builtin_nontemporal_load(&part[(size_t)b * 8 + k]);
+    block_reduce<K>(v, lds);
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < K; ++k) out[k] = v[k];
+        *ticket = 0u;  // ready for the next launch
+    }
+    return true;
+}
+
+template <class S>
+__global__ __launch_bounds__(LINK_BLOCK) void particle_links_kernel(const LinkArgs<S> A) {
+    using St = Store<S>;
+    constexpr bool EXACT = std::is_same<S, double>::value;
+    __shared__ double lds[(LINK_BLOCK / 64) * 7];
+    DevState* st = A.st;
+    const DevSquirmer& sq = st->sq;
+    const int NX = A.NX, NY = A.NY, NZ = A.NZ;
+    const LinkBox b = link_box(sq, NX, NY, NZ);
+    const int M = b.wx * b.wy * b.wz;
+    double acc[7] = {0, 0, 0, 0, 0, 0, 0};  // Fx Fy Fz Tx Ty Tz dmass
+    const int k = blockIdx.x * LINK_BLOCK + threadIdx.x;
+    if (k < M) {
+        const int ix = k % b.wx, iy = (k / b.wx) % b.wy, iz = k / (b.wx * b.wy);
+        const int x = wrap_once(b.x0 + ix, NX), y = wrap_once(b.y0 + iy, NY), z = wrap_once(b.z0 + iz, NZ);
+        const uint32_t n = (uint32_t)x + (uint32_t)NX * ((uint32_t)y + (uint32_t)NY * (uint32_t)z);
+        const uint8_t fl = A.flag[n];
+        if ((fl & TYPE_MASK) == FLUID && (fl & BOUNDARY)) {
+            double rho;
+            if constexpr (EXACT) rho = static_cast<const double*>(A.rho)[n];
+            else rho = (double)static_cast<const float*>(A.rho)[n];
+            for (int i = 1; i < Q; ++i) {
+                const int xn = x + CX[i] < 0 ? NX - 1 : (x + CX[i] >= NX ? 0 : x + CX[i]);
+                const int yn = y + CY[i] < 0 ? NY - 1 : (y + CY[i] >= NY ? 0 : y + CY[i]);
+                const int zn = z + CZ[i] < 0 ? NZ - 1 : (z + CZ[i] >= NZ ? 0 : z + CZ[i]);
+                const uint32_t m = (uint32_t)xn + (uint32_t)NX * ((uint32_t)yn + (uint32_t)NY * (uint32_t)zn);
+                if ((A.flag[m] & TYPE_MASK) != PARTICLE) continue;
+                S* slot = A.f + (size_t)i * A.SS + m;
+                const double fpost = to_real<S>(*slot, i);
+                double rb[3], ub[3];
+                link_velocity(sq, NX, NY, NZ, A.walls_y, A.walls_z, x + 0.5 * CX[i], y + 0.5 * CY[i],
+                              z + 0.5 * CZ[i], rb, ub);
+                const double cub = CX[i] * ub[0] + CY[i] * ub[1] + CZ[i] * ub[2];
+                const double fb = fpost - 6.0 * W[i] * rho * cub;
+                *slot = from_real<S>(fb, i);
+                const double s = fpost + fb, dl = fpost - fb;
+                const double dpx = s * CX[i] - dl * ub[0];
+                const double dpy = s * CY[i] - dl * ub[1];
+                const double dpz = s * CZ[i] - dl * ub[2];
+                acc[0] += dpx; acc[1] += dpy; acc[2] += dpz;
+                acc[3] += rb[1] * dpz - rb[2] * dpy;
+                acc[4] += rb[2] * dpx - rb[0] * dpz;
+                acc[5] += rb[0] * dpy - rb[1] * dpx;
+                acc[6] += fb - fpost;
+            }
+        }
+    }
+    (void)EXACT;
+    block_reduce<7>(acc, lds);
+    double tot[7];
+    if (last_block_sum<7>(A.part, &st->link_ticket, acc, tot, lds) && threadIdx.x == 0) {
+        for (int d = 0; d < 3; ++d) { st->sq.Fh[d] = tot[d]; st->sq.Th[d] = tot[3 + d]; }
+        st->link_dmass = tot[6];
+    }
+}
+
+// Wall momentum exchange (lbm.cpp:209, 219): for every WALL link the bounced
+// population equals f*_i, so dp = 2 f*_i c_i.  Full-grid sweep, on request.
+template <class S>
+__global__ __launch_bounds__(256) void wall_force_kernel(const S* __restrict__ f, const uint8_t* __restrict__ flag,
+                                                         double* part, DevState* st, int NX, int NY, int NZ,
+                                                         uint32_t N, uint32_t SS) {
+    __shared__ double lds[4 * 3];
+    const uint32_t n = blockIdx.x * 256u + threadIdx.x;
+    double acc[3] = {0.0, 0.0, 0.0};
+    if (n < N) {
+        const uint8_t fl = flag[n];
+        if ((fl & TYPE_MASK) == FLUID && (fl & BOUNDARY)) {
+            const int x = (int)(n % (uint32_t)NX), y = (int)((n / (uint32_t)NX) % (uint32_t)NY),
+                      z = (int)(n / ((uint32_t)NX * (uint32_t)NY));
+            for (int i = 1; i < Q; ++i) {
+                const int xn = x + CX[i] < 0 ? NX - 1 : (x + CX[i] >= NX ? 0 : x + CX[i]);
+                const int yn = y + CY[i] < 0 ? NY - 1 : (y + CY[i] >= NY ? 0 : y + CY[i]);
+                const int zn = z + CZ[i] < 0 ? NZ - 1 : (z + CZ[i] >= NZ ? 0 : z + CZ[i]);
+                const uint32_t m = (uint32_t)xn + (uint32_t)NX * ((uint32_t)yn + (uint32_t)NY * (uint32_t)zn);
+                if ((flag[m] & TYPE_MASK) != WALL) continue;
+                const double fpost = to_real<S>(f[(size_t)i * SS + m], i);
+                const double s = fpost + fpost;
+                acc[0] += s * CX[i]; acc[1] += s * CY[i]; acc[2] += s * CZ[i];
+            }
+        }
+    }
+    block_reduce<3>(acc, lds);
+    double tot[3];
+    if (last_block_sum<3>(part, &st->wall_ticket, acc, tot, lds) && threadIdx.x == 0)
+        for (int d = 0; d < 3; ++d) st->wallForce[d] = tot[d];
+}
+
+}  // namespace lbmhip

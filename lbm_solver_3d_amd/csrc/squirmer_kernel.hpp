@@ -15,13 +15,14 @@
 //      flags that just changed, write the equilibrium of uncovered nodes
 //      into the slots the next step reads, and refresh BOUNDARY bits.
 //
-// Storage slot of the population f_i(n) that the next step reads
-// ("loc(n, i)"): after an odd step (natural layout) slot i at n; after an
-// even step (shifted layout) slot opp(i) at n - c_i when that node is FLUID,
-// else slot i at n.  In the shifted layout a flag change therefore moves
-// the slots of the links between the changed node and its unchanged FLUID
-// neighbours; phase D applies exactly those moves.  Every such link owns a
-// pair of slots no other link touches, so phase D runs without ordering.
+// Storage slot of the population f_j(n) that the next step reads
+// ("loc(n, j)", see step_kernel.hpp): after an even step slot opp(j) at
+// n - c_j whatever the flags, so no slot moves; after an odd step slot j at n
+// when n - c_j is FLUID, else slot opp(j) at n - c_j.  After an odd step a
+// flag change therefore moves the slots of the links between the changed
+// node and its unchanged FLUID neighbours; phase D applies exactly those
+// moves.  Every such link owns a pair of slots no other link touches, so
+// phase D runs without ordering.
 #pragma once
 #include "device_state.hpp"
 #include "step_kernel.hpp"
@@ -47,7 +48,6 @@ struct RemapArgs {
     uint32_t SS;     // slot stride (elements between populations i and i+1)
     int walls_y, walls_z;
     int shifted;     // layout after the last step: 1 after an even step
-    int acc_slot;    // accumulator slot of the last step
     int shift_slot;  // mass-shift slot the next step reads
 };
 
@@ -106,8 +106,7 @@ __global__ __launch_bounds__(SQ_THREADS) void squirmer_kernel(const RemapArgs<S>
 
     // ---------------- A: rigid body (lbm.cpp:257-262) ----------------
     if (tid == 0) {
-        const double* acc = st->acc[A.acc_slot];
-        for (int d = 0; d < 3; ++d) { sq.Fh[d] = acc[3 + d]; sq.Th[d] = acc[6 + d]; }
+        // sq.Fh / sq.Th: published by particle_links_kernel after the step
         // applyWallPotential (lbm.cpp:231-241)
         sq.Fw[0] = sq.Fw[1] = sq.Fw[2] = 0.0;
         if (A.walls_y) {
@@ -279,14 +278,18 @@ __global__ __launch_bounds__(SQ_THREADS) void squirmer_kernel(const RemapArgs<S>
                 double fv = 0.0;
                 if (i < Q) {
                     size_t slot = (size_t)i * SS + n;
-                    if (A.shifted && i > 0) {
+                    if (i > 0) {
                         const uint32_t m = (uint32_t)wrapi(x - cxi, NX) +
                                            (uint32_t)NX * ((uint32_t)wrapi(y - cyi, NY) +
                                                            (uint32_t)NY * (uint32_t)wrapi(z - czi, NZ));
-                        const uint8_t fm = A.flag[m];
-                        uint8_t tm = fm & TYPE_MASK;
-                        if (fm & FLIPPED) tm = (tm == FLUID) ? PARTICLE : FLUID;
-                        if (tm == FLUID) slot = (size_t)OPP[i] * SS + m;
+                        bool up_fluid = false;
+                        if (!A.shifted) {
+                            const uint8_t fm = A.flag[m];
+                            uint8_t tm = fm & TYPE_MASK;
+                            if (fm & FLIPPED) tm = (tm == FLUID) ? PARTICLE : FLUID;  // flags before this remap
+                            up_fluid = tm == FLUID;
+                        }
+                        if (!up_fluid) slot = (size_t)OPP[i] * SS + m;
                     }
                     fv = to_real<S>(A.f[slot], i);
                     if (dsh != 0.0) fv += W[i] * dsh;
@@ -358,24 +361,28 @@ __global__ __launch_bounds__(SQ_THREADS) void squirmer_kernel(const RemapArgs<S>
         const uint32_t e = list[k];
         const uint32_t n = e & 0x7FFFFFFFu;
         const bool cover = (e >> 31) != 0;
-        if (!A.shifted || i == 0) {
-            if (!cover) f[(size_t)i * SS + n] = from_real<S>(equilibrium(i, lrr[k], lub[k][0], lub[k][1], lub[k][2]), i);
+        if (i == 0) {
+            if (!cover) f[n] = from_real<S>(equilibrium(0, lrr[k], lub[k][0], lub[k][1], lub[k][2]), 0);
             continue;
         }
         const int x = (int)(n % (uint32_t)NX), y = (int)((n / (uint32_t)NX) % (uint32_t)NY),
                   z = (int)(n / ((uint32_t)NX * (uint32_t)NY));
         const uint32_t m = (uint32_t)wrapi(x - CX[i], NX) +
                            (uint32_t)NX * ((uint32_t)wrapi(y - CY[i], NY) + (uint32_t)NY * (uint32_t)wrapi(z - CZ[i], NZ));
+        const size_t own = (size_t)i * SS + n, far = (size_t)OPP[i] * SS + m;
+        if (A.shifted) {  // loc(n, i) = slot opp(i) at n - c_i: nothing moves
+            if (!cover) f[far] = from_real<S>(equilibrium(i, lrr[k], lub[k][0], lub[k][1], lub[k][2]), i);
+            continue;
+        }
         const uint8_t fm = A.flag[m];
         const bool mfluid = (fm & TYPE_MASK) == FLUID, mflip = (fm & FLIPPED) != 0;
-        const size_t sin_ = (size_t)i * SS + n, sout = (size_t)OPP[i] * SS + m;
         if (!cover) {
             const S fe = from_real<S>(equilibrium(i, lrr[k], lub[k][0], lub[k][1], lub[k][2]), i);
-            if (mfluid && !mflip) { f[sin_] = f[sout]; f[sout] = fe; }
-            else if (mfluid) f[sout] = fe;
-            else f[sin_] = fe;
+            if (mfluid && !mflip) { f[far] = f[own]; f[own] = fe; }  // m's bounced f_opp(i) back to m
+            else if (mfluid) f[own] = fe;
+            else f[far] = fe;
         } else {
-            if (mfluid && !mflip) f[sout] = f[sin_];
+            if (mfluid && !mflip) f[own] = f[far];  // m's f_opp(i) now bounces off n
         }
     }
     __syncthreads();
@@ -421,7 +428,8 @@ __global__ __launch_bounds__(SQ_THREADS) void squirmer_kernel(const RemapArgs<S>
 }
 
 // ---------------------------------------------------------------------------
-// Logical population f_i(n) for download / upload (any layout).
+// Logical population f_i(n) for download / upload (either layout; for
+// non-fluid nodes slot i at n).
 // ---------------------------------------------------------------------------
 template <class S>
 __global__ __launch_bounds__(256) void gather_kernel(const S* __restrict__ f, const uint8_t* __restrict__ flag,
@@ -435,10 +443,10 @@ __global__ __launch_bounds__(256) void gather_kernel(const S* __restrict__ f, co
     const double dsh = st->shift[shift_slot];
     for (int i = 0; i < Q; ++i) {
         size_t slot = (size_t)i * SS + n;
-        if (fluid && shifted && i > 0) {
+        if (fluid && i > 0) {
             const uint32_t m = (uint32_t)wrapi(x - CX[i], NX) +
                                (uint32_t)NX * ((uint32_t)wrapi(y - CY[i], NY) + (uint32_t)NY * (uint32_t)wrapi(z - CZ[i], NZ));
-            if ((flag[m] & TYPE_MASK) == FLUID) slot = (size_t)OPP[i] * SS + m;
+            if (shifted || (flag[m] & TYPE_MASK) != FLUID) slot = (size_t)OPP[i] * SS + m;
         }
         double v = to_real<S>(f[slot], i);
         if (fluid && dsh != 0.0) v += W[i] * dsh;
@@ -454,13 +462,16 @@ __global__ __launch_bounds__(256) void scatter_kernel(S* __restrict__ f, const u
     if (n >= N) return;
     const int x = (int)(n % (uint32_t)NX), y = (int)((n / (uint32_t)NX) % (uint32_t)NY),
               z = (int)(n / ((uint32_t)NX * (uint32_t)NY));
+    // only FLUID nodes own populations: the slots of solid nodes may hold the
+    // bounced populations of their fluid neighbours
     const bool fluid = (flag[n] & TYPE_MASK) == FLUID;
+    if (!fluid) return;
     for (int i = 0; i < Q; ++i) {
         size_t slot = (size_t)i * SS + n;
-        if (fluid && shifted && i > 0) {
+        if (fluid && i > 0) {
             const uint32_t m = (uint32_t)wrapi(x - CX[i], NX) +
                                (uint32_t)NX * ((uint32_t)wrapi(y - CY[i], NY) + (uint32_t)NY * (uint32_t)wrapi(z - CZ[i], NZ));
-            if ((flag[m] & TYPE_MASK) == FLUID) slot = (size_t)OPP[i] * SS + m;
+            if (shifted || (flag[m] & TYPE_MASK) != FLUID) slot = (size_t)OPP[i] * SS + m;
         }
         f[slot] = from_real<S>(in[(size_t)i * N + n], i);
     }
@@ -482,10 +493,10 @@ __global__ __launch_bounds__(256) void mass_kernel(const S* __restrict__ f, cons
         double s = 0.0;
         for (int i = 0; i < Q; ++i) {
             size_t slot = (size_t)i * SS + n;
-            if (shifted && i > 0) {
+            if (i > 0) {
                 const uint32_t m = (uint32_t)wrapi(x - CX[i], NX) +
                                    (uint32_t)NX * ((uint32_t)wrapi(y - CY[i], NY) + (uint32_t)NY * (uint32_t)wrapi(z - CZ[i], NZ));
-                if ((flag[m] & TYPE_MASK) == FLUID) slot = (size_t)OPP[i] * SS + m;
+                if (shifted || (flag[m] & TYPE_MASK) != FLUID) slot = (size_t)OPP[i] * SS + m;
             }
             double v = to_real<S>(f[slot], i);
             if (dsh != 0.0) v += W[i] * dsh;

@@ -3,31 +3,33 @@
 // One thread per lattice node, one population array updated in place with
 // the AA access pattern (P. Bailey et al., ICPP 2009):
 //
-//   even step t: node n reads f_i from slot i at n, collides and writes
-//                f*_i into slot opp(i) at n            (all accesses local)
-//   odd step t:  node n reads f_i from slot opp(i) at n - c_i, collides and
-//                writes f*_i into slot i at n + c_i    (all accesses remote)
+//   even step: node n reads f_i from slot i at n, collides and writes f*_i
+//              into slot opp(i) at n                  (all accesses local)
+//   odd step:  node n reads f_i from slot opp(i) at n - c_i, collides and
+//              writes f*_i into slot i at n + c_i     (all accesses remote)
 //
-// After an odd step the array is in the natural layout (slot i at n holds
-// the f_i that node n collides next), after an even step it is "shifted".
 // Each slot is read and written by one thread only, so the update is race
-// free without a second array.  This is the reference's push step
-// (src/utils/lbm.cpp:128-224) with the same post-collision values; only the
-// places the values are kept differ.
+// free in place.  The post-collision values are the reference's
+// (src/utils/lbm.cpp:128-224); only the places they are kept differ.
 //
-// Halfway bounce-back is folded into the same stores.  A link (n, i) with a
-// non-fluid neighbour n + c_i returns f*_i - 6 w_i rho (c_i . u_b) as
-// f_opp(i)(n) of the next step (lbm.cpp:189-199); the value goes to slot
-// opp(i) at n in both parities, which is exactly where the next step's node
-// n reads f_opp(i) when its upstream neighbour is solid.  Node n only looks
-// at neighbour flags when its own flag byte carries the BOUNDARY bit, so
-// bulk wavefronts run the plain stream code.
+// Halfway bounce-back.  The odd step stays uniform: a link (n, i) whose
+// neighbour n + c_i is solid still writes f*_i into slot i at n + c_i, i.e.
+// into the solid node's own slot, which no fluid node uses otherwise.  The
+// even step then reads a population whose upstream node is solid from that
+// place (slot opp(j) at n - c_j) and, for a solid downstream node, stores the
+// bounced value into slot i at n + c_i, where the next odd step's node n
+// reads f_opp(i).  So the bounced value of link (n, i) always sits in slot i
+// of the solid node n + c_i after either parity; the wall bounce-back is that
+// plain copy (lbm.cpp:197-199 with u_b = 0), and the moving-boundary
+// correction and the momentum exchange of PARTICLE links are applied to
+// those slots afterwards by particle_links_kernel.  Only even-step lanes
+// whose flag byte carries BOUNDARY touch neighbour flags, and they do so in
+// addition to the uniform accesses, which every lane performs.
 //
-// Momentum exchange (lbm.cpp:202-215): PARTICLE links accumulate force and
-// torque in FP64 per thread, then per block through wavefront reductions and
-// LDS, then one FP64 atomic per component per block into the step's
-// accumulator.  WALL links do the same on request (the wall force is only a
-// diagnostic).
+// Storage slot of the population f_j(n) that node n collides next
+// ("loc(n, j)"): after an even step ("shifted" layout) slot opp(j) at
+// n - c_j, whatever the flags; after an odd step ("natural" layout) slot j at
+// n if n - c_j is FLUID, else slot opp(j) at n - c_j.
 #pragma once
 #include <type_traits>
 
@@ -218,32 +220,6 @@ __device__ __forceinline__ void link_velocity(const DevSquirmer& sq, int NX, int
     ub[2] += amp * (ct * rh[2] - sq.e[2]);
 }
 
-// Moving-boundary bounce-back of one particle link (lbm.cpp:189-215), FP64:
-// f_b and the link's momentum exchange and torque.  Kept out of line:
-// particle links are a thin shell of nodes.
-struct LinkResult {
-    double fb, dp[3], tq[3];
-};
-__device__ __noinline__ LinkResult particle_link(const DevSquirmer& sq, int NX, int NY, int NZ, int wallsY,
-                                                 int wallsZ, int x, int y, int z, int cx, int cy, int cz, double w6,
-                                                 double fpost, double rho) {
-    double rb[3], ub[3];
-    link_velocity(sq, NX, NY, NZ, wallsY, wallsZ, x + 0.5 * cx, y + 0.5 * cy, z + 0.5 * cz, rb, ub);
-    const double cub = cx * ub[0] + cy * ub[1] + cz * ub[2];
-    const double fb = fpost - w6 * rho * cub;
-    const double s = fpost + fb, dl = fpost - fb;
-    const double dpx = s * cx - dl * ub[0];
-    const double dpy = s * cy - dl * ub[1];
-    const double dpz = s * cz - dl * ub[2];
-    LinkResult o;
-    o.fb = fb;
-    o.dp[0] = dpx; o.dp[1] = dpy; o.dp[2] = dpz;
-    o.tq[0] = rb[1] * dpz - rb[2] * dpy;
-    o.tq[1] = rb[2] * dpx - rb[0] * dpz;
-    o.tq[2] = rb[0] * dpy - rb[1] * dpx;
-    return o;
-}
-
 // Wavefront + block reduction of K doubles; thread 0 of the block gets the sums.
 template <int K>
 __device__ __forceinline__ void block_reduce(double (&v)[K], double* lds) {
@@ -274,18 +250,21 @@ __device__ __forceinline__ void block_reduce(double (&v)[K], double* lds) {
 // Addressing: a wavefront is 64 consecutive x of one (y, z) row (tiles are
 // at least 64 wide), so the row offsets of the 9 (dy, dz) neighbour rows are
 // wavefront-uniform and live in scalar registers together with the 19 slot
-// bases; a lane only holds its three x offsets (x-1, x, x+1, periodic).
-// Every load and store is then one global access with a scalar base and a
-// 32-bit lane offset.  Only odd-step wavefronts that contain BOUNDARY nodes
-// take the per-lane path that redirects solid links to the node's own slots.
+// bases; a lane only holds its three x byte offsets (x-1, x, x+1, periodic).
+// Every uniform load and store is one global access with a scalar base and a
+// 32-bit lane offset.
 // ---------------------------------------------------------------------------
+#ifndef LBM_STEP_WAVES_PER_EU
+#define LBM_STEP_WAVES_PER_EU 4
+#endif
+
 template <class S, bool ODD, int MODE>
-__global__ __launch_bounds__(BLOCK) void aa_step_kernel(const StepArgs<S> A, const FastCoef K) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(LBM_STEP_WAVES_PER_EU)))
+void aa_step_kernel(const StepArgs<S> A, const FastCoef K) {
     using St = Store<S>;
     using T = typename St::C;
     constexpr bool EXACT = std::is_same<S, double>::value;
     constexpr bool MACRO = (MODE & STEP_MACRO) != 0;
-    constexpr bool WALLF = (MODE & STEP_WALLF) != 0;
     constexpr bool MASS = (MODE & STEP_MASS) != 0;
 
     // ---- tile: the hardware deals blocks to the 8 XCDs round robin; give
@@ -308,64 +287,50 @@ __global__ __launch_bounds__(BLOCK) void aa_step_kernel(const StepArgs<S> A, con
     // wavefront-uniform row offsets of the neighbour rows (dy, dz in -1..1)
     const uint32_t yo[3] = {(uy_ == 0 ? NY - 1 : uy_ - 1) * NX, uy_ * NX, (uy_ + 1 == NY ? 0u : uy_ + 1) * NX};
     const uint32_t zo[3] = {(uz_ == 0 ? NZ - 1 : uz_ - 1) * NXY, uz_ * NXY, (uz_ + 1 == NZ ? 0u : uz_ + 1) * NXY};
-    // lane x offsets (x-1, x, x+1 with periodic wrap)
+    // lane x offsets (x-1, x, x+1 with periodic wrap), as element and byte offsets
     const uint32_t ux_ = inb ? (uint32_t)x : 0u;
     const uint32_t xo[3] = {ux_ == 0 ? NX - 1 : ux_ - 1, ux_, ux_ + 1 == NX ? 0u : ux_ + 1};
+    const uint32_t xb[3] = {xo[0] * (uint32_t)sizeof(S), xo[1] * (uint32_t)sizeof(S), xo[2] * (uint32_t)sizeof(S)};
     const uint32_t n = ux_ + yo[1] + zo[1];
 
     const uint8_t fl = inb ? A.flag[n] : (uint8_t)WALL;
     const bool active = (fl & TYPE_MASK) == FLUID;
-    const bool bnd = active && (fl & BOUNDARY);
 
     if (tile == 0 && threadIdx.x == 0) {
-        // clear the other parity's accumulators for step t+1
-        const int o = (int)((A.t + 1) & 1);
-        for (int k = 0; k < 9; ++k) A.st->acc[o][k] = 0.0;
-        A.st->shift[o] = 0.0;
+        // clear the slot step t+1 reads its mass shift from, and the remap's
+        // mass bookkeeping of the previous step
+        A.st->shift[(A.t + 1) & 1] = 0.0;
         A.st->remap_dmass = 0.0;
     }
 
-    // solid / particle neighbour masks (bit i: n + c_i)
-    uint32_t solid = 0, part = 0;
-    if (bnd) {
-        static_for<1, Q>([&](auto I) {
-            constexpr int i = decltype(I)::value;
-            const uint32_t m = xo[1 + CX[i]] + yo[1 + CY[i]] + zo[1 + CZ[i]];
-            const uint8_t tt = A.flag[m] & TYPE_MASK;
-            solid |= (uint32_t)(tt != FLUID) << i;
-            part |= (uint32_t)(tt == PARTICLE) << i;
-        });
-    }
-    // odd steps redirect solid links per lane: wavefronts without boundary
-    // nodes keep scalar-base addressing throughout
-    const bool wave_bnd = ODD && __builtin_amdgcn_ballot_w64(bnd) != 0;
-
     S* __restrict__ f = A.f;
-    // scalar base of population slot `s` shifted by a row offset, and lane
-    // byte offsets (32-bit, so every access is scalar base + lane offset)
+    // scalar base of population slot `s` shifted by a row offset
     auto base = [&](int s, uint32_t row) -> char* { return (char*)(f + ((size_t)s * SS + row)); };
-    const uint32_t xb[3] = {xo[0] * (uint32_t)sizeof(S), xo[1] * (uint32_t)sizeof(S), xo[2] * (uint32_t)sizeof(S)};
     auto at = [](char* b, uint32_t off) -> S* { return (S*)(b + off); };
 
-    double facc[9];
-    for (int k = 0; k < 9; ++k) facc[k] = 0.0;
     double excess = 0.0;
-
     if (active) {
         // ---- gather the 19 populations node n collides ----
         T g[Q];
-        if (!wave_bnd) {
-            static_for<0, Q>([&](auto I) {
+        static_for<0, Q>([&](auto I) {
+            constexpr int i = decltype(I)::value;
+            if constexpr (!ODD) g[i] = St::load(*at(base(i, yo[1] + zo[1]), xb[1]));
+            else g[i] = St::load(*at(base(OPP[i], yo[1 - CY[i]] + zo[1 - CZ[i]]), xb[1 - CX[i]]));
+        });
+
+        // even step, boundary node: populations whose upstream node is solid
+        // come back from that node's slot opp(j) (bounce-back)
+        uint32_t solid = 0;
+        if (!ODD && (fl & BOUNDARY)) {
+            static_for<1, Q>([&](auto I) {
                 constexpr int i = decltype(I)::value;
-                if constexpr (!ODD) g[i] = St::load(*at(base(i, yo[1] + zo[1]), xb[1]));
-                else g[i] = St::load(*at(base(OPP[i], yo[1 - CY[i]] + zo[1 - CZ[i]]), xb[1 - CX[i]]));
+                const uint32_t m = xo[1 + CX[i]] + yo[1 + CY[i]] + zo[1 + CZ[i]];
+                solid |= (uint32_t)((A.flag[m] & TYPE_MASK) != FLUID) << i;
             });
-        } else {
-            static_for<0, Q>([&](auto I) {
-                constexpr int i = decltype(I)::value;
-                const S* p = at(base(OPP[i], yo[1 - CY[i]] + zo[1 - CZ[i]]), xb[1 - CX[i]]);
-                if ((solid >> OPP[i]) & 1u) p = at(base(i, yo[1] + zo[1]), xb[1]);
-                g[i] = St::load(*p);
+            static_for<1, Q>([&](auto I) {
+                constexpr int j = decltype(I)::value;
+                if ((solid >> OPP[j]) & 1u)
+                    g[j] = St::load(*at(base(OPP[j], yo[1 - CY[j]] + zo[1 - CZ[j]]), xb[1 - CX[j]]));
             });
         }
 
@@ -380,12 +345,11 @@ __global__ __launch_bounds__(BLOCK) void aa_step_kernel(const StepArgs<S> A, con
         }
 
         // ---- collide ----
-        double rho_d;
         T rho_t, m0, m1, m2;  // macroscopic outputs (real units)
         if constexpr (EXACT) {
             double r, u, v, wz;
             collide_exact(g, A.omPlus, A.omMinus, A.Fx, A.Fy, A.Fz, r, u, v, wz);
-            rho_d = r; rho_t = r; m0 = u; m1 = v; m2 = wz;
+            rho_t = r; m0 = u; m1 = v; m2 = wz;
             if constexpr (MASS) {
                 double s = 0.0;
                 static_for<0, Q>([&](auto I) { s += g[decltype(I)::value]; });
@@ -395,7 +359,7 @@ __global__ __launch_bounds__(BLOCK) void aa_step_kernel(const StepArgs<S> A, con
             float rho, jx, jy, jz, drs;
             if (A.Fx != 0.0 || A.Fy != 0.0 || A.Fz != 0.0) collide_fast<true>(g, K, rho, jx, jy, jz, drs);
             else collide_fast<false>(g, K, rho, jx, jy, jz, drs);
-            rho_d = rho; rho_t = rho;
+            rho_t = rho;
             if constexpr (MACRO) {
                 const float isr = K.inv_s / rho;
                 m0 = jx * isr; m1 = jy * isr; m2 = jz * isr;
@@ -406,7 +370,8 @@ __global__ __launch_bounds__(BLOCK) void aa_step_kernel(const StepArgs<S> A, con
         }
 
         // ---- macroscopic fields: all fluid nodes on request, else the
-        // band around the squirmer that the remap reads (lbm.cpp:152, 307)
+        // band around the squirmer that the remap and the link kernel read
+        // (lbm.cpp:152, 198, 307)
         if constexpr (MACRO) {
             static_cast<T*>(A.rho)[n] = rho_t;
             static_cast<T*>(A.ux)[n] = m0;
@@ -422,75 +387,24 @@ __global__ __launch_bounds__(BLOCK) void aa_step_kernel(const StepArgs<S> A, con
             }
         }
 
-        // ---- moving-boundary bounce-back on particle links (FP64) ----
-        if (bnd && part) {
-            const DevSquirmer& sq = A.st->sq;
+        // ---- stream ----
+        static_for<0, Q>([&](auto I) {
+            constexpr int i = decltype(I)::value;
+            const S val = St::store(g[i]);
+            if constexpr (!ODD) *at(base(OPP[i], yo[1] + zo[1]), xb[1]) = val;
+            else *at(base(i, yo[1 + CY[i]] + zo[1 + CZ[i]]), xb[1 + CX[i]]) = val;
+        });
+        // even step, boundary node: bounce-back into the solid neighbour's slot i
+        if (!ODD && solid) {
             static_for<1, Q>([&](auto I) {
                 constexpr int i = decltype(I)::value;
-                if ((part >> i) & 1u) {
-                    double fpost;
-                    if constexpr (EXACT) fpost = g[i];
-                    else fpost = W[i] + (double)g[i] * (1.0 / St::scale);
-                    const LinkResult lr = particle_link(sq, A.NX, A.NY, A.NZ, A.walls_y, A.walls_z, x, y, z, CX[i],
-                                                        CY[i], CZ[i], 6.0 * W[i], fpost, rho_d);
-                    const double fb = lr.fb;
-                    facc[3] += lr.dp[0]; facc[4] += lr.dp[1]; facc[5] += lr.dp[2];
-                    facc[6] += lr.tq[0]; facc[7] += lr.tq[1]; facc[8] += lr.tq[2];
-                    if constexpr (MASS) excess += fb - fpost;
-                    if constexpr (EXACT) g[i] = fb;
-                    else g[i] = (T)((fb - W[i]) * St::scale);
-                }
-            });
-        }
-        if constexpr (WALLF) {
-            if (bnd) {
-                const uint32_t wl = solid & ~part;
-                static_for<1, Q>([&](auto I) {
-                    constexpr int i = decltype(I)::value;
-                    if ((wl >> i) & 1u) {
-                        double fpost;
-                        if constexpr (EXACT) fpost = g[i];
-                        else fpost = W[i] + (double)g[i] * (1.0 / St::scale);
-                        const double s = fpost + fpost;  // u_b = 0: f_b = f*
-                        if constexpr (CX[i] != 0) facc[0] += s * CX[i];
-                        if constexpr (CY[i] != 0) facc[1] += s * CY[i];
-                        if constexpr (CZ[i] != 0) facc[2] += s * CZ[i];
-                    }
-                });
-            }
-        }
-
-        // ---- stream ----
-        if (!wave_bnd) {
-            static_for<0, Q>([&](auto I) {
-                constexpr int i = decltype(I)::value;
-                const S val = St::store(g[i]);
-                if constexpr (!ODD) *at(base(OPP[i], yo[1] + zo[1]), xb[1]) = val;
-                else *at(base(i, yo[1 + CY[i]] + zo[1 + CZ[i]]), xb[1 + CX[i]]) = val;
-            });
-        } else {
-            static_for<0, Q>([&](auto I) {
-                constexpr int i = decltype(I)::value;
-                const S val = St::store(g[i]);
-                S* p = at(base(i, yo[1 + CY[i]] + zo[1 + CZ[i]]), xb[1 + CX[i]]);
-                if ((solid >> i) & 1u) p = at(base(OPP[i], yo[1] + zo[1]), xb[1]);
-                *p = val;
+                if ((solid >> i) & 1u) *at(base(i, yo[1 + CY[i]] + zo[1 + CZ[i]]), xb[1 + CX[i]]) = St::store(g[i]);
             });
         }
     }
 
-    // ---- block reductions ----
-    __shared__ double lds[(BLOCK / 64) * 9];
-    const bool links = (part != 0u) || (WALLF && solid != 0u);
-    if (__syncthreads_or(links)) {
-        block_reduce<9>(facc, lds);
-        if (threadIdx.x == 0) {
-            double* acc = A.st->acc[A.t & 1];
-            for (int k = 0; k < 9; ++k)
-                if (facc[k] != 0.0) atomicAdd(&acc[k], facc[k]);
-        }
-    }
     if constexpr (MASS) {
+        __shared__ double lds[BLOCK / 64];
         double m[1] = {excess};
         block_reduce<1>(m, lds);
         if (threadIdx.x == 0) A.mass_part[tile] = m[0];
