@@ -41,6 +41,12 @@ struct DevState {
     int band[6];        // x0, y0, z0 (wrapped) and widths of the rho band
     int err;            // device error bits (ERR_*)
     int changed;        // nodes changed by the last remap
+    // Centre and radius the PARTICLE flags were last classified with: every
+    // non-WALL node of the remap box is PARTICLE exactly when it lies within
+    // rf of xf (set at creation and by each remap; a host-side squirmer
+    // update leaves it alone).
+    double xf[3];
+    double rf;
 };
 
 constexpr int ERR_REMAP_OVERFLOW = 1;

@@ -6,4 +6,10 @@ namespace lbmhip {
 template hipError_t launch_step<float>(const StepArgs<float>&, const FastCoef&, bool, int, hipStream_t);
 template hipError_t launch_step<_Float16>(const StepArgs<_Float16>&, const FastCoef&, bool, int, hipStream_t);
 template hipError_t launch_step<bf16_t>(const StepArgs<bf16_t>&, const FastCoef&, bool, int, hipStream_t);
+template hipError_t launch_boundary<float>(const StepArgs<float>&, const FastCoef&, const BoundaryArgs&, int, int,
+                                            hipStream_t);
+template hipError_t launch_boundary<_Float16>(const StepArgs<_Float16>&, const FastCoef&, const BoundaryArgs&, int, int,
+                                            hipStream_t);
+template hipError_t launch_boundary<bf16_t>(const StepArgs<bf16_t>&, const FastCoef&, const BoundaryArgs&, int, int,
+                                            hipStream_t);
 }  // namespace lbmhip

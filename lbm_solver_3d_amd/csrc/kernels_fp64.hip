@@ -86,6 +86,8 @@ LBMHIP_INSTANTIATE(bf16_t)
 #undef LBMHIP_INSTANTIATE
 
 template hipError_t launch_step<double>(const StepArgs<double>&, const FastCoef&, bool, int, hipStream_t);
+template hipError_t launch_boundary<double>(const StepArgs<double>&, const FastCoef&, const BoundaryArgs&, int, int,
+                                            hipStream_t);
 
 // ---------------------------------------------------------------------------
 // Mass correction (lbm.cpp:324-339): d = 1 - m / n_fluid with

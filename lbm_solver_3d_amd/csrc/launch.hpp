@@ -17,6 +17,10 @@ template <class S>
 hipError_t launch_step(const StepArgs<S>& a, const FastCoef& k, bool odd, int mode, hipStream_t s);
 
 template <class S>
+hipError_t launch_boundary(const StepArgs<S>& a, const FastCoef& k, const BoundaryArgs& b, int blocks, int mode,
+                           hipStream_t s);
+
+template <class S>
 hipError_t launch_squirmer(const RemapArgs<S>& a, hipStream_t s);
 
 template <class S>
@@ -51,6 +55,8 @@ hipError_t launch_macro_to_double(const void* src, int is_double, double* dst, u
 
 #define LBMHIP_DECLARE(S)                                                                                          \
     extern template hipError_t launch_step<S>(const StepArgs<S>&, const FastCoef&, bool, int, hipStream_t);     \
+    extern template hipError_t launch_boundary<S>(const StepArgs<S>&, const FastCoef&, const BoundaryArgs&, int, \
+                                                  int, hipStream_t);                                             \
     extern template hipError_t launch_squirmer<S>(const RemapArgs<S>&, hipStream_t);                            \
     extern template hipError_t launch_links<S>(const LinkArgs<S>&, int, hipStream_t);                           \
     extern template hipError_t launch_wall_force<S>(const S*, const uint8_t*, double*, DevState*, int, int, int, \

@@ -27,4 +27,23 @@ hipError_t launch_step(const StepArgs<S>& a, const FastCoef& k, bool odd, int mo
     return odd ? launch_step_p<S, true>(a, k, mode, s) : launch_step_p<S, false>(a, k, mode, s);
 }
 
+template <class S, int MODE>
+static hipError_t launch_boundary_v(const StepArgs<S>& a, const FastCoef& k, const BoundaryArgs& b, int blocks,
+                                    hipStream_t s) {
+    if (blocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL((aa_boundary_kernel<S, MODE>), dim3(blocks), dim3(BLOCK), 0, s, a, k, b);
+    return hipGetLastError();
+}
+
+template <class S>
+hipError_t launch_boundary(const StepArgs<S>& a, const FastCoef& k, const BoundaryArgs& b, int blocks, int mode,
+                           hipStream_t s) {
+    switch (mode & (STEP_MACRO | STEP_MASS)) {
+        case 0: return launch_boundary_v<S, 0>(a, k, b, blocks, s);
+        case STEP_MACRO: return launch_boundary_v<S, STEP_MACRO>(a, k, b, blocks, s);
+        case STEP_MASS: return launch_boundary_v<S, STEP_MASS>(a, k, b, blocks, s);
+        default: return launch_boundary_v<S, STEP_MACRO | STEP_MASS>(a, k, b, blocks, s);
+    }
+}
+
 }  // namespace lbmhip

@@ -64,6 +64,9 @@ struct lbm_hip_ctx {
     double* diag_out = nullptr;   // 4 doubles
     double* red_part = nullptr;   // 8 * max(LINK_MAX_BLOCKS, ceil(N/256)) doubles (link / wall kernels)
     int link_blocks = 0;
+    uint32_t* wall_rows = nullptr;  // rows next to a wall (even-step boundary chunks)
+    BoundaryArgs bargs{};
+    int bblocks = 0;                // blocks of aa_boundary_kernel
     int gx = 0, gy = 0, gz = 0, tx_shift = 8, ty = 1, G = 0, nb256 = 0;
     double omPlus = 0, omMinus = 0, nu = 0;
     FastCoef K{};
@@ -72,6 +75,7 @@ struct lbm_hip_ctx {
     bool macro_valid = true;  // initial rho = 1, u = 0 are the reference's values
     bool wallf_valid = false;
     bool mass_valid = false;
+    int last_bparts = 0;  // boundary-kernel mass partials written by the last step
     size_t device_bytes = 0;
     std::vector<hipEvent_t> events;
     // one captured cycle of the driver loop (step, advance, mass correction)
@@ -247,7 +251,7 @@ void free_ctx(lbm_hip_ctx* c) {
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
     for (void* q : {c->f, (void*)c->flag, c->rho, c->ux, c->uy, c->uz, (void*)c->st, (void*)c->mass_part,
-                    (void*)c->diag_part, (void*)c->diag_out, (void*)c->red_part})
+                    (void*)c->diag_part, (void*)c->diag_out, (void*)c->red_part, (void*)c->wall_rows})
         if (q) hipFree(q);
     for (hipEvent_t e : c->events) hipEventDestroy(e);
     if (c->graph) hipGraphExecDestroy(c->graph);
@@ -304,6 +308,31 @@ int lbm_hip_create(const lbm_hip_params* pin, lbm_hip_ctx** out) {
     c->G = c->gx * c->gy * c->gz;
     c->nb256 = (int)((c->N + 255) / 256);
 
+    // even-step boundary chunks: rows next to a wall, and the squirmer's box rows
+    std::vector<uint32_t> wall_rows;
+    for (int z = 0; z < pin->nz; ++z)
+        for (int y = 0; y < pin->ny; ++y) {
+            const bool solid_row = (pin->walls_y && (y == 0 || y == pin->ny - 1)) ||
+                                   (pin->walls_z && (z == 0 || z == pin->nz - 1));
+            const bool next_to_wall = (pin->walls_y && (y == 1 || y == pin->ny - 2)) ||
+                                      (pin->walls_z && (z == 1 || z == pin->nz - 2));
+            if (!solid_row && next_to_wall) wall_rows.push_back((uint32_t)y + (uint32_t)pin->ny * (uint32_t)z);
+        }
+    {
+        BoundaryArgs& b = c->bargs;
+        b.n_wall_rows = (int)wall_rows.size();
+        b.nchx = (pin->nx + 63) / 64;
+        b.wy_lo = pin->walls_y ? 1 : -1; b.wy_hi = pin->walls_y ? pin->ny - 2 : -1;
+        b.wz_lo = pin->walls_z ? 1 : -1; b.wz_hi = pin->walls_z ? pin->nz - 2 : -1;
+        if (pin->squirmer) {
+            const int w = (int)std::ceil(2.0 * pin->radius) + 5;
+            b.box_rows_max = std::min(w, pin->ny) * std::min(w, pin->nz);
+            b.box_chunks = std::min(b.nchx, (w + 63) / 64 + 2);
+        }
+        const long long waves = (long long)b.n_wall_rows * b.nchx + (long long)b.box_rows_max * b.box_chunks;
+        c->bblocks = (int)((waves + (BLOCK / 64) - 1) / (BLOCK / 64));
+    }
+
     auto cleanup = [&](int code) {
         std::string m = g_err;
         free_ctx(c);
@@ -320,14 +349,15 @@ int lbm_hip_create(const lbm_hip_params* pin, lbm_hip_ctx** out) {
         c->device_bytes += bytes;
         return true;
     };
-    const int nparts = c->G > c->nb256 ? c->G : c->nb256;
+    const int nparts = std::max(c->G + c->bblocks, c->nb256);
     if (!dmalloc(&c->f, (size_t)Q * c->SS * c->ssize) || !dmalloc((void**)&c->flag, N) ||
         !dmalloc(&c->rho, N * c->msize) || !dmalloc(&c->ux, N * c->msize) || !dmalloc(&c->uy, N * c->msize) ||
         !dmalloc(&c->uz, N * c->msize) || !dmalloc((void**)&c->st, sizeof(DevState)) ||
         !dmalloc((void**)&c->mass_part, sizeof(double) * nparts) ||
         !dmalloc((void**)&c->diag_part, sizeof(double) * 4 * c->nb256) ||
         !dmalloc((void**)&c->diag_out, sizeof(double) * 4) ||
-        !dmalloc((void**)&c->red_part, sizeof(double) * 8 * std::max(LINK_MAX_BLOCKS, c->nb256)))
+        !dmalloc((void**)&c->red_part, sizeof(double) * 8 * std::max(LINK_MAX_BLOCKS, c->nb256)) ||
+        !dmalloc((void**)&c->wall_rows, sizeof(uint32_t) * std::max<size_t>(1, wall_rows.size())))
         return cleanup(fail(LBM_HIP_ENOMEM, "device allocation failed"));
 
     // flags: wall layers (lbm.cpp:72-78)
@@ -367,6 +397,8 @@ int lbm_hip_create(const lbm_hip_params* pin, lbm_hip_ctx** out) {
                     const bool inside = r[0] * r[0] + r[1] * r[1] + r[2] * r[2] < R * R;
                     flag[n] = inside ? PARTICLE : FLUID;
                 }
+        for (int d = 0; d < 3; ++d) hs.xf[d] = s.X[d];
+        hs.rf = R;
         band_of(hs.sq, c->NX, c->NY, c->NZ, hs.band);
         c->link_blocks = link_blocks_for(hs.sq.R, c->NX, c->NY, c->NZ);
         if (c->link_blocks > LINK_MAX_BLOCKS) return cleanup(fail(LBM_HIP_EINVAL, "squirmer radius too large"));
@@ -393,6 +425,12 @@ int lbm_hip_create(const lbm_hip_params* pin, lbm_hip_ctx** out) {
             }
     hs.nfluid = nf;
 
+    c->bargs.wall_rows = c->wall_rows;
+    c->bargs.mass_part = c->mass_part + c->G;
+    if (!wall_rows.empty() &&
+        hipMemcpyAsync(c->wall_rows, wall_rows.data(), sizeof(uint32_t) * wall_rows.size(), hipMemcpyHostToDevice,
+                       c->stream) != hipSuccess)
+        return cleanup(fail(LBM_HIP_EDEVICE, "context initialisation failed"));
     if (hipMemcpyAsync(c->flag, dflag.data(), N, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
         hipMemcpyAsync(c->st, &hs, sizeof hs, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
         launch_fill_macro(c->rho, c->ux, c->uy, c->uz, c->prec == LBM_HIP_FP64, c->N, c->stream) != hipSuccess ||
@@ -418,7 +456,9 @@ int lbm_hip_step(lbm_hip_ctx* c, int flags) {
     hipError_t e = hipSuccess;
     dispatch(c->prec, [&](auto* tag) -> int {
         using S = std::remove_pointer_t<decltype(tag)>;
-        e = launch_step<S>(step_args<S>(c), c->K, odd, mode, c->stream);
+        const StepArgs<S> sa = step_args<S>(c);
+        e = launch_step<S>(sa, c->K, odd, mode, c->stream);
+        if (e == hipSuccess && !odd) e = launch_boundary<S>(sa, c->K, c->bargs, c->bblocks, mode, c->stream);
         // moving-boundary bounce-back and force/torque on the squirmer
         if (e == hipSuccess && c->has_squirmer) e = launch_links<S>(link_args<S>(c), c->link_blocks, c->stream);
         if (e == hipSuccess && (flags & LBM_STEP_WALL_FORCE))
@@ -427,6 +467,7 @@ int lbm_hip_step(lbm_hip_ctx* c, int flags) {
         return 0;
     });
     if (e != hipSuccess) return fail(LBM_HIP_EDEVICE, std::string("step launch: ") + hipGetErrorString(e));
+    c->last_bparts = odd ? 0 : c->bblocks;
     ++c->t;
     c->macro_valid = (flags & LBM_STEP_MACRO) != 0;
     c->wallf_valid = (flags & LBM_STEP_WALL_FORCE) != 0;
@@ -452,7 +493,7 @@ int lbm_hip_correct_mass(lbm_hip_ctx* c) {
     hipError_t e;
     const int shift_slot = (int)(c->t & 1);
     if (c->mass_valid) {
-        e = launch_mass_finalize(c->mass_part, c->G, c->st, 1, shift_slot, c->stream);
+        e = launch_mass_finalize(c->mass_part, c->G + c->last_bparts, c->st, 1, shift_slot, c->stream);
     } else {
         const int shifted = layout_shifted(c) ? 1 : 0;
         e = hipSuccess;

@@ -87,54 +87,98 @@ __device__ bool last_block_sum(double* part, unsigned* ticket, const double (&mi
     return true;
 }
 
+// One block covers LINK_BLOCK consecutive nodes of the box.  Its threads
+// first find their node's PARTICLE links (one flag load per neighbour, all in
+// flight together) and compact them into LDS in (node, direction) order;
+// then every thread takes links of that list, so a wavefront's lanes all do
+// link work although only a thin shell of the box owns links.
 template <class S>
 __global__ __launch_bounds__(LINK_BLOCK) void particle_links_kernel(const LinkArgs<S> A) {
-    using St = Store<S>;
     constexpr bool EXACT = std::is_same<S, double>::value;
     __shared__ double lds[(LINK_BLOCK / 64) * 7];
+    __shared__ uint16_t lk[LINK_BLOCK * (Q - 1)];  // thread << 5 | direction
+    __shared__ double nrho[LINK_BLOCK];
+    __shared__ int nx[LINK_BLOCK], ny[LINK_BLOCK], nz[LINK_BLOCK];
+    __shared__ int wsum[LINK_BLOCK / 64];
     DevState* st = A.st;
     const DevSquirmer& sq = st->sq;
     const int NX = A.NX, NY = A.NY, NZ = A.NZ;
     const LinkBox b = link_box(sq, NX, NY, NZ);
     const int M = b.wx * b.wy * b.wz;
-    double acc[7] = {0, 0, 0, 0, 0, 0, 0};  // Fx Fy Fz Tx Ty Tz dmass
-    const int k = blockIdx.x * LINK_BLOCK + threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int k = blockIdx.x * LINK_BLOCK + tid;
+    auto node_of = [&](int x, int y, int z) -> uint32_t {
+        return (uint32_t)x + (uint32_t)NX * ((uint32_t)y + (uint32_t)NY * (uint32_t)z);
+    };
+    uint32_t pmask = 0;
     if (k < M) {
         const int ix = k % b.wx, iy = (k / b.wx) % b.wy, iz = k / (b.wx * b.wy);
         const int x = wrap_once(b.x0 + ix, NX), y = wrap_once(b.y0 + iy, NY), z = wrap_once(b.z0 + iz, NZ);
-        const uint32_t n = (uint32_t)x + (uint32_t)NX * ((uint32_t)y + (uint32_t)NY * (uint32_t)z);
+        const uint32_t n = node_of(x, y, z);
         const uint8_t fl = A.flag[n];
         if ((fl & TYPE_MASK) == FLUID && (fl & BOUNDARY)) {
-            double rho;
-            if constexpr (EXACT) rho = static_cast<const double*>(A.rho)[n];
-            else rho = (double)static_cast<const float*>(A.rho)[n];
-            for (int i = 1; i < Q; ++i) {
-                const int xn = x + CX[i] < 0 ? NX - 1 : (x + CX[i] >= NX ? 0 : x + CX[i]);
-                const int yn = y + CY[i] < 0 ? NY - 1 : (y + CY[i] >= NY ? 0 : y + CY[i]);
-                const int zn = z + CZ[i] < 0 ? NZ - 1 : (z + CZ[i] >= NZ ? 0 : z + CZ[i]);
-                const uint32_t m = (uint32_t)xn + (uint32_t)NX * ((uint32_t)yn + (uint32_t)NY * (uint32_t)zn);
-                if ((A.flag[m] & TYPE_MASK) != PARTICLE) continue;
-                S* slot = A.f + (size_t)i * A.SS + m;
-                const double fpost = to_real<S>(*slot, i);
-                double rb[3], ub[3];
-                link_velocity(sq, NX, NY, NZ, A.walls_y, A.walls_z, x + 0.5 * CX[i], y + 0.5 * CY[i],
-                              z + 0.5 * CZ[i], rb, ub);
-                const double cub = CX[i] * ub[0] + CY[i] * ub[1] + CZ[i] * ub[2];
-                const double fb = fpost - 6.0 * W[i] * rho * cub;
-                *slot = from_real<S>(fb, i);
-                const double s = fpost + fb, dl = fpost - fb;
-                const double dpx = s * CX[i] - dl * ub[0];
-                const double dpy = s * CY[i] - dl * ub[1];
-                const double dpz = s * CZ[i] - dl * ub[2];
-                acc[0] += dpx; acc[1] += dpy; acc[2] += dpz;
-                acc[3] += rb[1] * dpz - rb[2] * dpy;
-                acc[4] += rb[2] * dpx - rb[0] * dpz;
-                acc[5] += rb[0] * dpy - rb[1] * dpx;
-                acc[6] += fb - fpost;
-            }
+            if constexpr (EXACT) nrho[tid] = static_cast<const double*>(A.rho)[n];
+            else nrho[tid] = (double)static_cast<const float*>(A.rho)[n];
+            nx[tid] = x; ny[tid] = y; nz[tid] = z;
+            const int xm = x == 0 ? NX - 1 : x - 1, xp = x + 1 == NX ? 0 : x + 1;
+            const int ym = y == 0 ? NY - 1 : y - 1, yp = y + 1 == NY ? 0 : y + 1;
+            const int zm = z == 0 ? NZ - 1 : z - 1, zp = z + 1 == NZ ? 0 : z + 1;
+            const int xs[3] = {xm, x, xp}, ys[3] = {ym, y, yp}, zs[3] = {zm, z, zp};
+            static_for<1, Q>([&](auto I) {
+                constexpr int i = decltype(I)::value;
+                const uint32_t m = node_of(xs[1 + CX[i]], ys[1 + CY[i]], zs[1 + CZ[i]]);
+                pmask |= (uint32_t)((A.flag[m] & TYPE_MASK) == PARTICLE) << i;
+            });
         }
     }
-    (void)EXACT;
+    // block exclusive scan of the link counts, then the compacted list
+    const int cnt = __builtin_popcount(pmask);
+    int incl = cnt;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int v = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += v;
+    }
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    int base = 0, total = 0;
+    for (int w = 0; w < LINK_BLOCK / 64; ++w) {
+        const int v = wsum[w];
+        base += w < wave ? v : 0;
+        total += v;
+    }
+    int pos = base + incl - cnt;
+    while (pmask) {
+        const int i = __builtin_ctz(pmask);
+        pmask &= pmask - 1;
+        lk[pos++] = (uint16_t)(tid << 5 | i);
+    }
+    __syncthreads();
+
+    double acc[7] = {0, 0, 0, 0, 0, 0, 0};  // Fx Fy Fz Tx Ty Tz dmass
+    for (int q = tid; q < total; q += LINK_BLOCK) {
+        const int e = lk[q], t = e >> 5, i = e & 31;
+        const int x = nx[t], y = ny[t], z = nz[t];
+        const int xn = x + CX[i] < 0 ? NX - 1 : (x + CX[i] >= NX ? 0 : x + CX[i]);
+        const int yn = y + CY[i] < 0 ? NY - 1 : (y + CY[i] >= NY ? 0 : y + CY[i]);
+        const int zn = z + CZ[i] < 0 ? NZ - 1 : (z + CZ[i] >= NZ ? 0 : z + CZ[i]);
+        S* slot = A.f + (size_t)i * A.SS + node_of(xn, yn, zn);
+        const double fpost = to_real<S>(*slot, i);
+        double rb[3], ub[3];
+        link_velocity(sq, NX, NY, NZ, A.walls_y, A.walls_z, x + 0.5 * CX[i], y + 0.5 * CY[i], z + 0.5 * CZ[i],
+                      rb, ub);
+        const double cub = CX[i] * ub[0] + CY[i] * ub[1] + CZ[i] * ub[2];
+        const double fb = fpost - 6.0 * W[i] * nrho[t] * cub;
+        *slot = from_real<S>(fb, i);
+        const double s = fpost + fb, dl = fpost - fb;
+        const double dpx = s * CX[i] - dl * ub[0];
+        const double dpy = s * CY[i] - dl * ub[1];
+        const double dpz = s * CZ[i] - dl * ub[2];
+        acc[0] += dpx; acc[1] += dpy; acc[2] += dpz;
+        acc[3] += rb[1] * dpz - rb[2] * dpy;
+        acc[4] += rb[2] * dpx - rb[0] * dpz;
+        acc[5] += rb[0] * dpy - rb[1] * dpx;
+        acc[6] += fb - fpost;
+    }
     block_reduce<7>(acc, lds);
     double tot[7];
     if (last_block_sum<7>(A.part, &st->link_ticket, acc, tot, lds) && threadIdx.x == 0) {

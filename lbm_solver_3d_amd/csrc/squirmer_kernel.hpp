@@ -7,10 +7,16 @@
 // refill density averages the FLUID neighbours as they are at that point of
 // the sweep, and the cover/uncover momentum sums are order dependent.  Here
 //   B. all threads classify the box in parallel and compact the changed
-//      nodes in sweep order (per-thread chunks + one block scan);
-//   C. one wavefront walks the changed list in that order, one lane per
-//      direction, summing in the reference's order on every lane, so the
-//      FP64 results are the reference's bit for bit;
+//      nodes in sweep order (per-thread chunks + one block scan).  A node
+//      can only change if it lies within |X - xf| of the sphere surface
+//      (the flags are the inside test at xf), so only those nodes load
+//      their flag;
+//   C. all threads gather what the sweep reads at each changed node (its
+//      populations, its neighbours' flags and densities before the remap,
+//      and which neighbours are themselves in the list) into LDS; then one
+//      wavefront walks the list in order from LDS, one lane per direction,
+//      summing in the reference's order on every lane, so the FP64 results
+//      are the reference's bit for bit;
 //   D. all threads move populations whose AA storage slot depends on the
 //      flags that just changed, write the equilibrium of uncovered nodes
 //      into the slots the next step reads, and refresh BOUNDARY bits.
@@ -31,7 +37,8 @@
 namespace lbmhip {
 
 constexpr int SQ_THREADS = 1024;
-constexpr int REMAP_CAP = 1024;  // changed nodes per remap held in LDS
+constexpr int REMAP_CAP = 512;   // changed nodes per remap held in LDS
+constexpr int GATHER = 128;      // changed nodes per gather pass of phase C
 constexpr int BOX_MAX = 128;     // remap box columns per axis held in LDS
 
 template <class S>
@@ -67,13 +74,10 @@ __device__ inline double equilibrium(int i, double rho, double ux, double uy, do
 }
 
 __device__ inline int wrapi(int a, int n) { return ((a % n) + n) % n; }
-
-// lbm.cpp:100-104 at integer node coordinates
-__device__ inline void node_offset(const DevSquirmer& sq, int NX, int NY, int NZ, int wy, int wz, int x, int y, int z,
-                                   double r[3]) {
-    r[0] = x - sq.X[0]; r[0] -= NX * round(r[0] / NX);
-    r[1] = y - sq.X[1]; if (!wy) r[1] -= NY * round(r[1] / NY);
-    r[2] = z - sq.X[2]; if (!wz) r[2] -= NZ * round(r[2] / NZ);
+// a + d for d in {-1, 0, 1} on a periodic axis of length n, 0 <= a < n
+__device__ inline int step1(int a, int d, int n) {
+    const int b = a + d;
+    return b < 0 ? n - 1 : (b >= n ? 0 : b);
 }
 
 // Read a device-side macroscopic value as double.
@@ -81,6 +85,12 @@ template <class S>
 __device__ inline double load_macro(const void* p, uint32_t n) {
     if constexpr (std::is_same<S, double>::value) return static_cast<const double*>(p)[n];
     else return (double)static_cast<const float*>(p)[n];
+}
+// The value a density stored by store_macro reads back as.
+template <class S>
+__device__ inline double macro_round(double v) {
+    if constexpr (std::is_same<S, double>::value) return v;
+    else return (double)(float)v;
 }
 template <class S>
 __device__ inline void store_macro(void* p, uint32_t n, double v) {
@@ -90,21 +100,35 @@ __device__ inline void store_macro(void* p, uint32_t n, double v) {
 
 template <class S>
 __global__ __launch_bounds__(SQ_THREADS) void squirmer_kernel(const RemapArgs<S> A) {
-    __shared__ uint32_t list[REMAP_CAP];
+    __shared__ uint32_t list[REMAP_CAP];  // node | cover bit, in sweep order
+    __shared__ int lpos[REMAP_CAP];       // box position of each entry (ascending)
     __shared__ double lrr[REMAP_CAP], lub[REMAP_CAP][3];
+    __shared__ double gval[GATHER][Q];    // cover: f_i(n); uncover: rho of neighbour i
+    __shared__ uint8_t gfl[GATHER][Q];    // neighbour i's type before the remap
+    __shared__ int16_t gidx[GATHER][Q];   // neighbour i's list index, or -1
+    __shared__ double sprune[2];          // candidate band of squared distances
     __shared__ int sbox[6];  // x0, y0, z0, wx, wy, wz
     __shared__ int scount[SQ_THREADS / 64];
     __shared__ int sK, sOverflow;
     __shared__ double boxr[3][BOX_MAX];
     __shared__ int boxc[3][BOX_MAX];
 
+    __shared__ DevSquirmer sq;  // working copy; written back at the end
     DevState* st = A.st;
-    DevSquirmer& sq = st->sq;
     const int tid = threadIdx.x;
     const int NX = A.NX, NY = A.NY, NZ = A.NZ;
     const uint32_t SS = A.SS;
 
     // ---------------- A: rigid body (lbm.cpp:257-262) ----------------
+    {
+        // one coalesced copy of the squirmer state into LDS
+        const double* src = reinterpret_cast<const double*>(&st->sq);
+        double* dst = reinterpret_cast<double*>(&sq);
+        constexpr int nd = (int)(sizeof(DevSquirmer) / sizeof(double));
+        static_assert(sizeof(DevSquirmer) % sizeof(double) == 0, "DevSquirmer must be a whole number of doubles");
+        for (int k = tid; k < nd; k += SQ_THREADS) dst[k] = src[k];
+    }
+    __syncthreads();
     if (tid == 0) {
         // sq.Fh / sq.Th: published by particle_links_kernel after the step
         // applyWallPotential (lbm.cpp:231-241)
@@ -161,6 +185,21 @@ __global__ __launch_bounds__(SQ_THREADS) void squirmer_kernel(const RemapArgs<S>
         wy = wy < 0 ? 0 : wy;
         sbox[0] = x0; sbox[1] = y0; sbox[2] = z0; sbox[3] = wx; sbox[4] = wy; sbox[5] = wz;
         sOverflow = 0;
+        // Only nodes whose distance from X is within |X - xf| of the sphere
+        // surface can change (triangle inequality).  Applied when the box
+        // holds no periodic image twice and the motion is below one node.
+        double dl2 = 0.0;
+        for (int d = 0; d < 3; ++d) { const double dd = sq.X[d] - st->xf[d]; dl2 += dd * dd; }
+        const double dl = sqrt(dl2), rf = st->rf, margin = 1e-6;
+        const bool cut = (wx > NX - 4) || (!A.walls_y && wy > NY - 4) || (!A.walls_z && wz > NZ - 4);
+        if (dl < 1.0 && !cut) {
+            const double lo = fmin(R, rf - dl) - margin, hi = fmax(R, rf + dl) + margin;
+            sprune[0] = lo > 0.0 ? lo * lo : -1.0;
+            sprune[1] = hi * hi;
+        } else {
+            sprune[0] = -1.0;
+            sprune[1] = __builtin_inf();
+        }
     }
     __syncthreads();
 
@@ -173,7 +212,10 @@ __global__ __launch_bounds__(SQ_THREADS) void squirmer_kernel(const RemapArgs<S>
     // change / cover bits in registers for the compaction pass.
     const int bx0 = sbox[0], by0 = sbox[1], bz0 = sbox[2], bwx = sbox[3], bwy = sbox[4], bwz = sbox[5];
     if (bwx > BOX_MAX || bwy > BOX_MAX || bwz > BOX_MAX) {
-        if (tid == 0) atomicOr(&st->err, ERR_REMAP_OVERFLOW);
+        if (tid == 0) {
+            atomicOr(&st->err, ERR_REMAP_OVERFLOW);
+            st->sq = sq;
+        }
         return;  // uniform
     }
     for (int k = tid; k < bwx + bwy + bwz; k += SQ_THREADS) {
@@ -189,168 +231,249 @@ __global__ __launch_bounds__(SQ_THREADS) void squirmer_kernel(const RemapArgs<S>
         boxc[d][j] = cw;
     }
     __syncthreads();
-    const int M = bwx * bwy * bwz;
-    const int chunk = (M + SQ_THREADS - 1) / SQ_THREADS;
-    const int k0 = tid * chunk, k1 = (k0 + chunk < M) ? k0 + chunk : M;
+    // One thread per (y, z) column of the box, columns in sweep order, in
+    // rounds of SQ_THREADS columns.  With the candidate band [lo2, hi2] of
+    // squared distances, a column's candidates are the x positions with
+    // |rx| in [sqrt(lo2 - s), sqrt(hi2 - s)], s = ry^2 + rz^2: at most two
+    // short intervals (rx = rx(0) + ix, the box not being cut by the period).
+    // Without a band the whole column is a candidate.  Candidates load their
+    // flag and take the exact inside test; a pass counts, a block scan
+    // places, a second pass writes the list.
     const double R2 = sq.R * sq.R;
     const uint32_t uNX = (uint32_t)NX, uNXY = (uint32_t)NX * (uint32_t)NY;
     auto node_at = [&](int ix, int iy, int iz) -> uint32_t {
         return (uint32_t)boxc[0][ix] + uNX * (uint32_t)boxc[1][iy] + uNXY * (uint32_t)boxc[2][iz];
     };
-    uint64_t chg = 0, cov = 0;  // bit j: position k0 + j changes / is covered
-    int cnt = 0;
-    if (k0 < k1) {
-        int ix = k0 % bwx, iy = (k0 / bwx) % bwy, iz = k0 / (bwx * bwy);
-        for (int kb = k0; kb < k1; kb += 8) {
-            uint8_t fv[8];
-            int jx[8], jy[8], jz[8];
-#pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                jx[j] = ix; jy[j] = iy; jz[j] = iz;
-                fv[j] = (kb + j < k1) ? A.flag[node_at(ix, iy, iz)] : (uint8_t)WALL;
-                if (++ix == bwx) { ix = 0; if (++iy == bwy) { iy = 0; ++iz; } }
-            }
-#pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                const uint8_t t = fv[j] & TYPE_MASK;
-                if (t == WALL) continue;
-                const double rx = boxr[0][jx[j]], ry = boxr[1][jy[j]], rz = boxr[2][jz[j]];
-                const bool inside = rx * rx + ry * ry + rz * rz < R2;
-                if (inside != (t == PARTICLE)) {
-                    const int b = kb + j - k0;
-                    if (b < 64) {
-                        chg |= 1ull << b;
-                        if (inside) cov |= 1ull << b;
-                    }
-                    ++cnt;
+    const double lo2 = sprune[0], hi2 = sprune[1];
+    const bool banded = hi2 < __builtin_inf();
+    const int lane = tid & 63, wave = tid >> 6;
+    const int ncol = bwy * bwz;
+    int K = 0;  // list length so far (uniform)
+    for (int r0 = 0; r0 < ncol; r0 += SQ_THREADS) {
+        const int col = r0 + tid;
+        int iy = 0, iz = 0, a1 = 0, b1 = -1, a2 = 0, b2 = -1;
+        if (col < ncol) {
+            iy = col % bwy; iz = col / bwy;
+            if (!banded) {
+                b1 = bwx - 1;
+            } else {
+                const double ry = boxr[1][iy], rz = boxr[2][iz];
+                const double sc = ry * ry + rz * rz;
+                if (sc <= hi2) {
+                    const double xb = sqrt(hi2 - sc), xa = lo2 > sc ? sqrt(lo2 - sc) : 0.0;
+                    const double off = boxr[0][0];
+                    const double eps = 1e-7;
+                    a1 = (int)ceil(-xb - off - eps); b1 = (int)floor(-xa - off + eps);
+                    a2 = (int)ceil(xa - off - eps);  b2 = (int)floor(xb - off + eps);
+                    if (b1 >= a2) { b1 = b2; b2 = -1; }  // the two intervals meet
+                    a1 = a1 < 0 ? 0 : a1; b1 = b1 > bwx - 1 ? bwx - 1 : b1;
+                    a2 = a2 < 0 ? 0 : a2; b2 = b2 > bwx - 1 ? bwx - 1 : b2;
+                    if (b1 >= a2 && b2 >= a2) a2 = b1 + 1;  // no position twice
                 }
             }
         }
-    }
-    if (chunk > 64 && tid == 0) atomicOr(&st->err, ERR_REMAP_OVERFLOW);
-    // block exclusive scan of cnt
-    const int lane = tid & 63, wave = tid >> 6;
-    int incl = cnt;
-    for (int off = 1; off < 64; off <<= 1) {
-        const int v = __shfl_up(incl, off, 64);
-        if (lane >= off) incl += v;
-    }
-    if (lane == 63) scount[wave] = incl;
-    __syncthreads();
-    if (tid == 0) {
-        int tot = 0;
-        for (int w = 0; w < SQ_THREADS / 64; ++w) { const int c = scount[w]; scount[w] = tot; tot += c; }
-        sK = tot;
-        if (tot > REMAP_CAP || chunk > 64) { sOverflow = 1; atomicOr(&st->err, ERR_REMAP_OVERFLOW); }
-    }
-    __syncthreads();
-    const int K = sK;
-    if (sOverflow) return;  // uniform: the host reports the error
-    {
-        int pos = scount[wave] + incl - cnt;
-        while (chg) {
-            const int b = __builtin_ctzll(chg);
-            chg &= chg - 1;
-            const int k = k0 + b;
-            const int ix = k % bwx, iy = (k / bwx) % bwy, iz = k / (bwx * bwy);
-            list[pos++] = node_at(ix, iy, iz) | (((cov >> b) & 1ull) ? 0x80000000u : 0u);
+        // visit the candidates of this column in x order
+        auto visit = [&](auto&& on_change) {
+            const double ry = boxr[1][iy], rz = boxr[2][iz];
+            for (int pass = 0; pass < 2; ++pass) {
+                const int lo = pass ? a2 : a1, hi = pass ? b2 : b1;
+                for (int i0 = lo; i0 <= hi; i0 += 4) {  // flag loads four at a time
+                    uint8_t fv[4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) fv[j] = i0 + j <= hi ? A.flag[node_at(i0 + j, iy, iz)] : (uint8_t)WALL;
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const uint8_t t = fv[j] & TYPE_MASK;
+                        if (t == WALL) continue;
+                        const double rx = boxr[0][i0 + j];
+                        const bool inside = rx * rx + ry * ry + rz * rz < R2;  // lbm.cpp:287
+                        if (inside != (t == PARTICLE)) on_change(i0 + j, inside);
+                    }
+                }
+            }
+        };
+        int cnt = 0;
+        visit([&](int, bool) { ++cnt; });
+        // block exclusive scan of cnt
+        int incl = cnt;
+        for (int off = 1; off < 64; off <<= 1) {
+            const int v = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += v;
         }
+        if (lane == 63) scount[wave] = incl;
+        __syncthreads();
+        if (tid == 0) {
+            int tot = K;
+            for (int w = 0; w < SQ_THREADS / 64; ++w) { const int c = scount[w]; scount[w] = tot; tot += c; }
+            sK = tot;
+            if (tot > REMAP_CAP) { sOverflow = 1; atomicOr(&st->err, ERR_REMAP_OVERFLOW); }
+        }
+        __syncthreads();
+        if (sOverflow) {  // uniform: the host reports the error
+            if (tid == 0) st->sq = sq;
+            return;
+        }
+        int pos = scount[wave] + incl - cnt;
+        visit([&](int ix, bool inside) {
+            lpos[pos] = ix + bwx * (iy + bwy * iz);
+            list[pos++] = node_at(ix, iy, iz) | (inside ? 0x80000000u : 0u);
+        });
+        K = sK;
+        __syncthreads();  // scount / sK reused by the next round
     }
-    __syncthreads();
 
-    // ---------------- C: sequential cover / uncover (wave 0) ----------------
-    if (wave == 0) {
-        double Fc[3] = {0, 0, 0}, Tc[3] = {0, 0, 0}, dmass = 0.0;
-        long long dnf = 0;
-        const double dsh = st->shift[A.shift_slot];
-        const int i = lane;  // direction handled by this lane (lanes >= Q idle)
-        const int cxi = i < Q ? CX[i] : 0, cyi = i < Q ? CY[i] : 0, czi = i < Q ? CZ[i] : 0;
-        for (int k = 0; k < K; ++k) {
+    // ---------------- C: cover / uncover in sweep order ----------------
+    double Fc[3] = {0, 0, 0}, Tc[3] = {0, 0, 0}, dmass = 0.0;
+    long long dnf = 0;
+    const double dsh = st->shift[A.shift_slot];
+    const bool wrapx = bwx == NX, wrapy = bwy == NY, wrapz = bwz == NZ;
+    for (int c0 = 0; c0 < K; c0 += GATHER) {
+        const int nc = (K - c0 < GATHER) ? K - c0 : GATHER;
+        // C1: gather.  Flags already changed by an earlier pass carry FLIPPED,
+        // which gives back the type before this remap.
+        for (int w = tid; w < nc * Q; w += SQ_THREADS) {
+            const int kk = w / Q, i = w - kk * Q, k = c0 + kk;
             const uint32_t e = list[k];
             const uint32_t n = e & 0x7FFFFFFFu;
             const bool cover = (e >> 31) != 0;
-            const int x = (int)(n % (uint32_t)NX), y = (int)((n / (uint32_t)NX) % (uint32_t)NY),
-                      z = (int)(n / ((uint32_t)NX * (uint32_t)NY));
-            double r[3];
-            node_offset(sq, NX, NY, NZ, A.walls_y, A.walls_z, x, y, z, r);
+            const int p = lpos[k];
+            const int ix = p % bwx, iy = (p / bwx) % bwy, iz = p / (bwx * bwy);
+            const int x = boxc[0][ix], y = boxc[1][iy], z = boxc[2][iz];
             if (cover) {
-                // population f_i(n) after the last step (old flags), lbm.cpp:295-298
-                double fv = 0.0;
-                if (i < Q) {
-                    size_t slot = (size_t)i * SS + n;
-                    if (i > 0) {
-                        const uint32_t m = (uint32_t)wrapi(x - cxi, NX) +
-                                           (uint32_t)NX * ((uint32_t)wrapi(y - cyi, NY) +
-                                                           (uint32_t)NY * (uint32_t)wrapi(z - czi, NZ));
-                        bool up_fluid = false;
-                        if (!A.shifted) {
-                            const uint8_t fm = A.flag[m];
-                            uint8_t tm = fm & TYPE_MASK;
-                            if (fm & FLIPPED) tm = (tm == FLUID) ? PARTICLE : FLUID;  // flags before this remap
-                            up_fluid = tm == FLUID;
-                        }
-                        if (!up_fluid) slot = (size_t)OPP[i] * SS + m;
+                // population f_i(n) after the last step, lbm.cpp:295-298
+                size_t slot = (size_t)i * SS + n;
+                if (i > 0) {
+                    const uint32_t m = (uint32_t)step1(x, -CX[i], NX) +
+                                       (uint32_t)NX * ((uint32_t)step1(y, -CY[i], NY) +
+                                                       (uint32_t)NY * (uint32_t)step1(z, -CZ[i], NZ));
+                    bool up_fluid = false;
+                    if (!A.shifted) {
+                        const uint8_t fm = A.flag[m];
+                        uint8_t tm = fm & TYPE_MASK;
+                        if (fm & FLIPPED) tm = (tm == FLUID) ? PARTICLE : FLUID;
+                        up_fluid = tm == FLUID;
                     }
-                    fv = to_real<S>(A.f[slot], i);
-                    if (dsh != 0.0) fv += W[i] * dsh;
+                    if (!up_fluid) slot = (size_t)OPP[i] * SS + m;
                 }
-                double p[3] = {0, 0, 0}, msum = 0.0;
-                for (int j = 0; j < Q; ++j) {
-                    const double v = __shfl(fv, j, 64);
-                    p[0] += v * CX[j]; p[1] += v * CY[j]; p[2] += v * CZ[j];
-                    msum += v;
+                double fv = to_real<S>(A.f[slot], i);
+                if (dsh != 0.0) fv += W[i] * dsh;
+                gval[kk][i] = fv;
+            } else if (i > 0) {
+                // neighbour n + c_i: type and density before the remap
+                const uint32_t m = (uint32_t)step1(x, CX[i], NX) +
+                                   (uint32_t)NX * ((uint32_t)step1(y, CY[i], NY) + (uint32_t)NY * (uint32_t)step1(z, CZ[i], NZ));
+                const uint8_t fm = A.flag[m];
+                uint8_t tm = fm & TYPE_MASK;
+                if (fm & FLIPPED) tm = (tm == FLUID) ? PARTICLE : FLUID;
+                gfl[kk][i] = tm;
+                gval[kk][i] = tm == FLUID ? load_macro<S>(A.rho, m) : 0.0;
+                // its index in the list, if it is in the box and changes too
+                int jx = ix + CX[i], jy = iy + CY[i], jz = iz + CZ[i];
+                if (wrapx) jx = jx < 0 ? bwx - 1 : (jx == bwx ? 0 : jx);
+                if (wrapy) jy = jy < 0 ? bwy - 1 : (jy == bwy ? 0 : jy);
+                if (wrapz) jz = jz < 0 ? bwz - 1 : (jz == bwz ? 0 : jz);
+                int j = -1;
+                if (jx >= 0 && jx < bwx && jy >= 0 && jy < bwy && jz >= 0 && jz < bwz) {
+                    const int q = jx + bwx * (jy + bwy * jz);
+                    int a = 0, bnd = K;  // lower bound of q in lpos[0, K)
+                    while (a < bnd) {
+                        const int mid = (a + bnd) >> 1;
+                        if (lpos[mid] < q) a = mid + 1; else bnd = mid;
+                    }
+                    if (a < K && lpos[a] == q) j = a;
                 }
-                const double t[3] = {r[1] * p[2] - r[2] * p[1], r[2] * p[0] - r[0] * p[2], r[0] * p[1] - r[1] * p[0]};
-                for (int d = 0; d < 3; ++d) { Fc[d] += p[d]; Tc[d] += t[d]; }
-                dmass -= msum - 1.0;
-                --dnf;
-                if (lane == 0) A.flag[n] = (uint8_t)(PARTICLE | FLIPPED);
-            } else {
-                // refill density from FLUID neighbours as of this point of the sweep
-                double rv = 0.0;
-                int isf = 0;
-                if (i >= 1 && i < Q) {
-                    const uint32_t m = (uint32_t)wrapi(x + cxi, NX) +
-                                       (uint32_t)NX * ((uint32_t)wrapi(y + cyi, NY) +
-                                                       (uint32_t)NY * (uint32_t)wrapi(z + czi, NZ));
-                    if ((A.flag[m] & TYPE_MASK) == FLUID) { rv = load_macro<S>(A.rho, m); isf = 1; }
-                }
-                double rs = 0.0;
-                int c = 0;
-                for (int j = 1; j < Q; ++j) {
-                    const double v = __shfl(rv, j, 64);
-                    if (__shfl(isf, j, 64)) { rs += v; ++c; }
-                }
-                const double rr = c ? rs / c : 1.0;
-                double ub[3];  // Squirmer::surfaceVelocity(r, ub, false)
-                ub[0] = sq.U[0] + (sq.W[1] * r[2] - sq.W[2] * r[1]);
-                ub[1] = sq.U[1] + (sq.W[2] * r[0] - sq.W[0] * r[2]);
-                ub[2] = sq.U[2] + (sq.W[0] * r[1] - sq.W[1] * r[0]);
-                double fsum = 0.0;
-                for (int j = 0; j < Q; ++j) fsum += equilibrium(j, rr, ub[0], ub[1], ub[2]);
-                const double p[3] = {rr * ub[0], rr * ub[1], rr * ub[2]};
-                const double t[3] = {r[1] * p[2] - r[2] * p[1], r[2] * p[0] - r[0] * p[2], r[0] * p[1] - r[1] * p[0]};
-                for (int d = 0; d < 3; ++d) { Fc[d] -= p[d]; Tc[d] -= t[d]; }
-                dmass += fsum - 1.0;
-                ++dnf;
-                if (lane == 0) {
-                    store_macro<S>(A.rho, n, rr);
-                    store_macro<S>(A.ux, n, ub[0]);
-                    store_macro<S>(A.uy, n, ub[1]);
-                    store_macro<S>(A.uz, n, ub[2]);
-                    A.flag[n] = (uint8_t)(FLUID | FLIPPED);
-                    lrr[k] = rr; lub[k][0] = ub[0]; lub[k][1] = ub[1]; lub[k][2] = ub[2];
-                }
+                gidx[kk][i] = (int16_t)j;
             }
-            // make this node's flag / rho visible to the next iteration's loads
-            __threadfence_block();
         }
+        __syncthreads();
+        // C2: the sweep, from LDS
+        if (wave == 0) {
+            const int i = lane;  // direction handled by this lane (lanes >= Q idle)
+            for (int kk = 0; kk < nc; ++kk) {
+                const int k = c0 + kk;
+                const uint32_t e = list[k];
+                const uint32_t n = e & 0x7FFFFFFFu;
+                const bool cover = (e >> 31) != 0;
+                const int p = lpos[k];
+                const int ix = p % bwx, iy = (p / bwx) % bwy, iz = p / (bwx * bwy);
+                const double r[3] = {boxr[0][ix], boxr[1][iy], boxr[2][iz]};  // lbm.cpp:100-104
+                if (cover) {
+                    const double fv = i < Q ? gval[kk][i] : 0.0;
+                    double pm[3] = {0, 0, 0}, msum = 0.0;
+                    for (int j = 0; j < Q; ++j) {
+                        const double v = __shfl(fv, j, 64);
+                        pm[0] += v * CX[j]; pm[1] += v * CY[j]; pm[2] += v * CZ[j];
+                        msum += v;
+                    }
+                    const double t[3] = {r[1] * pm[2] - r[2] * pm[1], r[2] * pm[0] - r[0] * pm[2],
+                                         r[0] * pm[1] - r[1] * pm[0]};
+                    for (int d = 0; d < 3; ++d) { Fc[d] += pm[d]; Tc[d] += t[d]; }
+                    dmass -= msum - 1.0;
+                    --dnf;
+                    if (lane == 0) A.flag[n] = (uint8_t)(PARTICLE | FLIPPED);
+                } else {
+                    // refill density from FLUID neighbours as of this point of the sweep
+                    double rv = 0.0;
+                    int isf = 0;
+                    if (i >= 1 && i < Q) {
+                        const int j = gidx[kk][i];
+                        if (j >= 0 && j < k) {  // changed earlier in the sweep
+                            isf = (list[j] >> 31) == 0;
+                            if (isf) rv = macro_round<S>(lrr[j]);
+                        } else {
+                            isf = gfl[kk][i] == FLUID;
+                            if (isf) rv = gval[kk][i];
+                        }
+                    }
+                    double rs = 0.0;
+                    int c = 0;
+                    for (int j = 1; j < Q; ++j) {
+                        const double v = __shfl(rv, j, 64);
+                        if (__shfl(isf, j, 64)) { rs += v; ++c; }
+                    }
+                    const double rr = c ? rs / c : 1.0;
+                    double ub[3];  // Squirmer::surfaceVelocity(r, ub, false)
+                    ub[0] = sq.U[0] + (sq.W[1] * r[2] - sq.W[2] * r[1]);
+                    ub[1] = sq.U[1] + (sq.W[2] * r[0] - sq.W[0] * r[2]);
+                    ub[2] = sq.U[2] + (sq.W[0] * r[1] - sq.W[1] * r[0]);
+                    double fsum = 0.0;
+                    for (int j = 0; j < Q; ++j) fsum += equilibrium(j, rr, ub[0], ub[1], ub[2]);
+                    const double pm[3] = {rr * ub[0], rr * ub[1], rr * ub[2]};
+                    const double t[3] = {r[1] * pm[2] - r[2] * pm[1], r[2] * pm[0] - r[0] * pm[2],
+                                         r[0] * pm[1] - r[1] * pm[0]};
+                    for (int d = 0; d < 3; ++d) { Fc[d] -= pm[d]; Tc[d] -= t[d]; }
+                    dmass += fsum - 1.0;
+                    ++dnf;
+                    if (lane == 0) {
+                        store_macro<S>(A.rho, n, rr);
+                        store_macro<S>(A.ux, n, ub[0]);
+                        store_macro<S>(A.uy, n, ub[1]);
+                        store_macro<S>(A.uz, n, ub[2]);
+                        A.flag[n] = (uint8_t)(FLUID | FLIPPED);
+                        lrr[k] = rr; lub[k][0] = ub[0]; lub[k][1] = ub[1]; lub[k][2] = ub[2];
+                    }
+                }
+                // lrr[k] is read by later iterations on other lanes of this
+                // wave: keep the compiler from reordering across iterations
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                __builtin_amdgcn_wave_barrier();
+            }
+        }
+        __syncthreads();
+    }
+    if (wave == 0) {
         if (lane == 0) {
             for (int d = 0; d < 3; ++d) { sq.Fc[d] += Fc[d]; sq.Tc[d] += Tc[d]; }
             st->remap_dmass += dmass;
             st->nfluid += dnf;
             st->changed = K;
         }
+    }
+    __syncthreads();
+    {
+        const double* src = reinterpret_cast<const double*>(&sq);
+        double* dst = reinterpret_cast<double*>(&st->sq);
+        for (int k = tid; k < (int)(sizeof(DevSquirmer) / sizeof(double)); k += SQ_THREADS) dst[k] = src[k];
     }
     __syncthreads();
 
@@ -367,8 +490,8 @@ __global__ __launch_bounds__(SQ_THREADS) void squirmer_kernel(const RemapArgs<S>
         }
         const int x = (int)(n % (uint32_t)NX), y = (int)((n / (uint32_t)NX) % (uint32_t)NY),
                   z = (int)(n / ((uint32_t)NX * (uint32_t)NY));
-        const uint32_t m = (uint32_t)wrapi(x - CX[i], NX) +
-                           (uint32_t)NX * ((uint32_t)wrapi(y - CY[i], NY) + (uint32_t)NY * (uint32_t)wrapi(z - CZ[i], NZ));
+        const uint32_t m = (uint32_t)step1(x, -CX[i], NX) +
+                           (uint32_t)NX * ((uint32_t)step1(y, -CY[i], NY) + (uint32_t)NY * (uint32_t)step1(z, -CZ[i], NZ));
         const size_t own = (size_t)i * SS + n, far = (size_t)OPP[i] * SS + m;
         if (A.shifted) {  // loc(n, i) = slot opp(i) at n - c_i: nothing moves
             if (!cover) f[far] = from_real<S>(equilibrium(i, lrr[k], lub[k][0], lub[k][1], lub[k][2]), i);
@@ -397,14 +520,15 @@ __global__ __launch_bounds__(SQ_THREADS) void squirmer_kernel(const RemapArgs<S>
         const uint32_t n0 = list[k] & 0x7FFFFFFFu;
         const int x0 = (int)(n0 % (uint32_t)NX), y0 = (int)((n0 / (uint32_t)NX) % (uint32_t)NY),
                   z0 = (int)(n0 / ((uint32_t)NX * (uint32_t)NY));
-        const int x = wrapi(x0 + CX[i], NX), y = wrapi(y0 + CY[i], NY), z = wrapi(z0 + CZ[i], NZ);
+        const int x = step1(x0, CX[i], NX), y = step1(y0, CY[i], NY), z = step1(z0, CZ[i], NZ);
         const uint32_t m = (uint32_t)x + (uint32_t)NX * ((uint32_t)y + (uint32_t)NY * (uint32_t)z);
         const uint8_t t = A.flag[m] & TYPE_MASK;
         bool b = false;
         if (t == FLUID) {
+#pragma unroll
             for (int j = 1; j < Q; ++j) {
-                const uint32_t mm = (uint32_t)wrapi(x + CX[j], NX) +
-                                    (uint32_t)NX * ((uint32_t)wrapi(y + CY[j], NY) + (uint32_t)NY * (uint32_t)wrapi(z + CZ[j], NZ));
+                const uint32_t mm = (uint32_t)step1(x, CX[j], NX) +
+                                    (uint32_t)NX * ((uint32_t)step1(y, CY[j], NY) + (uint32_t)NY * (uint32_t)step1(z, CZ[j], NZ));
                 b |= (A.flag[mm] & TYPE_MASK) != FLUID;
             }
         }
@@ -424,6 +548,8 @@ __global__ __launch_bounds__(SQ_THREADS) void squirmer_kernel(const RemapArgs<S>
             wd[d] = wdt;
         }
         for (int d = 0; d < 3; ++d) { st->band[d] = lo[d]; st->band[3 + d] = wd[d]; }
+        for (int d = 0; d < 3; ++d) st->xf[d] = sq.X[d];
+        st->rf = sq.R;
     }
 }
 

@@ -202,11 +202,16 @@ __device__ __forceinline__ void collide_fast(float (&g)[Q], const FastCoef& K, f
 // Squirmer surface velocity at a link midpoint (lbm.cpp:100-104 minimum
 // image, squirmer.cpp:35-46 rigid-body motion + two-mode slip), FP64.
 // ---------------------------------------------------------------------------
+// r - N round(r / N) (lbm.cpp:100-104) without the division where it cannot
+// matter: for |r| < N / 4 the quotient rounds to zero and r comes back as is.
+__device__ __forceinline__ double min_image(double r, int N) {
+    return fabs(r) < 0.25 * N ? r : r - N * round(r / N);
+}
 __device__ __forceinline__ void link_velocity(const DevSquirmer& sq, int NX, int NY, int NZ, int wallsY, int wallsZ,
                                               double px, double py, double pz, double rb[3], double ub[3]) {
-    rb[0] = px - sq.X[0]; rb[0] -= NX * round(rb[0] / NX);
-    rb[1] = py - sq.X[1]; if (!wallsY) rb[1] -= NY * round(rb[1] / NY);
-    rb[2] = pz - sq.X[2]; if (!wallsZ) rb[2] -= NZ * round(rb[2] / NZ);
+    rb[0] = min_image(px - sq.X[0], NX);
+    rb[1] = py - sq.X[1]; if (!wallsY) rb[1] = min_image(rb[1], NY);
+    rb[2] = pz - sq.X[2]; if (!wallsZ) rb[2] = min_image(rb[2], NZ);
     ub[0] = sq.U[0] + (sq.W[1] * rb[2] - sq.W[2] * rb[1]);
     ub[1] = sq.U[1] + (sq.W[2] * rb[0] - sq.W[0] * rb[2]);
     ub[2] = sq.U[2] + (sq.W[0] * rb[1] - sq.W[1] * rb[0]);
@@ -218,6 +223,35 @@ __device__ __forceinline__ void link_velocity(const DevSquirmer& sq, int NX, int
     ub[0] += amp * (ct * rh[0] - sq.e[0]);
     ub[1] += amp * (ct * rh[1] - sq.e[1]);
     ub[2] += amp * (ct * rh[2] - sq.e[2]);
+}
+
+// Rows of the squirmer's link box (nodes within R + 1 of X) and the range of
+// 64-node x chunks that cover it.
+struct LinkBoxDims {
+    int y0, z0, wy, wz, c0, nck;
+};
+__device__ inline int wrap_coord(int a, int n) {
+    a %= n;
+    return a < 0 ? a + n : a;
+}
+__device__ inline LinkBoxDims link_box_dims(const DevSquirmer& sq, int NX, int NY, int NZ) {
+    LinkBoxDims b;
+    const int x0 = (int)floor(sq.X[0] - sq.R) - 1, x1 = (int)ceil(sq.X[0] + sq.R) + 1;
+    const int y0 = (int)floor(sq.X[1] - sq.R) - 1, y1 = (int)ceil(sq.X[1] + sq.R) + 1;
+    const int z0 = (int)floor(sq.X[2] - sq.R) - 1, z1 = (int)ceil(sq.X[2] + sq.R) + 1;
+    b.y0 = y0; b.z0 = z0;
+    b.wy = (y1 - y0 + 1) < NY ? (y1 - y0 + 1) : NY;
+    b.wz = (z1 - z0 + 1) < NZ ? (z1 - z0 + 1) : NZ;
+    const int nchx = (NX + 63) / 64;
+    const int wx = (x1 - x0 + 1) < NX ? (x1 - x0 + 1) : NX;
+    const int xs = wrap_coord(x0, NX);
+    b.c0 = xs / 64;
+    if (xs + wx > NX)  // wraps around x: chunks c0 .. last, then 0 .. the one holding the tail
+        b.nck = (nchx - b.c0) + (xs + wx - 1 - NX) / 64 + 1;
+    else
+        b.nck = (xs + wx - 1) / 64 - b.c0 + 1;
+    if (b.nck > nchx) b.nck = nchx;
+    return b;
 }
 
 // Wavefront + block reduction of K doubles; thread 0 of the block gets the sums.
@@ -245,26 +279,161 @@ __device__ __forceinline__ void block_reduce(double (&v)[K], double* lds) {
 }
 
 // ---------------------------------------------------------------------------
-// The kernel.
+// The kernels.
 //
-// Addressing: a wavefront is 64 consecutive x of one (y, z) row (tiles are
-// at least 64 wide), so the row offsets of the 9 (dy, dz) neighbour rows are
-// wavefront-uniform and live in scalar registers together with the 19 slot
-// bases; a lane only holds its three x byte offsets (x-1, x, x+1, periodic).
-// Every uniform load and store is one global access with a scalar base and a
-// 32-bit lane offset.
+// Addressing: a wavefront is 64 consecutive x of one (y, z) row (chunks of
+// 64 aligned to multiples of 64), so the row offsets of the 9 (dy, dz)
+// neighbour rows are wavefront-uniform and live in scalar registers together
+// with the 19 slot bases; a lane only holds its three x byte offsets (x-1,
+// x, x+1, periodic).  Every uniform load and store is one global access with
+// a scalar base and a 32-bit lane offset.
+//
+// Even steps run in two kernels: aa_step_kernel skips every 64-node chunk
+// that contains a BOUNDARY node, and aa_boundary_kernel updates exactly those
+// chunks (same test), so the bulk kernel carries no bounce-back code at all.
+// Odd steps need none (see the layout notes above) and run in aa_step_kernel
+// alone.
 // ---------------------------------------------------------------------------
 #ifndef LBM_STEP_WAVES_PER_EU
 #define LBM_STEP_WAVES_PER_EU 4
 #endif
 
-template <class S, bool ODD, int MODE>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(LBM_STEP_WAVES_PER_EU)))
-void aa_step_kernel(const StepArgs<S> A, const FastCoef K) {
+struct RowGeom {
+    uint32_t xo[3], xb[3], yo[3], zo[3], n;
+};
+
+template <class S>
+__device__ __forceinline__ RowGeom row_geom(uint32_t x, uint32_t y, uint32_t z, uint32_t NX, uint32_t NY,
+                                            uint32_t NZ) {
+    RowGeom g;
+    const uint32_t NXY = NX * NY;
+    g.yo[0] = (y == 0 ? NY - 1 : y - 1) * NX; g.yo[1] = y * NX; g.yo[2] = (y + 1 == NY ? 0u : y + 1) * NX;
+    g.zo[0] = (z == 0 ? NZ - 1 : z - 1) * NXY; g.zo[1] = z * NXY; g.zo[2] = (z + 1 == NZ ? 0u : z + 1) * NXY;
+    g.xo[0] = x == 0 ? NX - 1 : x - 1; g.xo[1] = x; g.xo[2] = x + 1 == NX ? 0u : x + 1;
+    for (int k = 0; k < 3; ++k) g.xb[k] = g.xo[k] * (uint32_t)sizeof(S);
+    g.n = x + g.yo[1] + g.zo[1];
+    return g;
+}
+
+// Update one fluid node; returns its excess mass sum_i f*_i - 1 when MASS.
+// BPATH: even-step node of a boundary chunk (bounce-back redirections).
+template <class S, bool ODD, int MODE, bool BPATH>
+__device__ __forceinline__ double node_update(const StepArgs<S>& A, const FastCoef& K, const RowGeom& rg, int x,
+                                              int y, int z, uint8_t fl) {
     using St = Store<S>;
     using T = typename St::C;
     constexpr bool EXACT = std::is_same<S, double>::value;
     constexpr bool MACRO = (MODE & STEP_MACRO) != 0;
+    constexpr bool MASS = (MODE & STEP_MASS) != 0;
+    const uint32_t SS = A.SS, n = rg.n;
+    S* __restrict__ f = A.f;
+    auto base = [&](int s, uint32_t row) -> char* { return (char*)(f + ((size_t)s * SS + row)); };
+    auto at = [](char* b, uint32_t off) -> S* { return (S*)(b + off); };
+    const uint32_t* xb = rg.xb;
+    const uint32_t* yo = rg.yo;
+    const uint32_t* zo = rg.zo;
+
+    // ---- gather the 19 populations node n collides ----
+    T g[Q];
+    static_for<0, Q>([&](auto I) {
+        constexpr int i = decltype(I)::value;
+        if constexpr (!ODD) g[i] = St::load(*at(base(i, yo[1] + zo[1]), xb[1]));
+        else g[i] = St::load(*at(base(OPP[i], yo[1 - CY[i]] + zo[1 - CZ[i]]), xb[1 - CX[i]]));
+    });
+    // even step, boundary node: populations whose upstream node is solid
+    // come back from that node's slot opp(j) (bounce-back)
+    uint32_t solid = 0;
+    if constexpr (BPATH) {
+        if (fl & BOUNDARY) {
+            static_for<1, Q>([&](auto I) {
+                constexpr int i = decltype(I)::value;
+                const uint32_t m = rg.xo[1 + CX[i]] + yo[1 + CY[i]] + zo[1 + CZ[i]];
+                solid |= (uint32_t)((A.flag[m] & TYPE_MASK) != FLUID) << i;
+            });
+            static_for<1, Q>([&](auto I) {
+                constexpr int j = decltype(I)::value;
+                if ((solid >> OPP[j]) & 1u)
+                    g[j] = St::load(*at(base(OPP[j], yo[1 - CY[j]] + zo[1 - CZ[j]]), xb[1 - CX[j]]));
+            });
+        }
+    }
+
+    // pending mass-correction shift (lbm.cpp:337), applied on load
+    const double dsh = A.st->shift[A.t & 1];
+    if (dsh != 0.0) {
+        static_for<0, Q>([&](auto I) {
+            constexpr int i = decltype(I)::value;
+            if constexpr (EXACT) g[i] = g[i] + W[i] * dsh;
+            else g[i] += (T)(W[i] * dsh * St::scale);
+        });
+    }
+
+    // ---- collide ----
+    double excess = 0.0;
+    T rho_t, m0, m1, m2;  // macroscopic outputs (real units)
+    if constexpr (EXACT) {
+        double r, u, v, wz;
+        collide_exact(g, A.omPlus, A.omMinus, A.Fx, A.Fy, A.Fz, r, u, v, wz);
+        rho_t = r; m0 = u; m1 = v; m2 = wz;
+        if constexpr (MASS) {
+            double s = 0.0;
+            static_for<0, Q>([&](auto I) { s += g[decltype(I)::value]; });
+            excess = s - 1.0;
+        }
+    } else {
+        float rho, jx, jy, jz, drs;
+        if (A.Fx != 0.0 || A.Fy != 0.0 || A.Fz != 0.0) collide_fast<true>(g, K, rho, jx, jy, jz, drs);
+        else collide_fast<false>(g, K, rho, jx, jy, jz, drs);
+        rho_t = rho;
+        if constexpr (MACRO) {
+            const float isr = K.inv_s / rho;
+            m0 = jx * isr; m1 = jy * isr; m2 = jz * isr;
+        } else {
+            m0 = m1 = m2 = 0.f;
+        }
+        if constexpr (MASS) excess = (double)drs * (double)K.inv_s;
+    }
+
+    // ---- macroscopic fields: all fluid nodes on request, else the band
+    // around the squirmer that the remap and the link kernel read
+    // (lbm.cpp:152, 198, 307)
+    if constexpr (MACRO) {
+        static_cast<T*>(A.rho)[n] = rho_t;
+        static_cast<T*>(A.ux)[n] = m0;
+        static_cast<T*>(A.uy)[n] = m1;
+        static_cast<T*>(A.uz)[n] = m2;
+    } else {
+        if (A.has_squirmer) {
+            const int* b = A.st->band;
+            int dx = x - b[0]; dx += dx < 0 ? A.NX : 0;
+            int dy = y - b[1]; dy += dy < 0 ? A.NY : 0;
+            int dz = z - b[2]; dz += dz < 0 ? A.NZ : 0;
+            if (dx < b[3] && dy < b[4] && dz < b[5]) static_cast<T*>(A.rho)[n] = rho_t;
+        }
+    }
+
+    // ---- stream ----
+    static_for<0, Q>([&](auto I) {
+        constexpr int i = decltype(I)::value;
+        const S val = St::store(g[i]);
+        if constexpr (!ODD) *at(base(OPP[i], yo[1] + zo[1]), xb[1]) = val;
+        else *at(base(i, yo[1 + CY[i]] + zo[1 + CZ[i]]), xb[1 + CX[i]]) = val;
+    });
+    // even step, boundary node: bounce-back into the solid neighbour's slot i
+    if constexpr (BPATH) {
+        if (solid) {
+            static_for<1, Q>([&](auto I) {
+                constexpr int i = decltype(I)::value;
+                if ((solid >> i) & 1u) *at(base(i, yo[1 + CY[i]] + zo[1 + CZ[i]]), xb[1 + CX[i]]) = St::store(g[i]);
+            });
+        }
+    }
+    return excess;
+}
+
+template <class S, bool ODD, int MODE>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(LBM_STEP_WAVES_PER_EU)))
+void aa_step_kernel(const StepArgs<S> A, const FastCoef K) {
     constexpr bool MASS = (MODE & STEP_MASS) != 0;
 
     // ---- tile: the hardware deals blocks to the 8 XCDs round robin; give
@@ -279,22 +448,16 @@ void aa_step_kernel(const StepArgs<S> A, const FastCoef K) {
     const int ly = __builtin_amdgcn_readfirstlane(threadIdx.x >> A.tx_shift);  // one row per wavefront
     const int x = (tgx << A.tx_shift) + lx, y = tgy * A.ty + ly, z = tgz;
 
-    const uint32_t NX = (uint32_t)A.NX, NY = (uint32_t)A.NY, NZ = (uint32_t)A.NZ, SS = A.SS;
-    const uint32_t NXY = NX * NY;
+    const uint32_t NX = (uint32_t)A.NX, NY = (uint32_t)A.NY, NZ = (uint32_t)A.NZ;
     const bool row_ok = (uint32_t)y < NY;  // wavefront-uniform
     const bool inb = row_ok && (uint32_t)x < NX;
-    const uint32_t uy_ = row_ok ? (uint32_t)y : 0u, uz_ = (uint32_t)z;
-    // wavefront-uniform row offsets of the neighbour rows (dy, dz in -1..1)
-    const uint32_t yo[3] = {(uy_ == 0 ? NY - 1 : uy_ - 1) * NX, uy_ * NX, (uy_ + 1 == NY ? 0u : uy_ + 1) * NX};
-    const uint32_t zo[3] = {(uz_ == 0 ? NZ - 1 : uz_ - 1) * NXY, uz_ * NXY, (uz_ + 1 == NZ ? 0u : uz_ + 1) * NXY};
-    // lane x offsets (x-1, x, x+1 with periodic wrap), as element and byte offsets
-    const uint32_t ux_ = inb ? (uint32_t)x : 0u;
-    const uint32_t xo[3] = {ux_ == 0 ? NX - 1 : ux_ - 1, ux_, ux_ + 1 == NX ? 0u : ux_ + 1};
-    const uint32_t xb[3] = {xo[0] * (uint32_t)sizeof(S), xo[1] * (uint32_t)sizeof(S), xo[2] * (uint32_t)sizeof(S)};
-    const uint32_t n = ux_ + yo[1] + zo[1];
-
-    const uint8_t fl = inb ? A.flag[n] : (uint8_t)WALL;
-    const bool active = (fl & TYPE_MASK) == FLUID;
+    const RowGeom rg = row_geom<S>(inb ? (uint32_t)x : 0u, row_ok ? (uint32_t)y : 0u, (uint32_t)z, NX, NY, NZ);
+    const uint8_t fl = inb ? A.flag[rg.n] : (uint8_t)WALL;
+    bool active = (fl & TYPE_MASK) == FLUID;
+    if constexpr (!ODD) {
+        // chunks with bounce-back links belong to aa_boundary_kernel
+        if (__builtin_amdgcn_ballot_w64(active && (fl & BOUNDARY)) != 0) active = false;
+    }
 
     if (tile == 0 && threadIdx.x == 0) {
         // clear the slot step t+1 reads its mass shift from, and the remap's
@@ -303,111 +466,72 @@ void aa_step_kernel(const StepArgs<S> A, const FastCoef K) {
         A.st->remap_dmass = 0.0;
     }
 
-    S* __restrict__ f = A.f;
-    // scalar base of population slot `s` shifted by a row offset
-    auto base = [&](int s, uint32_t row) -> char* { return (char*)(f + ((size_t)s * SS + row)); };
-    auto at = [](char* b, uint32_t off) -> S* { return (S*)(b + off); };
-
     double excess = 0.0;
-    if (active) {
-        // ---- gather the 19 populations node n collides ----
-        T g[Q];
-        static_for<0, Q>([&](auto I) {
-            constexpr int i = decltype(I)::value;
-            if constexpr (!ODD) g[i] = St::load(*at(base(i, yo[1] + zo[1]), xb[1]));
-            else g[i] = St::load(*at(base(OPP[i], yo[1 - CY[i]] + zo[1 - CZ[i]]), xb[1 - CX[i]]));
-        });
-
-        // even step, boundary node: populations whose upstream node is solid
-        // come back from that node's slot opp(j) (bounce-back)
-        uint32_t solid = 0;
-        if (!ODD && (fl & BOUNDARY)) {
-            static_for<1, Q>([&](auto I) {
-                constexpr int i = decltype(I)::value;
-                const uint32_t m = xo[1 + CX[i]] + yo[1 + CY[i]] + zo[1 + CZ[i]];
-                solid |= (uint32_t)((A.flag[m] & TYPE_MASK) != FLUID) << i;
-            });
-            static_for<1, Q>([&](auto I) {
-                constexpr int j = decltype(I)::value;
-                if ((solid >> OPP[j]) & 1u)
-                    g[j] = St::load(*at(base(OPP[j], yo[1 - CY[j]] + zo[1 - CZ[j]]), xb[1 - CX[j]]));
-            });
-        }
-
-        // pending mass-correction shift (lbm.cpp:337), applied on load
-        const double dsh = A.st->shift[A.t & 1];
-        if (dsh != 0.0) {
-            static_for<0, Q>([&](auto I) {
-                constexpr int i = decltype(I)::value;
-                if constexpr (EXACT) g[i] = g[i] + W[i] * dsh;
-                else g[i] += (T)(W[i] * dsh * St::scale);
-            });
-        }
-
-        // ---- collide ----
-        T rho_t, m0, m1, m2;  // macroscopic outputs (real units)
-        if constexpr (EXACT) {
-            double r, u, v, wz;
-            collide_exact(g, A.omPlus, A.omMinus, A.Fx, A.Fy, A.Fz, r, u, v, wz);
-            rho_t = r; m0 = u; m1 = v; m2 = wz;
-            if constexpr (MASS) {
-                double s = 0.0;
-                static_for<0, Q>([&](auto I) { s += g[decltype(I)::value]; });
-                excess = s - 1.0;
-            }
-        } else {
-            float rho, jx, jy, jz, drs;
-            if (A.Fx != 0.0 || A.Fy != 0.0 || A.Fz != 0.0) collide_fast<true>(g, K, rho, jx, jy, jz, drs);
-            else collide_fast<false>(g, K, rho, jx, jy, jz, drs);
-            rho_t = rho;
-            if constexpr (MACRO) {
-                const float isr = K.inv_s / rho;
-                m0 = jx * isr; m1 = jy * isr; m2 = jz * isr;
-            } else {
-                m0 = m1 = m2 = 0.f;
-            }
-            if constexpr (MASS) excess = (double)drs * (double)K.inv_s;
-        }
-
-        // ---- macroscopic fields: all fluid nodes on request, else the
-        // band around the squirmer that the remap and the link kernel read
-        // (lbm.cpp:152, 198, 307)
-        if constexpr (MACRO) {
-            static_cast<T*>(A.rho)[n] = rho_t;
-            static_cast<T*>(A.ux)[n] = m0;
-            static_cast<T*>(A.uy)[n] = m1;
-            static_cast<T*>(A.uz)[n] = m2;
-        } else {
-            if (A.has_squirmer) {
-                const int* b = A.st->band;
-                int dx = x - b[0]; dx += dx < 0 ? (int)NX : 0;
-                int dy = y - b[1]; dy += dy < 0 ? (int)NY : 0;
-                int dz = z - b[2]; dz += dz < 0 ? (int)NZ : 0;
-                if (dx < b[3] && dy < b[4] && dz < b[5]) static_cast<T*>(A.rho)[n] = rho_t;
-            }
-        }
-
-        // ---- stream ----
-        static_for<0, Q>([&](auto I) {
-            constexpr int i = decltype(I)::value;
-            const S val = St::store(g[i]);
-            if constexpr (!ODD) *at(base(OPP[i], yo[1] + zo[1]), xb[1]) = val;
-            else *at(base(i, yo[1 + CY[i]] + zo[1 + CZ[i]]), xb[1 + CX[i]]) = val;
-        });
-        // even step, boundary node: bounce-back into the solid neighbour's slot i
-        if (!ODD && solid) {
-            static_for<1, Q>([&](auto I) {
-                constexpr int i = decltype(I)::value;
-                if ((solid >> i) & 1u) *at(base(i, yo[1 + CY[i]] + zo[1 + CZ[i]]), xb[1 + CX[i]]) = St::store(g[i]);
-            });
-        }
-    }
+    if (active) excess = node_update<S, ODD, MODE, false>(A, K, rg, x, y, z, fl);
 
     if constexpr (MASS) {
         __shared__ double lds[BLOCK / 64];
         double m[1] = {excess};
         block_reduce<1>(m, lds);
         if (threadIdx.x == 0) A.mass_part[tile] = m[0];
+    }
+}
+
+// Boundary chunks of an even step.  Work list: every 64-node chunk of the
+// rows next to a wall (static, `wall_rows`), then every chunk of the rows of
+// the squirmer's link box that are not wall rows (from the device state).
+struct BoundaryArgs {
+    const uint32_t* wall_rows;  // y + NY * z of each row next to a wall
+    int n_wall_rows;
+    int nchx;                   // 64-node chunks per row
+    int box_rows_max, box_chunks;  // bound of box rows and chunks per box row
+    int wy_lo, wy_hi, wz_lo, wz_hi;  // wall-adjacent coordinates (-1: none)
+    double* mass_part;          // partials after the step kernel's
+};
+
+template <class S, int MODE>
+__global__ __launch_bounds__(BLOCK) void aa_boundary_kernel(const StepArgs<S> A, const FastCoef K,
+                                                            const BoundaryArgs B) {
+    constexpr bool MASS = (MODE & STEP_MASS) != 0;
+    const uint32_t NX = (uint32_t)A.NX, NY = (uint32_t)A.NY, NZ = (uint32_t)A.NZ;
+    const int wave = (int)(blockIdx.x * (BLOCK / 64)) + (int)(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    int y = -1, z = -1, chunk = 0;
+    const int nwall = B.n_wall_rows * B.nchx;
+    if (wave < nwall) {
+        const uint32_t r = B.wall_rows[wave / B.nchx];
+        y = (int)(r % NY); z = (int)(r / NY);
+        chunk = wave % B.nchx;
+    } else if (A.has_squirmer) {
+        const int w = wave - nwall;
+        const DevSquirmer& sq = A.st->sq;
+        const LinkBoxDims bd = link_box_dims(sq, A.NX, A.NY, A.NZ);
+        const int row = w / B.box_chunks, k = w % B.box_chunks;
+        if (row < bd.wy * bd.wz && k < bd.nck) {
+            const int iy = row % bd.wy, iz = row / bd.wy;
+            y = wrap_coord(bd.y0 + iy, A.NY);
+            z = wrap_coord(bd.z0 + iz, A.NZ);
+            if (y == B.wy_lo || y == B.wy_hi || z == B.wz_lo || z == B.wz_hi) y = -1;  // a wall row: done above
+            chunk = (bd.c0 + k) % B.nchx;
+        }
+    }
+    y = __builtin_amdgcn_readfirstlane(y);
+    z = __builtin_amdgcn_readfirstlane(z);
+    chunk = __builtin_amdgcn_readfirstlane(chunk);
+    const int x = chunk * 64 + lane;
+    const bool inb = y >= 0 && (uint32_t)x < NX;
+    const RowGeom rg = row_geom<S>(inb ? (uint32_t)x : 0u, y >= 0 ? (uint32_t)y : 0u, y >= 0 ? (uint32_t)z : 0u,
+                                   NX, NY, NZ);
+    const uint8_t fl = inb ? A.flag[rg.n] : (uint8_t)WALL;
+    bool active = (fl & TYPE_MASK) == FLUID;
+    if (__builtin_amdgcn_ballot_w64(active && (fl & BOUNDARY)) == 0) active = false;  // not a boundary chunk
+    double excess = 0.0;
+    if (active) excess = node_update<S, false, MODE, true>(A, K, rg, x, y, z, fl);
+    if constexpr (MASS) {
+        __shared__ double lds[BLOCK / 64];
+        double m[1] = {excess};
+        block_reduce<1>(m, lds);
+        if (threadIdx.x == 0) B.mass_part[blockIdx.x] = m[0];
     }
 }
 
