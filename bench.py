@@ -102,6 +102,23 @@ def cpu_baseline(budget_s: float = 12.0) -> dict | None:
                        f"512x256x128 workload after 2 untimed steps, FP64, {cores} OpenMP threads")
 
 
+TRAFFIC_FILE = ROOT / "profiles" / "hbm_traffic_aa_step_512x256x128.txt"
+
+
+def measured_traffic(precision: str):
+    """HBM bytes per launch of the collide-stream kernel (mean of the two AA
+    parities) from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs recorded in
+    profiles/, scaled by the counters' reading of a 1 GiB copy."""
+    try:
+        for line in TRAFFIC_FILE.read_text().splitlines():
+            p = line.split()
+            if len(p) == 3 and p[0] == "traffic_bytes_per_launch" and p[1] == precision:
+                return float(p[2])
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -166,7 +183,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(head["frac"], 4),
-            "traffic": None,
+            "traffic": measured_traffic(head["precision"]),
             "kernel": "aa_step_kernel",
             "bytes_per_lup": head["bytes_per_lup"],
             "kernel_ms": round(head["kernel_ms"], 5),
