@@ -36,9 +36,11 @@ WORKLOAD = dict(nx=512, ny=256, nz=128, tau=1.0, walls_y=0, walls_z=1, squirmer=
 MASS_EVERY = 10
 
 
-def measure(lbm, precision: str, steps: int, warmup: int) -> dict:
+def measure(lbm, precision: str, steps: int, warmup: int, tune: bool = True) -> dict:
     with lbm.Lattice(**WORKLOAD, precision=precision) as L:
         info = L.info()
+        # tile shape chosen by timing the candidates (state left unchanged)
+        tuning = L.autotune(trial_steps=20, mass_every=MASS_EVERY) if tune else None
         if warmup:
             L.time_run(warmup, MASS_EVERY)
         L.synchronize()
@@ -57,7 +59,38 @@ def measure(lbm, precision: str, steps: int, warmup: int) -> dict:
         kernel_ms=kern_ms / steps, kernel_mlups=N / kern_avg_s / 1e6,
         bytes_per_lup=b_lup, achieved_gbs=achieved, frac=achieved / HBM_PEAK_GBS,
         U=list(sq["U"]), X=list(sq["X"]),
+        tile_width=tuning["tile_width"] if tuning else None, xcd_remap=tuning["xcd_remap"] if tuning else None,
+        tuning=tuning,
     )
+
+
+def measure_ensemble(lbm, steps: int, warmup: int, count: int = 64) -> dict:
+    """The 64-run slit-channel ensemble (seed 0, FP16 store): every phase of a
+    step is one launch for all runs, the step cycle graph-replayed, tile shape
+    autotuned.  Beside it, the same runs advanced one after another, each on
+    its own (graph-replayed, autotuned) lattice."""
+    members = lbm.ensemble_members(count, 0)
+    with lbm.Batch(members, precision="fp16") as B:
+        info = B.info()
+        tuning = B.autotune(trial_steps=10, mass_every=MASS_EVERY)
+        if warmup:
+            B.time_run(warmup, MASS_EVERY)
+        t0 = time.perf_counter()
+        ms = B.time_run(steps, MASS_EVERY)
+        wall = time.perf_counter() - t0
+        info = B.info()
+    nodes = int(info["total_nodes"])
+    seq_ms = 0.0
+    for kv in members:
+        with lbm.Lattice(**kv, precision="fp16") as L:
+            L.autotune(trial_steps=10, mass_every=MASS_EVERY)
+            L.time_run(warmup or 10, MASS_EVERY)
+            tot, _ = L.time_run(steps, MASS_EVERY)
+            seq_ms += tot
+    return dict(runs=count, total_nodes=nodes, steps=steps, total_ms=ms, wall_s=wall, ms_per_step=ms / steps,
+                mlups=nodes * steps / (ms * 1e3), sequential_ms_per_step=seq_ms / steps,
+                sequential_mlups=nodes * steps / (seq_ms * 1e3), tuning=tuning,
+                device_gb=info["device_bytes"] / 1e9)
 
 
 def cpu_baseline(budget_s: float = 12.0) -> dict | None:
@@ -126,6 +159,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--precisions", default="fp16,fp32")
+    ap.add_argument("--workload", default="channel", choices=["channel", "ensemble"],
+                    help="channel: the 512x256x128 slit channel (headline); ensemble: 64 slit-channel runs batched")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device copy (torch) to calibrate rocprofv3 FETCH/WRITE counters")
     args = ap.parse_args()
@@ -145,6 +180,22 @@ def main():
 
     import lbm_solver_3d_amd as lbm
     lbm.load_library()
+    if args.workload == "ensemble":
+        print(f"[bench] ensemble: warmup {args.warmup}, timed {args.steps}", file=sys.stderr, flush=True)
+        e = measure_ensemble(lbm, args.steps, args.warmup)
+        print(json.dumps({
+            "metric": "MLUPS, batched ensemble of 64 slit-channel squirmer runs (seed 0), FP16 store",
+            "value": round(e["mlups"], 1), "unit": "MLUPS", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": round(e["ms_per_step"], 5), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "64 runs from seed 0: H in [32,96], grid 4Hx2HxH, walls in z, R in [4,8], "
+                                   "beta in [-5,5], pitch in [0,45) deg, z0 in [2R,H-2R], B1=1e-3, FP16 store",
+                       "total_nodes": e["total_nodes"], "parallelism": "single GPU"},
+            "sequential": {"mlups": round(e["sequential_mlups"], 1),
+                           "ms_per_step": round(e["sequential_ms_per_step"], 5)},
+            "batch_over_sequential": round(e["mlups"] / e["sequential_mlups"], 3),
+            "tuning": e["tuning"], "device_gb": round(e["device_gb"], 2)}))
+        return
     res = {}
     for prec in args.precisions.split(","):
         print(f"[bench] {prec}: warmup {args.warmup}, timed {args.steps}", file=sys.stderr, flush=True)
@@ -187,6 +238,8 @@ def main():
             "kernel": "aa_step_kernel",
             "bytes_per_lup": head["bytes_per_lup"],
             "kernel_ms": round(head["kernel_ms"], 5),
+            "tile_width": head["tile_width"],
+            "xcd_remap": head["xcd_remap"],
         },
         "cpu_baseline": cpu,
     }
@@ -198,7 +251,11 @@ def main():
         out["fp16_over_fp32"] = round(res["fp16"]["mlups"] / res["fp32"]["mlups"], 3)
     for k, v in res.items():
         out.setdefault("per_precision", {})[k] = {kk: (round(vv, 6) if isinstance(vv, float) else vv)
-                                                  for kk, vv in v.items() if kk not in ("U", "X")}
+                                                  for kk, vv in v.items() if kk not in ("U", "X", "tuning")}
+        if v.get("tuning"):
+            out["per_precision"][k]["tuning_ms_per_step"] = {
+                f"{c['tile_width']}{'x' if c['xcd_remap'] else ''}": round(c["ms_per_step"], 5)
+                for c in v["tuning"]["candidates"]}
     print(json.dumps(out))
 
 

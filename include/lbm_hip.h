@@ -170,6 +170,69 @@ int lbm_hip_get_info(lbm_hip_ctx* ctx, lbm_hip_info* out);
 int lbm_hip_time_run(lbm_hip_ctx* ctx, int steps, int mass_every, double* total_ms,
                      double* step_kernel_ms);
 
+/* ---- tile-shape autotuning ----
+ * No counterpart in the reference (its OpenMP loop has no launch shape).
+ * Candidates: collide-stream tiles of 64 x 4, 128 x 2 and 256 x 1 nodes
+ * (tile_width 64 / 128 / 256), each with and without the XCD-contiguous
+ * tile order.  Each candidate runs `trial_steps` driver-loop steps (graph
+ * replayed, after one untimed cycle) from the same saved state; the state is
+ * restored afterwards, so tuning does not advance the simulation.  The
+ * fastest candidate stays selected. */
+#define LBM_HIP_TUNE_CANDIDATES 6
+typedef struct lbm_hip_tuning {
+    int tile_width;       /* selected: 64, 128 or 256 nodes along x */
+    int xcd_remap;        /* selected: 1 = contiguous tile range per XCD */
+    double ms_per_step;   /* selected candidate's time per step */
+    int cand_tile_width[LBM_HIP_TUNE_CANDIDATES];
+    int cand_xcd_remap[LBM_HIP_TUNE_CANDIDATES];
+    double cand_ms_per_step[LBM_HIP_TUNE_CANDIDATES];
+} lbm_hip_tuning;
+
+int lbm_hip_autotune(lbm_hip_ctx* ctx, int trial_steps, int mass_every, lbm_hip_tuning* out);
+
+/* Select a tile shape directly (tile_width 64 / 128 / 256, xcd_remap 0 / 1). */
+int lbm_hip_set_tiling(lbm_hip_ctx* ctx, int tile_width, int xcd_remap);
+
+/* ---- ensemble of lattices ----
+ * No counterpart in the reference (one run per process there); the members
+ * are what `count` separate lbm_hip_create calls would make, and each
+ * advances exactly as lbm_hip_run would advance it alone.  All members share
+ * one precision and device.  Every phase of a step (collide-stream, even-step
+ * boundary chunks, particle links, squirmer update + remap, mass-correction
+ * finalize) is one launch for all members, and one cycle of the driver loop
+ * is captured in a HIP graph and replayed. */
+typedef struct lbm_hip_batch lbm_hip_batch;
+
+typedef struct lbm_hip_batch_info {
+    int count;                /* members */
+    int precision;
+    long long steps_done;
+    long long total_nodes;
+    int total_tiles;          /* blocks of one collide-stream launch */
+    int boundary_blocks;      /* blocks of one even-step boundary launch */
+    int link_blocks;          /* blocks of one particle-link launch */
+    int tile_width, xcd_remap;
+    size_t device_bytes;
+} lbm_hip_batch_info;
+
+int lbm_hip_batch_create(const lbm_hip_params* members, int count, lbm_hip_batch** out);
+void lbm_hip_batch_destroy(lbm_hip_batch* b);
+/* `steps` driver-loop steps of every member (step, advance_squirmer,
+ * correct_mass every `mass_every` for members with a squirmer); the last
+ * step carries `last_flags` (LBM_STEP_* as in lbm_hip_step). */
+int lbm_hip_batch_run(lbm_hip_batch* b, int steps, int mass_every, int last_flags);
+int lbm_hip_batch_synchronize(lbm_hip_batch* b);
+/* Member i as a context for the getters and transfers (get_squirmer,
+ * download, diagnostics, ...).  Waits for the batch's work first.  The
+ * context stays owned by the batch (lbm_hip_destroy ignores it); advancing it
+ * on its own makes lbm_hip_batch_run fail. */
+int lbm_hip_batch_member(lbm_hip_batch* b, int i, lbm_hip_ctx** out);
+int lbm_hip_batch_info_get(lbm_hip_batch* b, lbm_hip_batch_info* out);
+/* Autotuning as lbm_hip_autotune, one tile shape for all members. */
+int lbm_hip_batch_autotune(lbm_hip_batch* b, int trial_steps, int mass_every, lbm_hip_tuning* out);
+/* Bench support: time `steps` graph-replayed steps with HIP events. */
+int lbm_hip_batch_time_run(lbm_hip_batch* b, int steps, int mass_every, double* total_ms);
+
 /* Message of the last failed call on this thread ("" if none). */
 const char* lbm_hip_last_error(void);
 

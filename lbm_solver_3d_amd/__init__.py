@@ -107,6 +107,27 @@ def exported_symbols() -> list[str]:
     return sorted(set(re.findall(r"\b(lbm_hip_\w+)\s*\(", txt)))
 
 
+TUNE_CANDIDATES = 6
+
+
+class Tuning(C.Structure):
+    _fields_ = [("tile_width", C.c_int), ("xcd_remap", C.c_int), ("ms_per_step", C.c_double),
+                ("cand_tile_width", C.c_int * TUNE_CANDIDATES), ("cand_xcd_remap", C.c_int * TUNE_CANDIDATES),
+                ("cand_ms_per_step", C.c_double * TUNE_CANDIDATES)]
+
+    def as_dict(self) -> dict:
+        return dict(tile_width=self.tile_width, xcd_remap=self.xcd_remap, ms_per_step=self.ms_per_step,
+                    candidates=[dict(tile_width=self.cand_tile_width[k], xcd_remap=self.cand_xcd_remap[k],
+                                     ms_per_step=self.cand_ms_per_step[k]) for k in range(TUNE_CANDIDATES)])
+
+
+class BatchInfo(C.Structure):
+    _fields_ = [("count", C.c_int), ("precision", C.c_int), ("steps_done", C.c_longlong),
+                ("total_nodes", C.c_longlong), ("total_tiles", C.c_int), ("boundary_blocks", C.c_int),
+                ("link_blocks", C.c_int), ("tile_width", C.c_int), ("xcd_remap", C.c_int),
+                ("device_bytes", C.c_size_t)]
+
+
 def load_library(path: Path | None = None):
     """Load liblbm_hip.so; raises if it has not been built."""
     global _lib
@@ -137,6 +158,18 @@ def load_library(path: Path | None = None):
     lib.lbm_hip_synchronize.argtypes = [ctx]
     lib.lbm_hip_get_info.argtypes = [ctx, C.POINTER(Info)]
     lib.lbm_hip_time_run.argtypes = [ctx, C.c_int, C.c_int, dp, dp]
+    lib.lbm_hip_autotune.argtypes = [ctx, C.c_int, C.c_int, C.POINTER(Tuning)]
+    lib.lbm_hip_set_tiling.argtypes = [ctx, C.c_int, C.c_int]
+    bt = C.c_void_p
+    lib.lbm_hip_batch_create.argtypes = [P, C.c_int, C.POINTER(C.c_void_p)]
+    lib.lbm_hip_batch_destroy.argtypes = [bt]
+    lib.lbm_hip_batch_destroy.restype = None
+    lib.lbm_hip_batch_run.argtypes = [bt, C.c_int, C.c_int, C.c_int]
+    lib.lbm_hip_batch_synchronize.argtypes = [bt]
+    lib.lbm_hip_batch_member.argtypes = [bt, C.c_int, C.POINTER(C.c_void_p)]
+    lib.lbm_hip_batch_info_get.argtypes = [bt, C.POINTER(BatchInfo)]
+    lib.lbm_hip_batch_autotune.argtypes = [bt, C.c_int, C.c_int, C.POINTER(Tuning)]
+    lib.lbm_hip_batch_time_run.argtypes = [bt, C.c_int, C.c_int, dp]
     lib.lbm_hip_last_error.restype = C.c_char_p
     lib.lbm_hip_last_error.argtypes = []
     if path is None:
@@ -177,12 +210,16 @@ def _dp(a):
 class Lattice:
     """One lattice on one GPU: the reference's class LBM over the C ABI."""
 
-    def __init__(self, **kv):
+    def __init__(self, _handle=None, **kv):
         self.lib = load_library()
-        self.params = make_params(**kv)
-        h = C.c_void_p()
-        _check(self.lib, self.lib.lbm_hip_create(C.byref(self.params), C.byref(h)))
-        self.h = h
+        self.owned = _handle is None
+        if self.owned:
+            self.params = make_params(**kv)
+            h = C.c_void_p()
+            _check(self.lib, self.lib.lbm_hip_create(C.byref(self.params), C.byref(h)))
+            self.h = h
+        else:  # a member of a Batch: a view, freed with the batch
+            self.h = _handle
         info = self.info()
         self.NX, self.NY, self.NZ, self.N = info.nx, info.ny, info.nz, int(info.n)
         self.nu, self.omPlus, self.omMinus = info.nu, info.om_plus, info.om_minus
@@ -190,7 +227,8 @@ class Lattice:
 
     def close(self):
         if getattr(self, "h", None):
-            self.lib.lbm_hip_destroy(self.h)
+            if self.owned:
+                self.lib.lbm_hip_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -283,3 +321,98 @@ class Lattice:
         tot, k = C.c_double(), C.c_double()
         _check(self.lib, self.lib.lbm_hip_time_run(self.h, steps, mass_every, C.byref(tot), C.byref(k)))
         return tot.value, k.value
+
+    def autotune(self, trial_steps: int = 20, mass_every: int = 10) -> dict:
+        """Pick the collide-stream tile shape by timing the candidates from
+        the current state; the state is left as it was."""
+        t = Tuning()
+        _check(self.lib, self.lib.lbm_hip_autotune(self.h, trial_steps, mass_every, C.byref(t)))
+        return t.as_dict()
+
+    def set_tiling(self, tile_width: int, xcd_remap: int = 1):
+        _check(self.lib, self.lib.lbm_hip_set_tiling(self.h, tile_width, xcd_remap))
+
+
+class Batch:
+    """An ensemble of lattices advanced together (one launch per phase for
+    all members, graph-replayed time step)."""
+
+    def __init__(self, members: list[dict], precision: str = "fp16"):
+        self.lib = load_library()
+        arr = (Params * len(members))()
+        for k, kv in enumerate(members):
+            kv = dict(kv)
+            kv.setdefault("precision", precision)
+            arr[k] = make_params(**kv)
+        h = C.c_void_p()
+        _check(self.lib, self.lib.lbm_hip_batch_create(arr, len(members), C.byref(h)))
+        self.h = h
+        self.count = len(members)
+        self._members = [None] * self.count
+
+    def close(self):
+        if getattr(self, "h", None):
+            for m in self._members:
+                if m is not None:
+                    m.h = None
+            self.lib.lbm_hip_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def run(self, steps: int, mass_every: int = 10, last_flags: int = 0):
+        _check(self.lib, self.lib.lbm_hip_batch_run(self.h, steps, mass_every, last_flags))
+
+    def synchronize(self):
+        _check(self.lib, self.lib.lbm_hip_batch_synchronize(self.h))
+
+    def member(self, i: int) -> Lattice:
+        if self._members[i] is None:
+            hm = C.c_void_p()
+            _check(self.lib, self.lib.lbm_hip_batch_member(self.h, i, C.byref(hm)))
+            self._members[i] = Lattice(_handle=hm)
+        else:
+            self.synchronize()
+        return self._members[i]
+
+    def info(self) -> dict:
+        o = BatchInfo()
+        _check(self.lib, self.lib.lbm_hip_batch_info_get(self.h, C.byref(o)))
+        return {k: getattr(o, k) for k, _ in BatchInfo._fields_}
+
+    def autotune(self, trial_steps: int = 20, mass_every: int = 10) -> dict:
+        t = Tuning()
+        _check(self.lib, self.lib.lbm_hip_batch_autotune(self.h, trial_steps, mass_every, C.byref(t)))
+        return t.as_dict()
+
+    def time_run(self, steps: int, mass_every: int = 10) -> float:
+        tot = C.c_double()
+        _check(self.lib, self.lib.lbm_hip_batch_time_run(self.h, steps, mass_every, C.byref(tot)))
+        return tot.value
+
+
+def ensemble_members(count: int = 64, seed: int = 0) -> list[dict]:
+    """The batched-ensemble workload: `count` slit-channel squirmer runs drawn
+    with numpy's default_rng(seed), in order per run: channel height H
+    (integer in [32, 96]), radius R (integer in [4, 8]), beta (uniform in
+    [-5, 5]), pitch toward the wall (uniform in [0, 45) deg), centre height
+    (uniform in [2R, H - 2R]).  Grid 4H x 2H x H, walls in z, x and y
+    periodic, B1 = 1e-3, tau = 1, hard gap 1, pitch in the x-z plane."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        H = int(rng.integers(32, 97))
+        R = int(rng.integers(4, 9))
+        beta = float(rng.uniform(-5.0, 5.0))
+        pitch = float(rng.uniform(0.0, 45.0))
+        z0 = float(rng.uniform(2 * R, H - 2 * R))
+        out.append(dict(nx=4 * H, ny=2 * H, nz=H, tau=1.0, walls_y=0, walls_z=1, squirmer=1, radius=float(R),
+                        b1=1e-3, beta=beta, angle=pitch, z0=z0, planar=1, plane_xz=1, eps=0.2, hc=2.0, hmin=1.0))
+    return out

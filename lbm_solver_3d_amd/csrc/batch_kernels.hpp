@@ -1,0 +1,80 @@
+// batch_kernels.hpp -- one launch per phase for an ensemble of lattices.
+//
+// Every member lattice keeps its own arrays and device state; a table of
+// per-member argument blocks (BatchMember) sits in device memory.  Each
+// launch covers the work of all members: a block reads which member and
+// which of that member's tiles / blocks it is from a small map (member << 24
+// | local index), loads that member's arguments with uniform (scalar) loads
+// and runs the same body as the single-lattice kernel.  Members therefore
+// compute exactly what their own kernels compute; only the launch is shared.
+#pragma once
+#include "device_state.hpp"
+#include "links_kernel.hpp"
+#include "squirmer_kernel.hpp"
+#include "step_kernel.hpp"
+
+namespace lbmhip {
+
+template <class S>
+struct BatchMember {
+    StepArgs<S> A;      // step index set per launch
+    FastCoef K;
+    BoundaryArgs B;
+    LinkArgs<S> L;
+    RemapArgs<S> R;     // layout parity and shift slot set per launch
+    int lblocks;        // blocks of this member's link kernel
+    int has_squirmer;
+};
+
+// Per-member arguments of the mass-correction finalize.
+struct BatchFinalize {
+    const double* part;
+    DevState* st;
+    int G, bblocks, has_squirmer;
+};
+
+constexpr uint32_t BATCH_LOCAL_MASK = 0xFFFFFFu;
+
+template <class S, bool ODD, int MODE>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(LBM_STEP_WAVES_PER_EU)))
+void batch_step_kernel(const BatchMember<S>* __restrict__ M, const uint32_t* __restrict__ tile_map, int Gtot,
+                       long long t, int xcd) {
+    const int g = xcd ? xcd_tile(blockIdx.x, Gtot) : (int)blockIdx.x;
+    const uint32_t e = tile_map[g];
+    const BatchMember<S>& mb = M[e >> 24];
+    StepArgs<S> A = mb.A;
+    A.t = t;
+    step_tile<S, ODD, MODE>(A, mb.K, (int)(e & BATCH_LOCAL_MASK));
+}
+
+template <class S, int MODE>
+__global__ __launch_bounds__(BLOCK) void batch_boundary_kernel(const BatchMember<S>* __restrict__ M,
+                                                               const uint32_t* __restrict__ block_map, long long t) {
+    const uint32_t e = block_map[blockIdx.x];
+    const BatchMember<S>& mb = M[e >> 24];
+    StepArgs<S> A = mb.A;
+    A.t = t;
+    boundary_block<S, MODE>(A, mb.K, mb.B, (int)(e & BATCH_LOCAL_MASK));
+}
+
+template <class S>
+__global__ __launch_bounds__(LINK_BLOCK) void batch_links_kernel(const BatchMember<S>* __restrict__ M,
+                                                                 const uint32_t* __restrict__ block_map) {
+    const uint32_t e = block_map[blockIdx.x];
+    const BatchMember<S>& mb = M[e >> 24];
+    links_block<S>(mb.L, (int)(e & BATCH_LOCAL_MASK), mb.lblocks);
+}
+
+// One block per member; `t_after` is the number of steps done.
+template <class S>
+__global__ __launch_bounds__(SQ_THREADS) void batch_squirmer_kernel(const BatchMember<S>* __restrict__ M,
+                                                                    long long t_after) {
+    const BatchMember<S>& mb = M[blockIdx.x];
+    if (!mb.has_squirmer) return;
+    RemapArgs<S> R = mb.R;
+    R.shifted = (t_after > 0 && ((t_after - 1) & 1) == 0) ? 1 : 0;
+    R.shift_slot = (int)(t_after & 1);
+    squirmer_block<S>(R);
+}
+
+}  // namespace lbmhip

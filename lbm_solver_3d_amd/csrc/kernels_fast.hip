@@ -12,4 +12,13 @@ template hipError_t launch_boundary<_Float16>(const StepArgs<_Float16>&, const F
                                             hipStream_t);
 template hipError_t launch_boundary<bf16_t>(const StepArgs<bf16_t>&, const FastCoef&, const BoundaryArgs&, int, int,
                                             hipStream_t);
+#define LBMHIP_INSTANTIATE_BATCH(S)                                                                              \
+    template hipError_t launch_batch_step<S>(const BatchMember<S>*, const uint32_t*, int, long long, int, bool, \
+                                             int, hipStream_t);                                                 \
+    template hipError_t launch_batch_boundary<S>(const BatchMember<S>*, const uint32_t*, int, long long, int,    \
+                                                 hipStream_t);
+LBMHIP_INSTANTIATE_BATCH(float)
+LBMHIP_INSTANTIATE_BATCH(_Float16)
+LBMHIP_INSTANTIATE_BATCH(bf16_t)
+#undef LBMHIP_INSTANTIATE_BATCH
 }  // namespace lbmhip

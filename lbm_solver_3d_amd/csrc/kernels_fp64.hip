@@ -66,7 +66,23 @@ hipError_t launch_fill_populations(S* f, uint32_t SS, hipStream_t s) {
     return hipGetLastError();
 }
 
+template <class S>
+hipError_t launch_batch_links(const BatchMember<S>* m, const uint32_t* map, int blocks, hipStream_t s) {
+    if (blocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL((batch_links_kernel<S>), dim3(blocks), dim3(LINK_BLOCK), 0, s, m, map);
+    return hipGetLastError();
+}
+
+template <class S>
+hipError_t launch_batch_squirmer(const BatchMember<S>* m, int count, long long t_after, hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL((batch_squirmer_kernel<S>), dim3(count), dim3(SQ_THREADS), 0, s, m, t_after);
+    return hipGetLastError();
+}
+
 #define LBMHIP_INSTANTIATE(S)                                                                                     \
+    template hipError_t launch_batch_links<S>(const BatchMember<S>*, const uint32_t*, int, hipStream_t);         \
+    template hipError_t launch_batch_squirmer<S>(const BatchMember<S>*, int, long long, hipStream_t);            \
     template hipError_t launch_squirmer<S>(const RemapArgs<S>&, hipStream_t);                                    \
     template hipError_t launch_links<S>(const LinkArgs<S>&, int, hipStream_t);                                   \
     template hipError_t launch_wall_force<S>(const S*, const uint8_t*, double*, DevState*, int, int, int,       \
@@ -88,14 +104,18 @@ LBMHIP_INSTANTIATE(bf16_t)
 template hipError_t launch_step<double>(const StepArgs<double>&, const FastCoef&, bool, int, hipStream_t);
 template hipError_t launch_boundary<double>(const StepArgs<double>&, const FastCoef&, const BoundaryArgs&, int, int,
                                             hipStream_t);
+template hipError_t launch_batch_step<double>(const BatchMember<double>*, const uint32_t*, int, long long, int, bool,
+                                              int, hipStream_t);
+template hipError_t launch_batch_boundary<double>(const BatchMember<double>*, const uint32_t*, int, long long, int,
+                                                  hipStream_t);
 
 // ---------------------------------------------------------------------------
 // Mass correction (lbm.cpp:324-339): d = 1 - m / n_fluid with
 // m = n_fluid + excess, so d = -excess / n_fluid; the shift is added to the
 // slot the next step reads.  One block, fixed summation order.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(1024) void mass_finalize_kernel(const double* part, int nparts, DevState* st,
-                                                             int use_remap_dmass, int shift_slot) {
+__device__ void mass_finalize_block(const double* part, int nparts, DevState* st, int use_remap_dmass,
+                                    int shift_slot) {
     __shared__ double lds[16];
     // eight independent partial sums per thread keep eight loads in flight;
     // the combination order is fixed, so the result is deterministic
@@ -117,6 +137,27 @@ __global__ __launch_bounds__(1024) void mass_finalize_kernel(const double* part,
         const long long nf = st->nfluid;
         if (nf > 0) st->shift[shift_slot] += -e / (double)nf;
     }
+}
+
+__global__ __launch_bounds__(1024) void mass_finalize_kernel(const double* part, int nparts, DevState* st,
+                                                             int use_remap_dmass, int shift_slot) {
+    mass_finalize_block(part, nparts, st, use_remap_dmass, shift_slot);
+}
+
+// One block per ensemble member; members without a squirmer take no mass
+// correction (main.cpp:62-64).  `t_after`: steps done, the last one with mass
+// partials.
+__global__ __launch_bounds__(1024) void batch_finalize_kernel(const BatchFinalize* fa, long long t_after) {
+    const BatchFinalize& f = fa[blockIdx.x];
+    if (!f.has_squirmer) return;
+    const bool last_odd = ((t_after - 1) & 1) != 0;
+    mass_finalize_block(f.part, f.G + (last_odd ? 0 : f.bblocks), f.st, 1, (int)(t_after & 1));
+}
+
+hipError_t launch_batch_finalize(const BatchFinalize* fa, int count, long long t_after, hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(batch_finalize_kernel, dim3(count), dim3(1024), 0, s, fa, t_after);
+    return hipGetLastError();
 }
 
 hipError_t launch_mass_finalize(const double* part, int nparts, DevState* st, int use_remap_dmass, int shift_slot,

@@ -5,6 +5,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "batch_kernels.hpp"
 #include "device_state.hpp"
 #include "links_kernel.hpp"
 #include "squirmer_kernel.hpp"
@@ -45,6 +46,19 @@ hipError_t launch_mass(const S* f, const uint8_t* flag, double* part, int NX, in
 template <class S>
 hipError_t launch_fill_populations(S* f, uint32_t SS, hipStream_t s);
 
+// Ensemble launches (batch_kernels.hpp): all members in one launch per phase.
+template <class S>
+hipError_t launch_batch_step(const BatchMember<S>* m, const uint32_t* tile_map, int Gtot, long long t, int xcd,
+                             bool odd, int mode, hipStream_t s);
+template <class S>
+hipError_t launch_batch_boundary(const BatchMember<S>* m, const uint32_t* block_map, int blocks, long long t,
+                                 int mode, hipStream_t s);
+template <class S>
+hipError_t launch_batch_links(const BatchMember<S>* m, const uint32_t* block_map, int blocks, hipStream_t s);
+template <class S>
+hipError_t launch_batch_squirmer(const BatchMember<S>* m, int count, long long t_after, hipStream_t s);
+hipError_t launch_batch_finalize(const BatchFinalize* fa, int count, long long t_after, hipStream_t s);
+
 // Storage-independent kernels (kernels_fp64.hip).
 hipError_t launch_mass_finalize(const double* part, int nparts, DevState* st, int use_remap_dmass, int shift_slot,
                                 hipStream_t s);
@@ -67,7 +81,13 @@ hipError_t launch_macro_to_double(const void* src, int is_double, double* dst, u
                                                  uint32_t, int, hipStream_t);                                   \
     extern template hipError_t launch_mass<S>(const S*, const uint8_t*, double*, int, int, int, uint32_t,        \
                                               uint32_t, int, const DevState*, int, hipStream_t);                \
-    extern template hipError_t launch_fill_populations<S>(S*, uint32_t, hipStream_t);
+    extern template hipError_t launch_fill_populations<S>(S*, uint32_t, hipStream_t);                         \
+    extern template hipError_t launch_batch_step<S>(const BatchMember<S>*, const uint32_t*, int, long long, int, \
+                                                    bool, int, hipStream_t);                                    \
+    extern template hipError_t launch_batch_boundary<S>(const BatchMember<S>*, const uint32_t*, int, long long,  \
+                                                        int, hipStream_t);                                      \
+    extern template hipError_t launch_batch_links<S>(const BatchMember<S>*, const uint32_t*, int, hipStream_t);  \
+    extern template hipError_t launch_batch_squirmer<S>(const BatchMember<S>*, int, long long, hipStream_t);
 
 LBMHIP_DECLARE(double)
 LBMHIP_DECLARE(float)

@@ -46,4 +46,49 @@ hipError_t launch_boundary(const StepArgs<S>& a, const FastCoef& k, const Bounda
     }
 }
 
+template <class S, bool ODD, int MODE>
+static hipError_t launch_batch_step_v(const BatchMember<S>* m, const uint32_t* map, int Gtot, long long t, int xcd,
+                                      hipStream_t s) {
+    hipLaunchKernelGGL((batch_step_kernel<S, ODD, MODE>), dim3(Gtot), dim3(BLOCK), 0, s, m, map, Gtot, t, xcd);
+    return hipGetLastError();
+}
+
+template <class S, bool ODD>
+static hipError_t launch_batch_step_p(const BatchMember<S>* m, const uint32_t* map, int Gtot, long long t, int xcd,
+                                      int mode, hipStream_t s) {
+    switch (mode & (STEP_MACRO | STEP_MASS)) {
+        case 0: return launch_batch_step_v<S, ODD, 0>(m, map, Gtot, t, xcd, s);
+        case STEP_MACRO: return launch_batch_step_v<S, ODD, STEP_MACRO>(m, map, Gtot, t, xcd, s);
+        case STEP_MASS: return launch_batch_step_v<S, ODD, STEP_MASS>(m, map, Gtot, t, xcd, s);
+        default: return launch_batch_step_v<S, ODD, STEP_MACRO | STEP_MASS>(m, map, Gtot, t, xcd, s);
+    }
+}
+
+template <class S>
+hipError_t launch_batch_step(const BatchMember<S>* m, const uint32_t* map, int Gtot, long long t, int xcd, bool odd,
+                             int mode, hipStream_t s) {
+    if (Gtot <= 0) return hipSuccess;
+    return odd ? launch_batch_step_p<S, true>(m, map, Gtot, t, xcd, mode, s)
+               : launch_batch_step_p<S, false>(m, map, Gtot, t, xcd, mode, s);
+}
+
+template <class S, int MODE>
+static hipError_t launch_batch_boundary_v(const BatchMember<S>* m, const uint32_t* map, int blocks, long long t,
+                                          hipStream_t s) {
+    hipLaunchKernelGGL((batch_boundary_kernel<S, MODE>), dim3(blocks), dim3(BLOCK), 0, s, m, map, t);
+    return hipGetLastError();
+}
+
+template <class S>
+hipError_t launch_batch_boundary(const BatchMember<S>* m, const uint32_t* map, int blocks, long long t, int mode,
+                                 hipStream_t s) {
+    if (blocks <= 0) return hipSuccess;
+    switch (mode & (STEP_MACRO | STEP_MASS)) {
+        case 0: return launch_batch_boundary_v<S, 0>(m, map, blocks, t, s);
+        case STEP_MACRO: return launch_batch_boundary_v<S, STEP_MACRO>(m, map, blocks, t, s);
+        case STEP_MASS: return launch_batch_boundary_v<S, STEP_MASS>(m, map, blocks, t, s);
+        default: return launch_batch_boundary_v<S, STEP_MACRO | STEP_MASS>(m, map, blocks, t, s);
+    }
+}
+
 }  // namespace lbmhip

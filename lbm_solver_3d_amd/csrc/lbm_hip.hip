@@ -71,6 +71,8 @@ struct lbm_hip_ctx {
     BoundaryArgs bargs{};
     int bblocks = 0;                // blocks of aa_boundary_kernel
     int gx = 0, gy = 0, gz = 0, tx_shift = 8, ty = 1, G = 0, nb256 = 0;
+    int xcd_remap = 1;
+    int G_max = 0;  // largest tile count over the tile shapes (mass_part holds G_max + bblocks)
     double omPlus = 0, omMinus = 0, nu = 0;
     FastCoef K{};
     long long t = 0;  // steps done
@@ -84,6 +86,7 @@ struct lbm_hip_ctx {
     // one captured cycle of the driver loop (step, advance, mass correction)
     hipGraphExec_t graph = nullptr;
     int graph_cycle = 0, graph_mass_every = -1;
+    bool owned_by_batch = false;  // member of an lbm_hip_batch
 };
 
 extern "C" {
@@ -202,6 +205,7 @@ StepArgs<S> step_args(const lbm_hip_ctx* c) {
     a.NX = c->NX; a.NY = c->NY; a.NZ = c->NZ; a.N = c->N; a.SS = c->SS;
     a.walls_y = c->p.walls_y; a.walls_z = c->p.walls_z; a.has_squirmer = c->has_squirmer;
     a.gx = c->gx; a.gy = c->gy; a.gz = c->gz; a.tx_shift = c->tx_shift; a.ty = c->ty;
+    a.xcd_remap = c->xcd_remap;
     a.omPlus = c->omPlus; a.omMinus = c->omMinus;
     a.Fx = c->p.fx; a.Fy = c->p.fy; a.Fz = c->p.fz;
     a.t = c->t;
@@ -248,6 +252,25 @@ uint32_t slot_stride(uint32_t N, size_t ssize) {
 }
 
 bool layout_shifted(const lbm_hip_ctx* c) { return c->t > 0 && ((c->t - 1) & 1) == 0; }
+
+// Tiles of a block of BLOCK threads covering 2^ts x (BLOCK >> ts) nodes.
+int tiles_for(int nx, int ny, int nz, int ts) {
+    const int ty = BLOCK >> ts;
+    return ((nx + (1 << ts) - 1) >> ts) * ((ny + ty - 1) / ty) * nz;
+}
+
+// Tile shape of the collide-stream kernel (ts = log2 of the tile width, 6..8).
+// The boundary kernel's mass partials follow the step kernel's.
+void set_tile_shape(lbm_hip_ctx* c, int ts) {
+    c->tx_shift = ts;
+    c->ty = BLOCK >> ts;
+    c->gx = (c->p.nx + (1 << ts) - 1) >> ts;
+    c->gy = (c->p.ny + c->ty - 1) / c->ty;
+    c->gz = c->p.nz;
+    c->G = c->gx * c->gy * c->gz;
+    c->bargs.mass_part = c->mass_part ? c->mass_part + c->G : nullptr;
+    if (c->graph) { hipGraphExecDestroy(c->graph); c->graph = nullptr; }
+}
 
 void free_ctx(lbm_hip_ctx* c) {
     if (!c) return;
@@ -303,12 +326,8 @@ int lbm_hip_create(const lbm_hip_params* pin, lbm_hip_ctx** out) {
     c->K = make_coef(c->omPlus, c->omMinus, pin->fx, pin->fy, pin->fz, scale);
 
     // tiles: one wavefront row-chunk along x, block of 256 threads
-    c->tx_shift = pin->nx <= 64 ? 6 : (pin->nx <= 128 ? 7 : 8);
-    c->ty = BLOCK >> c->tx_shift;
-    c->gx = (pin->nx + (1 << c->tx_shift) - 1) >> c->tx_shift;
-    c->gy = (pin->ny + c->ty - 1) / c->ty;
-    c->gz = pin->nz;
-    c->G = c->gx * c->gy * c->gz;
+    for (int ts = 6; ts <= 8; ++ts) c->G_max = std::max(c->G_max, tiles_for(pin->nx, pin->ny, pin->nz, ts));
+    set_tile_shape(c, pin->nx <= 64 ? 6 : (pin->nx <= 128 ? 7 : 8));
     c->nb256 = (int)((c->N + 255) / 256);
 
     // even-step boundary chunks: rows next to a wall, and the squirmer's box rows
@@ -352,7 +371,7 @@ int lbm_hip_create(const lbm_hip_params* pin, lbm_hip_ctx** out) {
         c->device_bytes += bytes;
         return true;
     };
-    const int nparts = std::max(c->G + c->bblocks, c->nb256);
+    const int nparts = std::max(c->G_max + c->bblocks, c->nb256);
     if (!dmalloc(&c->f, (size_t)Q * c->SS * c->ssize) || !dmalloc((void**)&c->flag, N) ||
         !dmalloc(&c->rho, N * c->msize) || !dmalloc(&c->ux, N * c->msize) || !dmalloc(&c->uy, N * c->msize) ||
         !dmalloc(&c->uz, N * c->msize) || !dmalloc((void**)&c->st, sizeof(DevState)) ||
@@ -448,7 +467,10 @@ int lbm_hip_create(const lbm_hip_params* pin, lbm_hip_ctx** out) {
     return LBM_HIP_OK;
 }
 
-void lbm_hip_destroy(lbm_hip_ctx* ctx) { free_ctx(ctx); }
+void lbm_hip_destroy(lbm_hip_ctx* ctx) {
+    if (ctx && ctx->owned_by_batch) return;  // freed with its batch
+    free_ctx(ctx);
+}
 
 int lbm_hip_step(lbm_hip_ctx* c, int flags) {
     if (!c) return fail(LBM_HIP_EINVAL, "null context");
@@ -796,6 +818,521 @@ int lbm_hip_time_run(lbm_hip_ctx* c, int steps, int mass_every, double* total_ms
     if (total_ms) *total_ms = ms;
     if (step_kernel_ms) *step_kernel_ms = ksum * (double)steps / ksteps;
     return check_device_error(c);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Saved device state (autotuning runs candidates from the same state).
+// ---------------------------------------------------------------------------
+namespace {
+
+struct Snapshot {
+    void* buf[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    size_t bytes[7] = {0, 0, 0, 0, 0, 0, 0};
+    long long t = 0;
+    bool macro_valid = false, wallf_valid = false, mass_valid = false;
+    int last_bparts = 0;
+};
+
+void snapshot_parts(lbm_hip_ctx* c, void* (&src)[7], size_t (&bytes)[7]) {
+    src[0] = c->f; bytes[0] = (size_t)Q * c->SS * c->ssize;
+    src[1] = c->flag; bytes[1] = c->N;
+    src[2] = c->rho; src[3] = c->ux; src[4] = c->uy; src[5] = c->uz;
+    for (int k = 2; k < 6; ++k) bytes[k] = (size_t)c->N * c->msize;
+    src[6] = c->st; bytes[6] = sizeof(DevState);
+}
+
+void snapshot_free(Snapshot& s) {
+    for (void*& q : s.buf)
+        if (q) { hipFree(q); q = nullptr; }
+}
+
+int snapshot_take(lbm_hip_ctx* c, Snapshot& s, hipStream_t stream) {
+    void* src[7];
+    snapshot_parts(c, src, s.bytes);
+    for (int k = 0; k < 7; ++k) {
+        if (hipMalloc(&s.buf[k], s.bytes[k]) != hipSuccess) {
+            s.buf[k] = nullptr;
+            snapshot_free(s);
+            return fail(LBM_HIP_ENOMEM, "autotune: cannot allocate the state copy");
+        }
+        HIP_TRY(hipMemcpyAsync(s.buf[k], src[k], s.bytes[k], hipMemcpyDeviceToDevice, stream));
+    }
+    s.t = c->t;
+    s.macro_valid = c->macro_valid; s.wallf_valid = c->wallf_valid; s.mass_valid = c->mass_valid;
+    s.last_bparts = c->last_bparts;
+    return LBM_HIP_OK;
+}
+
+int snapshot_restore(lbm_hip_ctx* c, const Snapshot& s, hipStream_t stream) {
+    void* dst[7];
+    size_t bytes[7];
+    snapshot_parts(c, dst, bytes);
+    for (int k = 0; k < 7; ++k) HIP_TRY(hipMemcpyAsync(dst[k], s.buf[k], bytes[k], hipMemcpyDeviceToDevice, stream));
+    c->t = s.t;
+    c->macro_valid = s.macro_valid; c->wallf_valid = s.wallf_valid; c->mass_valid = s.mass_valid;
+    c->last_bparts = s.last_bparts;
+    return LBM_HIP_OK;
+}
+
+int tile_shift_of(int width) { return width == 64 ? 6 : (width == 128 ? 7 : (width == 256 ? 8 : -1)); }
+
+const int kTuneWidth[LBM_HIP_TUNE_CANDIDATES] = {64, 128, 256, 64, 128, 256};
+const int kTuneXcd[LBM_HIP_TUNE_CANDIDATES] = {1, 1, 1, 0, 0, 0};
+
+int graph_cycle_of(bool squirmer, int mass_every) {
+    return (squirmer && mass_every > 0) ? ((mass_every % 2 == 0) ? mass_every : 2 * mass_every) : 2;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lbm_hip_set_tiling(lbm_hip_ctx* c, int tile_width, int xcd_remap) {
+    if (!c) return fail(LBM_HIP_EINVAL, "null context");
+    const int ts = tile_shift_of(tile_width);
+    if (ts < 0) return fail(LBM_HIP_EINVAL, "tile_width must be 64, 128 or 256");
+    set_tile_shape(c, ts);
+    c->xcd_remap = xcd_remap ? 1 : 0;
+    c->mass_valid = false;  // partials of a pending mass step follow the old tiles
+    return LBM_HIP_OK;
+}
+
+int lbm_hip_autotune(lbm_hip_ctx* c, int trial_steps, int mass_every, lbm_hip_tuning* out) {
+    if (!c) return fail(LBM_HIP_EINVAL, "null context");
+    if (trial_steps < 1) return fail(LBM_HIP_EINVAL, "trial_steps must be >= 1");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    Snapshot snap;
+    int rc = snapshot_take(c, snap, c->stream);
+    if (rc) return rc;
+    while (c->events.size() < 2) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) { snapshot_free(snap); return fail(LBM_HIP_EDEVICE, "hipEventCreate"); }
+        c->events.push_back(e);
+    }
+    lbm_hip_tuning res{};
+    int best = -1;
+    const int warm = graph_cycle_of(c->has_squirmer, mass_every);
+    for (int k = 0; k < LBM_HIP_TUNE_CANDIDATES && rc == LBM_HIP_OK; ++k) {
+        res.cand_tile_width[k] = kTuneWidth[k];
+        res.cand_xcd_remap[k] = kTuneXcd[k];
+        if ((rc = lbm_hip_set_tiling(c, kTuneWidth[k], kTuneXcd[k]))) break;
+        if ((rc = snapshot_restore(c, snap, c->stream))) break;
+        if ((rc = run_loop(c, warm, mass_every, 0, true))) break;
+        float ms = 0.f;
+        if (hipEventRecord(c->events[0], c->stream) != hipSuccess) { rc = fail(LBM_HIP_EDEVICE, "event"); break; }
+        if ((rc = run_loop(c, trial_steps, mass_every, 0, true))) break;
+        if (hipEventRecord(c->events[1], c->stream) != hipSuccess || hipEventSynchronize(c->events[1]) != hipSuccess ||
+            hipEventElapsedTime(&ms, c->events[0], c->events[1]) != hipSuccess) {
+            rc = fail(LBM_HIP_EDEVICE, "autotune timing");
+            break;
+        }
+        res.cand_ms_per_step[k] = ms / trial_steps;
+        if (best < 0 || res.cand_ms_per_step[k] < res.cand_ms_per_step[best]) best = k;
+    }
+    if (rc == LBM_HIP_OK) rc = check_device_error(c);
+    if (best >= 0) {
+        const int r2 = lbm_hip_set_tiling(c, kTuneWidth[best], kTuneXcd[best]);
+        if (rc == LBM_HIP_OK) rc = r2;
+        res.tile_width = kTuneWidth[best];
+        res.xcd_remap = kTuneXcd[best];
+        res.ms_per_step = res.cand_ms_per_step[best];
+    }
+    const int r3 = snapshot_restore(c, snap, c->stream);
+    if (rc == LBM_HIP_OK) rc = r3;
+    hipStreamSynchronize(c->stream);
+    snapshot_free(snap);
+    if (rc == LBM_HIP_OK && out) *out = res;
+    return rc;
+}
+
+}  // extern "C"
+
+
+// ---------------------------------------------------------------------------
+// Ensemble of lattices: one launch per phase for all members.
+// ---------------------------------------------------------------------------
+struct lbm_hip_batch {
+    int device = 0, prec = 0, count = 0;
+    hipStream_t stream = nullptr;
+    std::vector<lbm_hip_ctx*> mem;
+    void* desc = nullptr;             // BatchMember<S>[count]
+    BatchFinalize* fin = nullptr;     // [count]
+    uint32_t* tile_map = nullptr;     // member << 24 | tile, sum of G_max entries
+    uint32_t* bmap = nullptr;         // member << 24 | boundary block
+    uint32_t* lmap = nullptr;         // member << 24 | link block
+    int Gtot = 0, Btot = 0, Ltot = 0;
+    bool any_squirmer = false;
+    long long t = 0;
+    hipGraphExec_t graph = nullptr;
+    int graph_cycle = 0, graph_mass_every = -1;
+    size_t device_bytes = 0;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+};
+
+namespace {
+
+void free_batch(lbm_hip_batch* b) {
+    if (!b) return;
+    hipSetDevice(b->device);
+    if (b->stream) hipStreamSynchronize(b->stream);
+    for (void* q : {b->desc, (void*)b->fin, (void*)b->tile_map, (void*)b->bmap, (void*)b->lmap})
+        if (q) hipFree(q);
+    if (b->graph) hipGraphExecDestroy(b->graph);
+    for (hipEvent_t e : b->ev)
+        if (e) hipEventDestroy(e);
+    for (lbm_hip_ctx* c : b->mem) {
+        if (!c) continue;
+        c->owned_by_batch = false;
+        free_ctx(c);
+    }
+    if (b->stream) hipStreamDestroy(b->stream);
+    delete b;
+}
+
+// (Re)build the member argument table and the block maps for the members'
+// current tile shapes.
+int batch_tables(lbm_hip_batch* b) {
+    std::vector<uint32_t> tmap, bm, lm;
+    std::vector<BatchFinalize> fin((size_t)b->count);
+    for (int m = 0; m < b->count; ++m) {
+        lbm_hip_ctx* c = b->mem[(size_t)m];
+        const uint32_t tag = (uint32_t)m << 24;
+        for (int k = 0; k < c->G; ++k) tmap.push_back(tag | (uint32_t)k);
+        for (int k = 0; k < c->bblocks; ++k) bm.push_back(tag | (uint32_t)k);
+        if (c->has_squirmer)
+            for (int k = 0; k < c->link_blocks; ++k) lm.push_back(tag | (uint32_t)k);
+        fin[(size_t)m] = BatchFinalize{c->mass_part, c->st, c->G, c->bblocks, c->has_squirmer ? 1 : 0};
+    }
+    b->Gtot = (int)tmap.size(); b->Btot = (int)bm.size(); b->Ltot = (int)lm.size();
+    HIP_TRY(hipMemcpyAsync(b->tile_map, tmap.data(), tmap.size() * sizeof(uint32_t), hipMemcpyHostToDevice, b->stream));
+    if (!bm.empty())
+        HIP_TRY(hipMemcpyAsync(b->bmap, bm.data(), bm.size() * sizeof(uint32_t), hipMemcpyHostToDevice, b->stream));
+    if (!lm.empty())
+        HIP_TRY(hipMemcpyAsync(b->lmap, lm.data(), lm.size() * sizeof(uint32_t), hipMemcpyHostToDevice, b->stream));
+    HIP_TRY(hipMemcpyAsync(b->fin, fin.data(), fin.size() * sizeof(BatchFinalize), hipMemcpyHostToDevice, b->stream));
+    int rc = LBM_HIP_OK;
+    dispatch(b->prec, [&](auto* tag) -> int {
+        using S = std::remove_pointer_t<decltype(tag)>;
+        std::vector<BatchMember<S>> d((size_t)b->count);
+        for (int m = 0; m < b->count; ++m) {
+            lbm_hip_ctx* c = b->mem[(size_t)m];
+            BatchMember<S>& x = d[(size_t)m];
+            x.A = step_args<S>(c);
+            x.K = c->K;
+            x.B = c->bargs;
+            x.L = link_args<S>(c);
+            x.R = remap_args<S>(c);
+            x.lblocks = c->link_blocks;
+            x.has_squirmer = c->has_squirmer ? 1 : 0;
+        }
+        if (hipMemcpyAsync(b->desc, d.data(), d.size() * sizeof(BatchMember<S>), hipMemcpyHostToDevice, b->stream) !=
+            hipSuccess)
+            rc = fail(LBM_HIP_EDEVICE, "batch: upload of member arguments failed");
+        // the host vector dies here: wait for the copy
+        if (rc == LBM_HIP_OK && hipStreamSynchronize(b->stream) != hipSuccess)
+            rc = fail(LBM_HIP_EDEVICE, "batch: synchronize failed");
+        return 0;
+    });
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    if (b->graph) { hipGraphExecDestroy(b->graph); b->graph = nullptr; }
+    return LBM_HIP_OK;
+}
+
+// One driver-loop step of every member (main.cpp:60-64).
+int batch_step(lbm_hip_batch* b, int mass_every, int extra_flags) {
+    const long long tnext = b->t + 1;
+    const bool mass = b->any_squirmer && mass_every > 0 && tnext % mass_every == 0;
+    int mode = mass ? STEP_MASS : 0;
+    if (extra_flags & LBM_STEP_MACRO) mode |= STEP_MACRO;
+    const bool odd = (b->t & 1) != 0;
+    hipError_t e = hipSuccess;
+    dispatch(b->prec, [&](auto* tag) -> int {
+        using S = std::remove_pointer_t<decltype(tag)>;
+        const auto* d = static_cast<const BatchMember<S>*>(b->desc);
+        e = launch_batch_step<S>(d, b->tile_map, b->Gtot, b->t, b->mem[0]->xcd_remap, odd, mode, b->stream);
+        if (e == hipSuccess && !odd) e = launch_batch_boundary<S>(d, b->bmap, b->Btot, b->t, mode, b->stream);
+        if (e == hipSuccess && b->any_squirmer) e = launch_batch_links<S>(d, b->lmap, b->Ltot, b->stream);
+        if (e == hipSuccess && (extra_flags & LBM_STEP_WALL_FORCE))
+            for (lbm_hip_ctx* c : b->mem)
+                if (e == hipSuccess)
+                    e = launch_wall_force<S>((const S*)c->f, c->flag, c->red_part, c->st, c->NX, c->NY, c->NZ, c->N,
+                                             c->SS, b->stream);
+        if (e == hipSuccess && b->any_squirmer) e = launch_batch_squirmer<S>(d, b->count, tnext, b->stream);
+        return 0;
+    });
+    if (e == hipSuccess && mass) e = launch_batch_finalize(b->fin, b->count, tnext, b->stream);
+    if (e != hipSuccess) return fail(LBM_HIP_EDEVICE, std::string("batch step launch: ") + hipGetErrorString(e));
+    b->t = tnext;
+    for (lbm_hip_ctx* c : b->mem) {
+        c->last_bparts = odd ? 0 : c->bblocks;
+        c->t = tnext;
+        c->macro_valid = (extra_flags & LBM_STEP_MACRO) != 0;
+        c->wallf_valid = (extra_flags & LBM_STEP_WALL_FORCE) != 0;
+        c->mass_valid = false;
+    }
+    return LBM_HIP_OK;
+}
+
+int batch_graph(lbm_hip_batch* b, int mass_every) {
+    if (b->graph && b->graph_mass_every == mass_every) return LBM_HIP_OK;
+    if (b->graph) { hipGraphExecDestroy(b->graph); b->graph = nullptr; }
+    const int cycle = graph_cycle_of(b->any_squirmer, mass_every);
+    const long long t_save = b->t;
+    std::vector<Snapshot> flags((size_t)b->count);  // host-side member fields only
+    for (int m = 0; m < b->count; ++m) {
+        lbm_hip_ctx* c = b->mem[(size_t)m];
+        flags[(size_t)m].t = c->t; flags[(size_t)m].last_bparts = c->last_bparts;
+        flags[(size_t)m].macro_valid = c->macro_valid; flags[(size_t)m].wallf_valid = c->wallf_valid;
+        flags[(size_t)m].mass_valid = c->mass_valid;
+    }
+    b->t = 0;  // capture with a cycle-aligned step index (the kernels use its parity only)
+    HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    int rc = LBM_HIP_OK;
+    for (int s = 0; s < cycle && rc == LBM_HIP_OK; ++s) rc = batch_step(b, mass_every, 0);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(b->stream, &g);
+    b->t = t_save;
+    for (int m = 0; m < b->count; ++m) {
+        lbm_hip_ctx* c = b->mem[(size_t)m];
+        c->t = flags[(size_t)m].t; c->last_bparts = flags[(size_t)m].last_bparts;
+        c->macro_valid = flags[(size_t)m].macro_valid; c->wallf_valid = flags[(size_t)m].wallf_valid;
+        c->mass_valid = flags[(size_t)m].mass_valid;
+    }
+    if (rc) { if (g) hipGraphDestroy(g); return rc; }
+    if (e != hipSuccess) return fail(LBM_HIP_EDEVICE, std::string("batch graph capture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        b->graph = nullptr;
+        return fail(LBM_HIP_EDEVICE, std::string("batch graph instantiate: ") + hipGetErrorString(e));
+    }
+    b->graph_cycle = cycle;
+    b->graph_mass_every = mass_every;
+    return LBM_HIP_OK;
+}
+
+int batch_loop(lbm_hip_batch* b, int steps, int mass_every, int last_flags) {
+    if (steps < 0) return fail(LBM_HIP_EINVAL, "steps must be >= 0");
+    for (lbm_hip_ctx* c : b->mem)
+        if (c->t != b->t) return fail(LBM_HIP_EINVAL, "a batch member was advanced outside the batch");
+    int rc, left = steps;
+    if (steps > 1) {
+        if ((rc = batch_graph(b, mass_every))) return rc;
+        const int C = b->graph_cycle;
+        while (left > 1 && (b->t % C) != 0) {
+            if ((rc = batch_step(b, mass_every, 0))) return rc;
+            --left;
+        }
+        const int reps = (left - 1) / C;
+        for (int r = 0; r < reps; ++r) HIP_TRY(hipGraphLaunch(b->graph, b->stream));
+        if (reps > 0) {
+            b->t += (long long)reps * C;
+            left -= reps * C;
+            for (lbm_hip_ctx* c : b->mem) {
+                c->t = b->t;
+                c->macro_valid = c->wallf_valid = c->mass_valid = false;
+                c->last_bparts = 0;  // a cycle ends on an odd step
+            }
+        }
+    }
+    while (left > 0) {
+        if ((rc = batch_step(b, mass_every, left == 1 ? last_flags : 0))) return rc;
+        --left;
+    }
+    return LBM_HIP_OK;
+}
+
+int batch_check(lbm_hip_batch* b) {
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    for (lbm_hip_ctx* c : b->mem) {
+        const int rc = check_device_error(c);
+        if (rc) return rc;
+    }
+    return LBM_HIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lbm_hip_batch_create(const lbm_hip_params* members, int count, lbm_hip_batch** out) {
+    if (!out) return fail(LBM_HIP_EINVAL, "null output pointer");
+    *out = nullptr;
+    if (!members || count < 1) return fail(LBM_HIP_EINVAL, "a batch needs at least one member");
+    if (count > 256) return fail(LBM_HIP_EINVAL, "a batch holds at most 256 members");
+    for (int m = 0; m < count; ++m) {
+        if (members[m].precision != members[0].precision)
+            return fail(LBM_HIP_EINVAL, "batch members must share one precision");
+        if (members[m].device != members[0].device) return fail(LBM_HIP_EINVAL, "batch members must share one device");
+        const int rc = lbm_hip_validate(&members[m]);
+        if (rc) return fail(rc, "member " + std::to_string(m) + ": " + g_err);
+    }
+    auto* b = new lbm_hip_batch();
+    b->device = members[0].device;
+    b->prec = members[0].precision;
+    b->count = count;
+    auto bail = [&](int code) {
+        std::string msg = g_err;
+        free_batch(b);
+        g_err = msg;
+        return code;
+    };
+    for (int m = 0; m < count; ++m) {
+        lbm_hip_ctx* c = nullptr;
+        const int rc = lbm_hip_create(&members[m], &c);
+        if (rc) { g_err = "member " + std::to_string(m) + ": " + g_err; return bail(rc); }
+        c->owned_by_batch = true;
+        b->mem.push_back(c);
+        b->device_bytes += c->device_bytes;
+        b->any_squirmer = b->any_squirmer || c->has_squirmer;
+        if (c->G_max >= (1 << 24) || c->bblocks >= (1 << 24) || c->link_blocks >= (1 << 24))
+            return bail(fail(LBM_HIP_EINVAL, "member " + std::to_string(m) + " too large for a batch"));
+    }
+    if (hipSetDevice(b->device) != hipSuccess) return bail(fail(LBM_HIP_EDEVICE, "hipSetDevice failed"));
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(LBM_HIP_EDEVICE, "hipStreamCreate failed"));
+    for (hipEvent_t& e : b->ev)
+        if (hipEventCreate(&e) != hipSuccess) return bail(fail(LBM_HIP_EDEVICE, "hipEventCreate failed"));
+    // members finished their set-up on their own streams
+    for (lbm_hip_ctx* c : b->mem)
+        if (hipStreamSynchronize(c->stream) != hipSuccess) return bail(fail(LBM_HIP_EDEVICE, "member set-up failed"));
+    size_t gmax = 0, bsum = 0, lsum = 0;
+    for (lbm_hip_ctx* c : b->mem) {
+        gmax += (size_t)c->G_max;
+        bsum += (size_t)c->bblocks;
+        lsum += c->has_squirmer ? (size_t)c->link_blocks : 0;
+    }
+    const size_t dsz = b->prec == LBM_HIP_FP64 ? sizeof(BatchMember<double>)
+                       : b->prec == LBM_HIP_FP32 ? sizeof(BatchMember<float>)
+                       : b->prec == LBM_HIP_FP16 ? sizeof(BatchMember<_Float16>)
+                                                 : sizeof(BatchMember<bf16_t>);
+    auto dmalloc = [&](void** q, size_t bytes) -> bool {
+        if (hipMalloc(q, bytes ? bytes : 4) != hipSuccess) { *q = nullptr; return false; }
+        b->device_bytes += bytes;
+        return true;
+    };
+    if (!dmalloc(&b->desc, dsz * (size_t)count) || !dmalloc((void**)&b->fin, sizeof(BatchFinalize) * (size_t)count) ||
+        !dmalloc((void**)&b->tile_map, sizeof(uint32_t) * gmax) || !dmalloc((void**)&b->bmap, sizeof(uint32_t) * bsum) ||
+        !dmalloc((void**)&b->lmap, sizeof(uint32_t) * lsum))
+        return bail(fail(LBM_HIP_ENOMEM, "batch: device allocation failed"));
+    const int rc = batch_tables(b);
+    if (rc) return bail(rc);
+    *out = b;
+    return LBM_HIP_OK;
+}
+
+void lbm_hip_batch_destroy(lbm_hip_batch* b) { free_batch(b); }
+
+int lbm_hip_batch_run(lbm_hip_batch* b, int steps, int mass_every, int last_flags) {
+    if (!b) return fail(LBM_HIP_EINVAL, "null batch");
+    return batch_loop(b, steps, mass_every, last_flags);
+}
+
+int lbm_hip_batch_synchronize(lbm_hip_batch* b) {
+    if (!b) return fail(LBM_HIP_EINVAL, "null batch");
+    return batch_check(b);
+}
+
+int lbm_hip_batch_member(lbm_hip_batch* b, int i, lbm_hip_ctx** out) {
+    if (!b || !out) return fail(LBM_HIP_EINVAL, "null argument");
+    if (i < 0 || i >= b->count) return fail(LBM_HIP_EINVAL, "member index out of range");
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    *out = b->mem[(size_t)i];
+    return LBM_HIP_OK;
+}
+
+int lbm_hip_batch_info_get(lbm_hip_batch* b, lbm_hip_batch_info* o) {
+    if (!b || !o) return fail(LBM_HIP_EINVAL, "null argument");
+    std::memset(o, 0, sizeof *o);
+    o->count = b->count;
+    o->precision = b->prec;
+    o->steps_done = b->t;
+    for (lbm_hip_ctx* c : b->mem) o->total_nodes += (long long)c->N;
+    o->total_tiles = b->Gtot;
+    o->boundary_blocks = b->Btot;
+    o->link_blocks = b->Ltot;
+    o->tile_width = 1 << b->mem[0]->tx_shift;
+    o->xcd_remap = b->mem[0]->xcd_remap;
+    o->device_bytes = b->device_bytes;
+    return LBM_HIP_OK;
+}
+
+int lbm_hip_batch_autotune(lbm_hip_batch* b, int trial_steps, int mass_every, lbm_hip_tuning* out) {
+    if (!b) return fail(LBM_HIP_EINVAL, "null batch");
+    if (trial_steps < 1) return fail(LBM_HIP_EINVAL, "trial_steps must be >= 1");
+    int rc = batch_check(b);
+    if (rc) return rc;
+    std::vector<Snapshot> snaps((size_t)b->count);
+    for (int m = 0; m < b->count && rc == LBM_HIP_OK; ++m) rc = snapshot_take(b->mem[(size_t)m], snaps[(size_t)m], b->stream);
+    const long long t_save = b->t;
+    auto restore_all = [&]() -> int {
+        for (int m = 0; m < b->count; ++m) {
+            const int r = snapshot_restore(b->mem[(size_t)m], snaps[(size_t)m], b->stream);
+            if (r) return r;
+        }
+        b->t = t_save;
+        return LBM_HIP_OK;
+    };
+    auto set_all = [&](int width, int xcd) -> int {
+        for (lbm_hip_ctx* c : b->mem) {
+            const int r = lbm_hip_set_tiling(c, width, xcd);
+            if (r) return r;
+        }
+        return batch_tables(b);
+    };
+    lbm_hip_tuning res{};
+    int best = -1;
+    const int warm = graph_cycle_of(b->any_squirmer, mass_every);
+    for (int k = 0; k < LBM_HIP_TUNE_CANDIDATES && rc == LBM_HIP_OK; ++k) {
+        res.cand_tile_width[k] = kTuneWidth[k];
+        res.cand_xcd_remap[k] = kTuneXcd[k];
+        if ((rc = set_all(kTuneWidth[k], kTuneXcd[k]))) break;
+        if ((rc = restore_all())) break;
+        if ((rc = batch_loop(b, warm, mass_every, 0))) break;
+        float ms = 0.f;
+        if (hipEventRecord(b->ev[0], b->stream) != hipSuccess) { rc = fail(LBM_HIP_EDEVICE, "event"); break; }
+        if ((rc = batch_loop(b, trial_steps, mass_every, 0))) break;
+        if (hipEventRecord(b->ev[1], b->stream) != hipSuccess || hipEventSynchronize(b->ev[1]) != hipSuccess ||
+            hipEventElapsedTime(&ms, b->ev[0], b->ev[1]) != hipSuccess) {
+            rc = fail(LBM_HIP_EDEVICE, "batch autotune timing");
+            break;
+        }
+        res.cand_ms_per_step[k] = ms / trial_steps;
+        if (best < 0 || res.cand_ms_per_step[k] < res.cand_ms_per_step[best]) best = k;
+    }
+    if (rc == LBM_HIP_OK) rc = batch_check(b);
+    if (best >= 0) {
+        const int r2 = set_all(kTuneWidth[best], kTuneXcd[best]);
+        if (rc == LBM_HIP_OK) rc = r2;
+        res.tile_width = kTuneWidth[best];
+        res.xcd_remap = kTuneXcd[best];
+        res.ms_per_step = res.cand_ms_per_step[best];
+    }
+    const int r3 = restore_all();
+    if (rc == LBM_HIP_OK) rc = r3;
+    hipStreamSynchronize(b->stream);
+    for (Snapshot& s : snaps) snapshot_free(s);
+    if (rc == LBM_HIP_OK && out) *out = res;
+    return rc;
+}
+
+int lbm_hip_batch_time_run(lbm_hip_batch* b, int steps, int mass_every, double* total_ms) {
+    if (!b) return fail(LBM_HIP_EINVAL, "null batch");
+    if (steps <= 0) return fail(LBM_HIP_EINVAL, "steps must be > 0");
+    int rc = batch_graph(b, mass_every);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    HIP_TRY(hipEventRecord(b->ev[0], b->stream));
+    if ((rc = batch_loop(b, steps, mass_every, 0))) return rc;
+    HIP_TRY(hipEventRecord(b->ev[1], b->stream));
+    HIP_TRY(hipEventSynchronize(b->ev[1]));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
+    if (total_ms) *total_ms = ms;
+    return batch_check(b);
 }
 
 }  // extern "C"

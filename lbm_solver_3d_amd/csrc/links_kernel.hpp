@@ -62,22 +62,24 @@ __device__ inline int wrap_once(int a, int n) {
 
 // Fixed-order combination of per-block partials by the last block to finish.
 // Returns true in the last block; `out` then holds the sums (thread 0).
+// `block` / `nblocks`: this block's index among the blocks that share the
+// ticket and the partials.
 template <int K>
 __device__ bool last_block_sum(double* part, unsigned* ticket, const double (&mine)[K], double (&out)[K],
-                               double* lds) {
+                               double* lds, unsigned block, unsigned nblocks) {
     __shared__ bool is_last;
     if (threadIdx.x == 0) {
-        for (int k = 0; k < K; ++k) part[(size_t)blockIdx.x * 8 + k] = mine[k];
+        for (int k = 0; k < K; ++k) part[(size_t)block * 8 + k] = mine[k];
         __threadfence();
         const unsigned t = atomicAdd(ticket, 1u);
-        is_last = t == gridDim.x - 1;
+        is_last = t == nblocks - 1;
     }
     __syncthreads();
     if (!is_last) return false;
     __threadfence();
     double v[K];
     for (int k = 0; k < K; ++k) v[k] = 0.0;
-    for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x)
+    for (unsigned b = threadIdx.x; b < nblocks; b += blockDim.x)
         for (int k = 0; k < K; ++k) v[k] += __builtin_nontemporal_load(&part[(size_t)b * 8 + k]);
     block_reduce<K>(v, lds);
     if (threadIdx.x == 0) {
@@ -93,7 +95,7 @@ __device__ bool last_block_sum(double* part, unsigned* ticket, const double (&mi
 // then every thread takes links of that list, so a wavefront's lanes all do
 // link work although only a thin shell of the box owns links.
 template <class S>
-__global__ __launch_bounds__(LINK_BLOCK) void particle_links_kernel(const LinkArgs<S> A) {
+__device__ __forceinline__ void links_block(const LinkArgs<S>& A, int block, int nblocks) {
     constexpr bool EXACT = std::is_same<S, double>::value;
     __shared__ double lds[(LINK_BLOCK / 64) * 7];
     __shared__ uint16_t lk[LINK_BLOCK * (Q - 1)];  // thread << 5 | direction
@@ -106,7 +108,7 @@ __global__ __launch_bounds__(LINK_BLOCK) void particle_links_kernel(const LinkAr
     const LinkBox b = link_box(sq, NX, NY, NZ);
     const int M = b.wx * b.wy * b.wz;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int k = blockIdx.x * LINK_BLOCK + tid;
+    const int k = block * LINK_BLOCK + tid;
     auto node_of = [&](int x, int y, int z) -> uint32_t {
         return (uint32_t)x + (uint32_t)NX * ((uint32_t)y + (uint32_t)NY * (uint32_t)z);
     };
@@ -181,10 +183,16 @@ __global__ __launch_bounds__(LINK_BLOCK) void particle_links_kernel(const LinkAr
     }
     block_reduce<7>(acc, lds);
     double tot[7];
-    if (last_block_sum<7>(A.part, &st->link_ticket, acc, tot, lds) && threadIdx.x == 0) {
+    if (last_block_sum<7>(A.part, &st->link_ticket, acc, tot, lds, (unsigned)block, (unsigned)nblocks) &&
+        threadIdx.x == 0) {
         for (int d = 0; d < 3; ++d) { st->sq.Fh[d] = tot[d]; st->sq.Th[d] = tot[3 + d]; }
         st->link_dmass = tot[6];
     }
+}
+
+template <class S>
+__global__ __launch_bounds__(LINK_BLOCK) void particle_links_kernel(const LinkArgs<S> A) {
+    links_block<S>(A, (int)blockIdx.x, (int)gridDim.x);
 }
 
 // Wall momentum exchange (lbm.cpp:209, 219): for every WALL link the bounced
@@ -215,7 +223,7 @@ __global__ __launch_bounds__(256) void wall_force_kernel(const S* __restrict__ f
     }
     block_reduce<3>(acc, lds);
     double tot[3];
-    if (last_block_sum<3>(part, &st->wall_ticket, acc, tot, lds) && threadIdx.x == 0)
+    if (last_block_sum<3>(part, &st->wall_ticket, acc, tot, lds, blockIdx.x, gridDim.x) && threadIdx.x == 0)
         for (int d = 0; d < 3; ++d) st->wallForce[d] = tot[d];
 }
 

@@ -99,7 +99,7 @@ __device__ inline void store_macro(void* p, uint32_t n, double v) {
 }
 
 template <class S>
-__global__ __launch_bounds__(SQ_THREADS) void squirmer_kernel(const RemapArgs<S> A) {
+__device__ void squirmer_block(const RemapArgs<S>& A) {
     __shared__ uint32_t list[REMAP_CAP];  // node | cover bit, in sweep order
     __shared__ int lpos[REMAP_CAP];       // box position of each entry (ascending)
     __shared__ double lrr[REMAP_CAP], lub[REMAP_CAP][3];
@@ -551,6 +551,11 @@ __global__ __launch_bounds__(SQ_THREADS) void squirmer_kernel(const RemapArgs<S>
         for (int d = 0; d < 3; ++d) st->xf[d] = sq.X[d];
         st->rf = sq.R;
     }
+}
+
+template <class S>
+__global__ __launch_bounds__(SQ_THREADS) void squirmer_kernel(const RemapArgs<S> A) {
+    squirmer_block<S>(A);
 }
 
 // ---------------------------------------------------------------------------

@@ -315,8 +315,6 @@ __device__ __forceinline__ RowGeom row_geom(uint32_t x, uint32_t y, uint32_t z, 
     return g;
 }
 
-// Update one fluid node; returns its excess mass sum_i f*_i - 1 when MASS.
-// BPATH: even-step node of a boundary chunk (bounce-back redirections).
 // Loads and stores that bypass the caches' retention.  The even step's bulk
 // reads each slot once and writes each slot once, with no reuse by other
 // wavefronts, where these hints let the in-place traffic stream faster; the
@@ -338,6 +336,8 @@ __device__ __forceinline__ void store_stream(S* p, S v) {
     else __builtin_nontemporal_store(v, p);
 }
 
+// Update one fluid node; returns its excess mass sum_i f*_i - 1 when MASS.
+// BPATH: even-step node of a boundary chunk (bounce-back redirections).
 template <class S, bool ODD, int MODE, bool BPATH>
 __device__ __forceinline__ double node_update(const StepArgs<S>& A, const FastCoef& K, const RowGeom& rg, int x,
                                               int y, int z, uint8_t fl) {
@@ -454,17 +454,18 @@ __device__ __forceinline__ double node_update(const StepArgs<S>& A, const FastCo
     return excess;
 }
 
-template <class S, bool ODD, int MODE>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(LBM_STEP_WAVES_PER_EU)))
-void aa_step_kernel(const StepArgs<S> A, const FastCoef K) {
-    constexpr bool MASS = (MODE & STEP_MASS) != 0;
-
-    // ---- tile: the hardware deals blocks to the 8 XCDs round robin; give
-    // each XCD a contiguous range of tiles so neighbouring rows share an L2.
-    const int G = A.gx * A.gy * A.gz;
-    const int lin = blockIdx.x;
+// The hardware deals blocks to the 8 XCDs round robin; this gives each XCD
+// a contiguous range of the G tiles, so neighbouring rows share an L2.
+__device__ __forceinline__ int xcd_tile(int lin, int G) {
     const int q8 = G >> 3, r8 = G & 7, xcd = lin & 7;
-    const int tile = xcd * q8 + (xcd < r8 ? xcd : r8) + (lin >> 3);
+    return xcd * q8 + (xcd < r8 ? xcd : r8) + (lin >> 3);
+}
+
+// One tile (one block of BLOCK threads, 2^tx_shift x ty nodes of a z plane)
+// of the collide-stream update.
+template <class S, bool ODD, int MODE>
+__device__ __forceinline__ void step_tile(const StepArgs<S>& A, const FastCoef& K, int tile) {
+    constexpr bool MASS = (MODE & STEP_MASS) != 0;
     const int tgx = tile % A.gx, trest = tile / A.gx;
     const int tgy = trest % A.gy, tgz = trest / A.gy;
     const int lx = threadIdx.x & ((1 << A.tx_shift) - 1);
@@ -500,6 +501,13 @@ void aa_step_kernel(const StepArgs<S> A, const FastCoef K) {
     }
 }
 
+template <class S, bool ODD, int MODE>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(LBM_STEP_WAVES_PER_EU)))
+void aa_step_kernel(const StepArgs<S> A, const FastCoef K) {
+    const int G = A.gx * A.gy * A.gz;
+    step_tile<S, ODD, MODE>(A, K, A.xcd_remap ? xcd_tile(blockIdx.x, G) : (int)blockIdx.x);
+}
+
 // Boundary chunks of an even step.  Work list: every 64-node chunk of the
 // rows next to a wall (static, `wall_rows`), then every chunk of the rows of
 // the squirmer's link box that are not wall rows (from the device state).
@@ -513,11 +521,11 @@ struct BoundaryArgs {
 };
 
 template <class S, int MODE>
-__global__ __launch_bounds__(BLOCK) void aa_boundary_kernel(const StepArgs<S> A, const FastCoef K,
-                                                            const BoundaryArgs B) {
+__device__ __forceinline__ void boundary_block(const StepArgs<S>& A, const FastCoef& K, const BoundaryArgs& B,
+                                               int block) {
     constexpr bool MASS = (MODE & STEP_MASS) != 0;
     const uint32_t NX = (uint32_t)A.NX, NY = (uint32_t)A.NY, NZ = (uint32_t)A.NZ;
-    const int wave = (int)(blockIdx.x * (BLOCK / 64)) + (int)(threadIdx.x >> 6);
+    const int wave = block * (BLOCK / 64) + (int)(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     int y = -1, z = -1, chunk = 0;
     const int nwall = B.n_wall_rows * B.nchx;
@@ -554,8 +562,14 @@ __global__ __launch_bounds__(BLOCK) void aa_boundary_kernel(const StepArgs<S> A,
         __shared__ double lds[BLOCK / 64];
         double m[1] = {excess};
         block_reduce<1>(m, lds);
-        if (threadIdx.x == 0) B.mass_part[blockIdx.x] = m[0];
+        if (threadIdx.x == 0) B.mass_part[block] = m[0];
     }
+}
+
+template <class S, int MODE>
+__global__ __launch_bounds__(BLOCK) void aa_boundary_kernel(const StepArgs<S> A, const FastCoef K,
+                                                            const BoundaryArgs B) {
+    boundary_block<S, MODE>(A, K, B, (int)blockIdx.x);
 }
 
 }  // namespace lbmhip

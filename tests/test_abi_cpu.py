@@ -61,3 +61,18 @@ def test_validate_accepts_reference_decks(lbm):
 def test_unknown_key_rejected(lbm):
     with pytest.raises(ValueError, match="unknown parameter"):
         lbm.make_params(not_a_key=1)
+
+
+def test_ensemble_members_are_deterministic_and_valid(lbm):
+    """The batched-ensemble workload (64 slit-channel runs from seed 0):
+    reproducible draws inside the stated ranges, every member valid."""
+    a, b = lbm.ensemble_members(64, 0), lbm.ensemble_members(64, 0)
+    assert a == b and len(a) == 64
+    assert a != lbm.ensemble_members(64, 1)
+    for m in a:
+        H, R = m["nz"], m["radius"]
+        assert 32 <= H <= 96 and (m["nx"], m["ny"]) == (4 * H, 2 * H)
+        assert 4 <= R <= 8 and R == int(R)
+        assert -5 <= m["beta"] <= 5 and 0 <= m["angle"] < 45
+        assert 2 * R <= m["z0"] <= H - 2 * R
+        lbm.validate(**m)

@@ -1,0 +1,122 @@
+"""The ensemble (lbm_hip_batch_*) and the tile-shape autotuner.
+
+Every member of a batch must advance exactly as the same lattice run alone
+through lbm_hip_run (same kernels, bodies and reduction partitions; only the
+launches are shared), and autotuning must leave the simulation state as it
+found it."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+MEMBERS = [
+    # slit channel with a pusher near the upper wall, pitched in x-z
+    dict(nx=40, ny=24, nz=20, walls_y=0, walls_z=1, squirmer=1, radius=4.0, b1=1.5e-2, beta=-3.0, angle=20.0,
+         z0=12.0, planar=1, plane_xz=1, hmin=1.0),
+    # periodic box with a neutral squirmer
+    dict(nx=24, ny=24, nz=24, walls_y=0, walls_z=0, squirmer=1, radius=4.0, b1=1.5e-2, beta=0.0, angle=0.0),
+    # duct without a squirmer, body force
+    dict(nx=4, ny=10, nz=12, tau=0.8, fx=5e-5, walls_y=1, walls_z=1),
+    # wide channel whose tiles are 256 nodes long
+    dict(nx=300, ny=20, nz=16, walls_y=1, walls_z=0, squirmer=1, radius=5.0, b1=1.5e-2, beta=1.0, angle=10.0),
+]
+
+
+def _state(L):
+    fl = L.flags()
+    out = dict(flag=fl, f=L.populations()[:, fl == 0])
+    if L.has_squirmer:
+        out["sq"] = L.squirmer()
+    return out
+
+
+def _same(a, b, rtol=0.0):
+    assert np.array_equal(a["flag"], b["flag"])
+    if rtol == 0.0:
+        assert np.array_equal(a["f"], b["f"]), float(np.abs(a["f"] - b["f"]).max())
+    else:
+        assert np.allclose(a["f"], b["f"], rtol=rtol, atol=rtol)
+    if "sq" in a:
+        for k in ("X", "e", "U", "W", "Fh", "Th"):
+            if rtol == 0.0:
+                assert np.array_equal(a["sq"][k], b["sq"][k]), k
+            else:
+                assert np.allclose(a["sq"][k], b["sq"][k], rtol=rtol, atol=1e-15), k
+        assert a["sq"]["contacts"] == b["sq"]["contacts"]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp16"])
+def test_batch_members_equal_separate_runs(lbm, prec):
+    steps = 37  # eager head, graph-replayed cycles, eager tail with moments
+    with lbm.Batch(MEMBERS, precision=prec) as B:
+        info = B.info()
+        assert info["count"] == len(MEMBERS)
+        assert info["total_nodes"] == sum(m["nx"] * m["ny"] * m["nz"] for m in MEMBERS)
+        B.run(steps, 10, lbm.STEP_ALL)
+        got = [_state(B.member(i)) for i in range(len(MEMBERS))]
+        assert B.info()["steps_done"] == steps
+    for i, kv in enumerate(MEMBERS):
+        with lbm.Lattice(**kv, precision=prec) as L:
+            L.run(steps, 10, lbm.STEP_ALL)
+            _same(got[i], _state(L))
+
+
+def test_batch_member_fields_after_macro_step(lbm):
+    with lbm.Batch(MEMBERS[:2], precision="fp32") as B:
+        B.run(12, 10, lbm.STEP_MACRO)
+        for i in range(2):
+            L = B.member(i)
+            d = L.diagnostics()
+            assert abs(d["mean_density"] - 1.0) < 1e-5
+            assert d["count_fluid"] == int((L.flags() == 0).sum())
+
+
+def test_batch_rejects_member_advanced_alone(lbm):
+    with lbm.Batch(MEMBERS[:2], precision="fp32") as B:
+        B.run(3, 10, 0)
+        B.member(1).step(0)
+        with pytest.raises(lbm.LbmHipError):
+            B.run(2, 10, 0)
+
+
+def test_batch_autotune_keeps_state_and_picks_a_candidate(lbm):
+    with lbm.Batch(MEMBERS, precision="fp64") as B:
+        B.run(6, 10, 0)
+        before = [_state(B.member(i)) for i in range(len(MEMBERS))]
+        t = B.autotune(trial_steps=4, mass_every=10)
+        assert t["tile_width"] in (64, 128, 256) and t["xcd_remap"] in (0, 1)
+        assert t["ms_per_step"] == min(c["ms_per_step"] for c in t["candidates"])
+        info = B.info()
+        assert (info["tile_width"], info["xcd_remap"]) == (t["tile_width"], t["xcd_remap"])
+        assert info["steps_done"] == 6
+        after = [_state(B.member(i)) for i in range(len(MEMBERS))]
+        for a, b in zip(before, after):
+            _same(a, b)
+        B.run(20, 10, 0)
+        tuned = [_state(B.member(i)) for i in range(len(MEMBERS))]
+    with lbm.Batch(MEMBERS, precision="fp64") as B:
+        B.run(26, 10, 0)
+        plain = [_state(B.member(i)) for i in range(len(MEMBERS))]
+    # another tile shape changes only the summation order of the mass partials
+    for a, b in zip(tuned, plain):
+        _same(a, b, rtol=1e-12)
+
+
+@pytest.mark.parametrize("width", [64, 128, 256])
+def test_tile_shapes_agree_without_squirmer(lbm, width):
+    kv = dict(nx=70, ny=18, nz=9, tau=0.8, fx=2e-5, walls_y=1, walls_z=1, precision="fp64")
+    with lbm.Lattice(**kv) as A, lbm.Lattice(**kv) as B:
+        B.set_tiling(width, 0)
+        A.run(25, 0, lbm.STEP_ALL)
+        B.run(25, 0, lbm.STEP_ALL)
+        assert np.array_equal(A.populations(), B.populations())
+
+
+def test_single_autotune_keeps_state(lbm):
+    kv = dict(MEMBERS[0], precision="fp16")
+    with lbm.Lattice(**kv) as L:
+        L.run(5, 10, 0)
+        a = _state(L)
+        t = L.autotune(trial_steps=4, mass_every=10)
+        assert t["tile_width"] in (64, 128, 256)
+        _same(a, _state(L))

@@ -3,9 +3,10 @@
 The FP64 mode runs the same AA in-place kernels as the fast modes, with the
 reference's operation order and no fused multiply-add.  Without a squirmer
 every population, density and velocity must be bit-identical to the oracle;
-with a squirmer the force/torque sums are reduced in a different order (tree
-+ atomics instead of a sequential sum), so populations are compared at
-1e-12 relative and node flags must be identical.
+with a squirmer the force/torque sums are reduced in a different (fixed)
+order -- per-block trees and a block-ordered sum instead of one sequential
+sum -- so populations are compared at 1e-12 relative and node flags must be
+identical.
 """
 import numpy as np
 import pytest
