@@ -64,7 +64,7 @@ def measure(lbm, precision: str, steps: int, warmup: int, tune: bool = True) -> 
     )
 
 
-def measure_ensemble(lbm, steps: int, warmup: int, count: int = 64) -> dict:
+def measure_ensemble(lbm, steps: int, warmup: int, count: int = 64, sequential: bool = True) -> dict:
     """The 64-run slit-channel ensemble (seed 0, FP16 store): every phase of a
     step is one launch for all runs, the step cycle graph-replayed, tile shape
     autotuned.  Beside it, the same runs advanced one after another, each on
@@ -81,15 +81,15 @@ def measure_ensemble(lbm, steps: int, warmup: int, count: int = 64) -> dict:
         info = B.info()
     nodes = int(info["total_nodes"])
     seq_ms = 0.0
-    for kv in members:
+    for kv in (members if sequential else []):
         with lbm.Lattice(**kv, precision="fp16") as L:
             L.autotune(trial_steps=10, mass_every=MASS_EVERY)
             L.time_run(warmup or 10, MASS_EVERY)
             tot, _ = L.time_run(steps, MASS_EVERY)
             seq_ms += tot
     return dict(runs=count, total_nodes=nodes, steps=steps, total_ms=ms, wall_s=wall, ms_per_step=ms / steps,
-                mlups=nodes * steps / (ms * 1e3), sequential_ms_per_step=seq_ms / steps,
-                sequential_mlups=nodes * steps / (seq_ms * 1e3), tuning=tuning,
+                mlups=nodes * steps / (ms * 1e3), sequential_ms_per_step=seq_ms / steps if sequential else None,
+                sequential_mlups=nodes * steps / (seq_ms * 1e3) if sequential else None, tuning=tuning,
                 device_gb=info["device_bytes"] / 1e9)
 
 
@@ -161,6 +161,8 @@ def main():
     ap.add_argument("--precisions", default="fp16,fp32")
     ap.add_argument("--workload", default="channel", choices=["channel", "ensemble"],
                     help="channel: the 512x256x128 slit channel (headline); ensemble: 64 slit-channel runs batched")
+    ap.add_argument("--no-sequential", action="store_true",
+                    help="ensemble: skip timing the runs one after another")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device copy (torch) to calibrate rocprofv3 FETCH/WRITE counters")
     args = ap.parse_args()
@@ -182,7 +184,7 @@ def main():
     lbm.load_library()
     if args.workload == "ensemble":
         print(f"[bench] ensemble: warmup {args.warmup}, timed {args.steps}", file=sys.stderr, flush=True)
-        e = measure_ensemble(lbm, args.steps, args.warmup)
+        e = measure_ensemble(lbm, args.steps, args.warmup, sequential=not args.no_sequential)
         print(json.dumps({
             "metric": "MLUPS, batched ensemble of 64 slit-channel squirmer runs (seed 0), FP16 store",
             "value": round(e["mlups"], 1), "unit": "MLUPS", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -191,9 +193,9 @@ def main():
             "config": {"workload": "64 runs from seed 0: H in [32,96], grid 4Hx2HxH, walls in z, R in [4,8], "
                                    "beta in [-5,5], pitch in [0,45) deg, z0 in [2R,H-2R], B1=1e-3, FP16 store",
                        "total_nodes": e["total_nodes"], "parallelism": "single GPU"},
-            "sequential": {"mlups": round(e["sequential_mlups"], 1),
-                           "ms_per_step": round(e["sequential_ms_per_step"], 5)},
-            "batch_over_sequential": round(e["mlups"] / e["sequential_mlups"], 3),
+            "sequential": ({"mlups": round(e["sequential_mlups"], 1),
+                            "ms_per_step": round(e["sequential_ms_per_step"], 5)} if e["sequential_mlups"] else None),
+            "batch_over_sequential": (round(e["mlups"] / e["sequential_mlups"], 3) if e["sequential_mlups"] else None),
             "tuning": e["tuning"], "device_gb": round(e["device_gb"], 2)}))
         return
     res = {}

@@ -93,6 +93,9 @@ typedef struct lbm_hip_info {
     uint64_t device_bytes;  /* device memory held by the context */
     int has_squirmer, walls_y, walls_z;
     int64_t count_fluid;    /* FLUID nodes now (LBM::countFluid, lbm.cpp:342) */
+    int row_pitch;          /* nodes per stored population row (nx, or nx rounded up to 64) */
+    int tile_width;         /* collide-stream tile width in x (64 / 128 / 256) */
+    int xcd_remap;          /* 1: contiguous tile range per XCD */
 } lbm_hip_info;
 
 typedef struct lbm_hip_ctx lbm_hip_ctx;

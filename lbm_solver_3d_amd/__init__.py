@@ -82,6 +82,7 @@ class Info(C.Structure):
         ("bytes_per_lup", C.c_double), ("device_bytes", C.c_uint64),
         ("has_squirmer", C.c_int), ("walls_y", C.c_int), ("walls_z", C.c_int),
         ("count_fluid", C.c_int64),
+        ("row_pitch", C.c_int), ("tile_width", C.c_int), ("xcd_remap", C.c_int),
     ]
 
 

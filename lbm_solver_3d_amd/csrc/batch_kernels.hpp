@@ -35,43 +35,63 @@ struct BatchFinalize {
 
 constexpr uint32_t BATCH_LOCAL_MASK = 0xFFFFFFu;
 
+// The member tables are not written during a launch: the kernels read them
+// through the constant address space, so the loads are scalar and may be
+// repeated where a value is needed, as with kernel arguments, instead of
+// holding copies in vector registers (which cost the collide-stream kernel a
+// third of its occupancy).
+template <class T>
+__device__ __forceinline__ const __attribute__((address_space(4))) T* as_constant(const T* p) {
+    return (const __attribute__((address_space(4))) T*)p;
+}
+
+template <class T>
+__device__ __forceinline__ T from_constant(const T* p) {
+    T out;
+    __builtin_memcpy(&out, as_constant(p), sizeof(T));
+    return out;
+}
+
 template <class S, bool ODD, int MODE>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(LBM_STEP_WAVES_PER_EU)))
 void batch_step_kernel(const BatchMember<S>* __restrict__ M, const uint32_t* __restrict__ tile_map, int Gtot,
                        long long t, int xcd) {
     const int g = xcd ? xcd_tile(blockIdx.x, Gtot) : (int)blockIdx.x;
-    const uint32_t e = tile_map[g];
-    const BatchMember<S>& mb = M[e >> 24];
-    StepArgs<S> A = mb.A;
+    const uint32_t e = as_constant(tile_map)[g];
+    const BatchMember<S>* mb = M + (e >> 24);
+    StepArgs<S> A = from_constant(&mb->A);
+    const FastCoef K = from_constant(&mb->K);
     A.t = t;
-    step_tile<S, ODD, MODE>(A, mb.K, (int)(e & BATCH_LOCAL_MASK));
+    step_tile<S, ODD, MODE>(A, K, (int)(e & BATCH_LOCAL_MASK));
 }
 
 template <class S, int MODE>
 __global__ __launch_bounds__(BLOCK) void batch_boundary_kernel(const BatchMember<S>* __restrict__ M,
                                                                const uint32_t* __restrict__ block_map, long long t) {
-    const uint32_t e = block_map[blockIdx.x];
-    const BatchMember<S>& mb = M[e >> 24];
-    StepArgs<S> A = mb.A;
+    const uint32_t e = as_constant(block_map)[blockIdx.x];
+    const BatchMember<S>* mb = M + (e >> 24);
+    StepArgs<S> A = from_constant(&mb->A);
+    const FastCoef K = from_constant(&mb->K);
+    const BoundaryArgs B = from_constant(&mb->B);
     A.t = t;
-    boundary_block<S, MODE>(A, mb.K, mb.B, (int)(e & BATCH_LOCAL_MASK));
+    boundary_block<S, MODE>(A, K, B, (int)(e & BATCH_LOCAL_MASK));
 }
 
 template <class S>
 __global__ __launch_bounds__(LINK_BLOCK) void batch_links_kernel(const BatchMember<S>* __restrict__ M,
                                                                  const uint32_t* __restrict__ block_map) {
-    const uint32_t e = block_map[blockIdx.x];
-    const BatchMember<S>& mb = M[e >> 24];
-    links_block<S>(mb.L, (int)(e & BATCH_LOCAL_MASK), mb.lblocks);
+    const uint32_t e = as_constant(block_map)[blockIdx.x];
+    const BatchMember<S>* mb = M + (e >> 24);
+    links_block<S>(from_constant(&mb->L), (int)(e & BATCH_LOCAL_MASK), from_constant(&mb->lblocks));
 }
 
 // One block per member; `t_after` is the number of steps done.
 template <class S>
 __global__ __launch_bounds__(SQ_THREADS) void batch_squirmer_kernel(const BatchMember<S>* __restrict__ M,
                                                                     long long t_after) {
-    const BatchMember<S>& mb = M[blockIdx.x];
-    if (!mb.has_squirmer) return;
-    RemapArgs<S> R = mb.R;
+    const BatchMember<S>* mb = M + blockIdx.x;
+    if (!from_constant(&mb->has_squirmer)) return;
+    RemapArgs<S> R = from_constant(&mb->R);
     R.shifted = (t_after > 0 && ((t_after - 1) & 1) == 0) ? 1 : 0;
     R.shift_slot = (int)(t_after & 1);
     squirmer_block<S>(R);

@@ -64,7 +64,8 @@ struct StepArgs {
     double* mass_part;    // one excess-mass partial per block
     int NX, NY, NZ;
     uint32_t N;
-    uint32_t SS;          // slot stride: population i of node n is f[i * SS + n]
+    uint32_t SS;          // slot stride: population i of node (x, y, z) is f[i * SS + x + PX (y + NY z)]
+    uint32_t PX;          // population row pitch (NX, or NX rounded up to 64 nodes)
     int walls_y, walls_z, has_squirmer;
     int gx, gy, gz;       // grid in tiles
     int tx_shift;         // log2 of the tile width in x

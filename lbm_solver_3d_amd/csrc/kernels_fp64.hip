@@ -20,33 +20,33 @@ hipError_t launch_links(const LinkArgs<S>& a, int blocks, hipStream_t s) {
 
 template <class S>
 hipError_t launch_wall_force(const S* f, const uint8_t* flag, double* part, DevState* st, int NX, int NY, int NZ,
-                             uint32_t N, uint32_t SS, hipStream_t s) {
+                             uint32_t N, uint32_t SS, uint32_t PX, hipStream_t s) {
     hipLaunchKernelGGL((wall_force_kernel<S>), dim3((N + 255) / 256), dim3(256), 0, s, f, flag, part, st, NX, NY, NZ,
-                       N, SS);
+                       N, SS, PX);
     return hipGetLastError();
 }
 
 template <class S>
 hipError_t launch_gather(const S* f, const uint8_t* flag, double* out, int NX, int NY, int NZ, uint32_t N,
-                         uint32_t SS, int shifted, const DevState* st, int shift_slot, hipStream_t s) {
+                         uint32_t SS, uint32_t PX, int shifted, const DevState* st, int shift_slot, hipStream_t s) {
     hipLaunchKernelGGL((gather_kernel<S>), dim3((N + 255) / 256), dim3(256), 0, s, f, flag, out, NX, NY, NZ, N, SS,
-                       shifted, st, shift_slot);
+                       PX, shifted, st, shift_slot);
     return hipGetLastError();
 }
 
 template <class S>
 hipError_t launch_scatter(S* f, const uint8_t* flag, const double* in, int NX, int NY, int NZ, uint32_t N,
-                          uint32_t SS, int shifted, hipStream_t s) {
+                          uint32_t SS, uint32_t PX, int shifted, hipStream_t s) {
     hipLaunchKernelGGL((scatter_kernel<S>), dim3((N + 255) / 256), dim3(256), 0, s, f, flag, in, NX, NY, NZ, N, SS,
-                       shifted);
+                       PX, shifted);
     return hipGetLastError();
 }
 
 template <class S>
 hipError_t launch_mass(const S* f, const uint8_t* flag, double* part, int NX, int NY, int NZ, uint32_t N,
-                       uint32_t SS, int shifted, const DevState* st, int shift_slot, hipStream_t s) {
+                       uint32_t SS, uint32_t PX, int shifted, const DevState* st, int shift_slot, hipStream_t s) {
     hipLaunchKernelGGL((mass_kernel<S>), dim3((N + 255) / 256), dim3(256), 0, s, f, flag, part, NX, NY, NZ, N, SS,
-                       shifted, st, shift_slot);
+                       PX, shifted, st, shift_slot);
     return hipGetLastError();
 }
 
@@ -86,13 +86,13 @@ hipError_t launch_batch_squirmer(const BatchMember<S>* m, int count, long long t
     template hipError_t launch_squirmer<S>(const RemapArgs<S>&, hipStream_t);                                    \
     template hipError_t launch_links<S>(const LinkArgs<S>&, int, hipStream_t);                                   \
     template hipError_t launch_wall_force<S>(const S*, const uint8_t*, double*, DevState*, int, int, int,       \
-                                             uint32_t, uint32_t, hipStream_t);                                  \
+                                             uint32_t, uint32_t, uint32_t, hipStream_t);                                  \
     template hipError_t launch_gather<S>(const S*, const uint8_t*, double*, int, int, int, uint32_t, uint32_t,   \
-                                         int, const DevState*, int, hipStream_t);                                \
+                                         uint32_t, int, const DevState*, int, hipStream_t);                                \
     template hipError_t launch_scatter<S>(S*, const uint8_t*, const double*, int, int, int, uint32_t, uint32_t,  \
-                                          int, hipStream_t);                                                     \
+                                          uint32_t, int, hipStream_t);                                                     \
     template hipError_t launch_mass<S>(const S*, const uint8_t*, double*, int, int, int, uint32_t, uint32_t,     \
-                                       int, const DevState*, int, hipStream_t);                                  \
+                                       uint32_t, int, const DevState*, int, hipStream_t);                                  \
     template hipError_t launch_fill_populations<S>(S*, uint32_t, hipStream_t);
 
 LBMHIP_INSTANTIATE(double)

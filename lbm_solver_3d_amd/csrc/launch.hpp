@@ -29,19 +29,19 @@ hipError_t launch_links(const LinkArgs<S>& a, int blocks, hipStream_t s);
 
 template <class S>
 hipError_t launch_wall_force(const S* f, const uint8_t* flag, double* part, DevState* st, int NX, int NY, int NZ,
-                             uint32_t N, uint32_t SS, hipStream_t s);
+                             uint32_t N, uint32_t SS, uint32_t PX, hipStream_t s);
 
 template <class S>
 hipError_t launch_gather(const S* f, const uint8_t* flag, double* out, int NX, int NY, int NZ, uint32_t N,
-                         uint32_t SS, int shifted, const DevState* st, int shift_slot, hipStream_t s);
+                         uint32_t SS, uint32_t PX, int shifted, const DevState* st, int shift_slot, hipStream_t s);
 
 template <class S>
 hipError_t launch_scatter(S* f, const uint8_t* flag, const double* in, int NX, int NY, int NZ, uint32_t N,
-                          uint32_t SS, int shifted, hipStream_t s);
+                          uint32_t SS, uint32_t PX, int shifted, hipStream_t s);
 
 template <class S>
 hipError_t launch_mass(const S* f, const uint8_t* flag, double* part, int NX, int NY, int NZ, uint32_t N,
-                       uint32_t SS, int shifted, const DevState* st, int shift_slot, hipStream_t s);
+                       uint32_t SS, uint32_t PX, int shifted, const DevState* st, int shift_slot, hipStream_t s);
 
 template <class S>
 hipError_t launch_fill_populations(S* f, uint32_t SS, hipStream_t s);
@@ -74,13 +74,13 @@ hipError_t launch_macro_to_double(const void* src, int is_double, double* dst, u
     extern template hipError_t launch_squirmer<S>(const RemapArgs<S>&, hipStream_t);                            \
     extern template hipError_t launch_links<S>(const LinkArgs<S>&, int, hipStream_t);                           \
     extern template hipError_t launch_wall_force<S>(const S*, const uint8_t*, double*, DevState*, int, int, int, \
-                                                    uint32_t, uint32_t, hipStream_t);                           \
+                                                    uint32_t, uint32_t, uint32_t, hipStream_t);                           \
     extern template hipError_t launch_gather<S>(const S*, const uint8_t*, double*, int, int, int, uint32_t,      \
-                                                uint32_t, int, const DevState*, int, hipStream_t);              \
+                                                uint32_t, uint32_t, int, const DevState*, int, hipStream_t);              \
     extern template hipError_t launch_scatter<S>(S*, const uint8_t*, const double*, int, int, int, uint32_t,     \
-                                                 uint32_t, int, hipStream_t);                                   \
+                                                 uint32_t, uint32_t, int, hipStream_t);                                   \
     extern template hipError_t launch_mass<S>(const S*, const uint8_t*, double*, int, int, int, uint32_t,        \
-                                              uint32_t, int, const DevState*, int, hipStream_t);                \
+                                              uint32_t, uint32_t, int, const DevState*, int, hipStream_t);                \
     extern template hipError_t launch_fill_populations<S>(S*, uint32_t, hipStream_t);                         \
     extern template hipError_t launch_batch_step<S>(const BatchMember<S>*, const uint32_t*, int, long long, int, \
                                                     bool, int, hipStream_t);                                    \

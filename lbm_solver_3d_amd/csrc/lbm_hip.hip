@@ -50,9 +50,17 @@ struct lbm_hip_ctx {
     hipEvent_t kev[2] = {nullptr, nullptr};
     int device = 0;
     hipStream_t stream = nullptr;
+    // even steps, when `concurrent_boundary`: the boundary-chunk kernel runs
+    // on `side`, concurrently with the bulk kernel on `stream` (they touch
+    // disjoint nodes), forked and joined by events.  Off by default: on one
+    // lattice the bulk kernel fills the GPU and the overlap gains nothing.
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+    bool concurrent_boundary = false;
     int NX = 0, NY = 0, NZ = 0;
     uint32_t N = 0;
-    uint32_t SS = 0;  // slot stride in elements (N plus padding)
+    uint32_t SS = 0;  // slot stride in elements (PX * NY * NZ plus padding)
+    uint32_t PX = 0;  // population row pitch: NX, or NX rounded up to 64 nodes
     int prec = 0;
     size_t ssize = 8, msize = 8;
     void* f = nullptr;
@@ -202,7 +210,7 @@ StepArgs<S> step_args(const lbm_hip_ctx* c) {
     a.f = (S*)c->f; a.flag = c->flag;
     a.rho = c->rho; a.ux = c->ux; a.uy = c->uy; a.uz = c->uz;
     a.st = c->st; a.mass_part = c->mass_part;
-    a.NX = c->NX; a.NY = c->NY; a.NZ = c->NZ; a.N = c->N; a.SS = c->SS;
+    a.NX = c->NX; a.NY = c->NY; a.NZ = c->NZ; a.N = c->N; a.SS = c->SS; a.PX = c->PX;
     a.walls_y = c->p.walls_y; a.walls_z = c->p.walls_z; a.has_squirmer = c->has_squirmer;
     a.gx = c->gx; a.gy = c->gy; a.gz = c->gz; a.tx_shift = c->tx_shift; a.ty = c->ty;
     a.xcd_remap = c->xcd_remap;
@@ -218,7 +226,7 @@ RemapArgs<S> remap_args(const lbm_hip_ctx* c) {
     a.f = (S*)c->f; a.flag = c->flag;
     a.rho = c->rho; a.ux = c->ux; a.uy = c->uy; a.uz = c->uz;
     a.st = c->st;
-    a.NX = c->NX; a.NY = c->NY; a.NZ = c->NZ; a.N = c->N; a.SS = c->SS;
+    a.NX = c->NX; a.NY = c->NY; a.NZ = c->NZ; a.N = c->N; a.SS = c->SS; a.PX = c->PX;
     a.walls_y = c->p.walls_y; a.walls_z = c->p.walls_z;
     a.shifted = (c->t > 0 && ((c->t - 1) & 1) == 0) ? 1 : 0;
     a.shift_slot = (int)(c->t & 1);
@@ -229,7 +237,7 @@ template <class S>
 LinkArgs<S> link_args(const lbm_hip_ctx* c) {
     LinkArgs<S> a{};
     a.f = (S*)c->f; a.flag = c->flag; a.rho = c->rho; a.st = c->st; a.part = c->red_part;
-    a.NX = c->NX; a.NY = c->NY; a.NZ = c->NZ; a.SS = c->SS;
+    a.NX = c->NX; a.NY = c->NY; a.NZ = c->NZ; a.SS = c->SS; a.PX = c->PX;
     a.walls_y = c->p.walls_y; a.walls_z = c->p.walls_z;
     return a;
 }
@@ -239,6 +247,21 @@ int link_blocks_for(double R, int NX, int NY, int NZ) {
     const int w = (int)std::ceil(2.0 * R) + 5;
     const long long M = (long long)std::min(w, NX) * std::min(w, NY) * std::min(w, NZ);
     return (int)((M + LINK_BLOCK - 1) / LINK_BLOCK);
+}
+
+// Population row pitch.  A wavefront covers 64 consecutive x of one row;
+// when rows start on a multiple of 64 nodes each wavefront's access to a
+// population is one aligned segment (not two partial ones), which measured
+// 1.6x faster on rows of 500 or 456 nodes.  Lanes past NX load and store
+// nothing, so the pad costs memory, not traffic: rows are padded when that
+// adds at most half again (LBM_HIP_ROW_PAD=0 turns it off).
+uint32_t row_pitch(int nx, int ny, int nz) {
+    const uint32_t px = ((uint32_t)nx + 63u) / 64u * 64u;
+    if (const char* e = std::getenv("LBM_HIP_ROW_PAD"))
+        if (std::atoi(e) == 0) return (uint32_t)nx;
+    if (px == (uint32_t)nx || 2 * px > 3 * (uint32_t)nx) return (uint32_t)nx;
+    if (19.0 * ((double)px * ny * nz + 65536.0) >= 4294967296.0) return (uint32_t)nx;
+    return px;
 }
 
 // Slot stride: N rounded up to 256 bytes, plus a pad so that the 19
@@ -276,12 +299,16 @@ void free_ctx(lbm_hip_ctx* c) {
     if (!c) return;
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
+    if (c->side) hipStreamSynchronize(c->side);
     for (void* q : {c->f, (void*)c->flag, c->rho, c->ux, c->uy, c->uz, (void*)c->st, (void*)c->mass_part,
                     (void*)c->diag_part, (void*)c->diag_out, (void*)c->red_part, (void*)c->wall_rows})
         if (q) hipFree(q);
     for (hipEvent_t e : c->events) hipEventDestroy(e);
+    for (hipEvent_t e : {c->fork, c->join})
+        if (e) hipEventDestroy(e);
     if (c->graph) hipGraphExecDestroy(c->graph);
     if (c->stream) hipStreamDestroy(c->stream);
+    if (c->side) hipStreamDestroy(c->side);
     delete c;
 }
 
@@ -316,7 +343,8 @@ int lbm_hip_create(const lbm_hip_params* pin, lbm_hip_ctx** out) {
     c->NX = pin->nx; c->NY = pin->ny; c->NZ = pin->nz;
     c->N = (uint32_t)((size_t)pin->nx * pin->ny * pin->nz);
     c->ssize = storage_bytes(c->prec);
-    c->SS = slot_stride(c->N, c->ssize);
+    c->PX = row_pitch(pin->nx, pin->ny, pin->nz);
+    c->SS = slot_stride(c->PX * (uint32_t)pin->ny * (uint32_t)pin->nz, c->ssize);
     c->msize = c->prec == LBM_HIP_FP64 ? 8 : 4;
     // lbm.cpp:67-69
     c->omPlus = 1.0 / pin->tau;
@@ -362,7 +390,11 @@ int lbm_hip_create(const lbm_hip_params* pin, lbm_hip_ctx** out) {
         return code;
     };
     if (hipSetDevice(c->device) != hipSuccess) return cleanup(fail(LBM_HIP_EDEVICE, "hipSetDevice failed"));
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+    if (const char* v = std::getenv("LBM_HIP_CONCURRENT_BOUNDARY")) c->concurrent_boundary = std::atoi(v) != 0;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->join, hipEventDisableTiming) != hipSuccess)
         return cleanup(fail(LBM_HIP_EDEVICE, "hipStreamCreate failed"));
 
     const size_t N = c->N;
@@ -482,15 +514,24 @@ int lbm_hip_step(lbm_hip_ctx* c, int flags) {
     dispatch(c->prec, [&](auto* tag) -> int {
         using S = std::remove_pointer_t<decltype(tag)>;
         const StepArgs<S> sa = step_args<S>(c);
-        if (c->kev[0]) e = hipEventRecord(c->kev[0], c->stream);
+        const bool fork = !odd && c->bblocks > 0 && c->concurrent_boundary;
+        if (fork) {
+            e = hipEventRecord(c->fork, c->stream);
+            if (e == hipSuccess) e = hipStreamWaitEvent(c->side, c->fork, 0);
+            if (e == hipSuccess) e = launch_boundary<S>(sa, c->K, c->bargs, c->bblocks, mode, c->side);
+            if (e == hipSuccess) e = hipEventRecord(c->join, c->side);
+        }
+        if (e == hipSuccess && c->kev[0]) e = hipEventRecord(c->kev[0], c->stream);
         if (e == hipSuccess) e = launch_step<S>(sa, c->K, odd, mode, c->stream);
         if (e == hipSuccess && c->kev[1]) e = hipEventRecord(c->kev[1], c->stream);
-        if (e == hipSuccess && !odd) e = launch_boundary<S>(sa, c->K, c->bargs, c->bblocks, mode, c->stream);
+        if (e == hipSuccess && fork) e = hipStreamWaitEvent(c->stream, c->join, 0);
+        if (e == hipSuccess && !odd && !fork)
+            e = launch_boundary<S>(sa, c->K, c->bargs, c->bblocks, mode, c->stream);
         // moving-boundary bounce-back and force/torque on the squirmer
         if (e == hipSuccess && c->has_squirmer) e = launch_links<S>(link_args<S>(c), c->link_blocks, c->stream);
         if (e == hipSuccess && (flags & LBM_STEP_WALL_FORCE))
             e = launch_wall_force<S>((const S*)c->f, c->flag, c->red_part, c->st, c->NX, c->NY, c->NZ, c->N, c->SS,
-                                     c->stream);
+                                     c->PX, c->stream);
         return 0;
     });
     if (e != hipSuccess) return fail(LBM_HIP_EDEVICE, std::string("step launch: ") + hipGetErrorString(e));
@@ -526,7 +567,7 @@ int lbm_hip_correct_mass(lbm_hip_ctx* c) {
         e = hipSuccess;
         dispatch(c->prec, [&](auto* tag) -> int {
             using S = std::remove_pointer_t<decltype(tag)>;
-            e = launch_mass<S>((const S*)c->f, c->flag, c->mass_part, c->NX, c->NY, c->NZ, c->N, c->SS, shifted,
+            e = launch_mass<S>((const S*)c->f, c->flag, c->mass_part, c->NX, c->NY, c->NZ, c->N, c->SS, c->PX, shifted,
                                c->st, shift_slot, c->stream);
             return 0;
         });
@@ -697,7 +738,7 @@ int lbm_hip_download(lbm_hip_ctx* c, double* f, double* rho, double* ux, double*
         const int shifted = layout_shifted(c) ? 1 : 0;
         dispatch(c->prec, [&](auto* tag) -> int {
             using S = std::remove_pointer_t<decltype(tag)>;
-            e = launch_gather<S>((const S*)c->f, c->flag, tmp, c->NX, c->NY, c->NZ, c->N, c->SS, shifted, c->st,
+            e = launch_gather<S>((const S*)c->f, c->flag, tmp, c->NX, c->NY, c->NZ, c->N, c->SS, c->PX, shifted, c->st,
                                  (int)(c->t & 1), c->stream);
             return 0;
         });
@@ -743,7 +784,7 @@ int lbm_hip_upload_populations(lbm_hip_ctx* c, const double* f) {
     if (e == hipSuccess)
         dispatch(c->prec, [&](auto* tag) -> int {
             using S = std::remove_pointer_t<decltype(tag)>;
-            e = launch_scatter<S>((S*)c->f, c->flag, tmp, c->NX, c->NY, c->NZ, c->N, c->SS, shifted, c->stream);
+            e = launch_scatter<S>((S*)c->f, c->flag, tmp, c->NX, c->NY, c->NZ, c->N, c->SS, c->PX, shifted, c->stream);
             return 0;
         });
     const double zero = 0.0;
@@ -768,6 +809,9 @@ int lbm_hip_get_info(lbm_hip_ctx* c, lbm_hip_info* o) {
     HIP_TRY(hipMemcpyAsync(&nf, &c->st->nfluid, sizeof nf, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     o->count_fluid = nf;
+    o->row_pitch = (int)c->PX;
+    o->tile_width = 1 << c->tx_shift;
+    o->xcd_remap = c->xcd_remap;
     return LBM_HIP_OK;
 }
 
@@ -969,6 +1013,9 @@ struct lbm_hip_batch {
     int graph_cycle = 0, graph_mass_every = -1;
     size_t device_bytes = 0;
     hipEvent_t ev[2] = {nullptr, nullptr};
+    hipStream_t side = nullptr;  // even-step boundary chunks, concurrent with the bulk launch
+    hipEvent_t fork = nullptr, join = nullptr;
+    bool concurrent_boundary = false;
 };
 
 namespace {
@@ -979,9 +1026,11 @@ void free_batch(lbm_hip_batch* b) {
     if (b->stream) hipStreamSynchronize(b->stream);
     for (void* q : {b->desc, (void*)b->fin, (void*)b->tile_map, (void*)b->bmap, (void*)b->lmap})
         if (q) hipFree(q);
+    if (b->side) hipStreamSynchronize(b->side);
     if (b->graph) hipGraphExecDestroy(b->graph);
-    for (hipEvent_t e : b->ev)
+    for (hipEvent_t e : {b->ev[0], b->ev[1], b->fork, b->join})
         if (e) hipEventDestroy(e);
+    if (b->side) hipStreamDestroy(b->side);
     for (lbm_hip_ctx* c : b->mem) {
         if (!c) continue;
         c->owned_by_batch = false;
@@ -1052,14 +1101,23 @@ int batch_step(lbm_hip_batch* b, int mass_every, int extra_flags) {
     dispatch(b->prec, [&](auto* tag) -> int {
         using S = std::remove_pointer_t<decltype(tag)>;
         const auto* d = static_cast<const BatchMember<S>*>(b->desc);
-        e = launch_batch_step<S>(d, b->tile_map, b->Gtot, b->t, b->mem[0]->xcd_remap, odd, mode, b->stream);
-        if (e == hipSuccess && !odd) e = launch_batch_boundary<S>(d, b->bmap, b->Btot, b->t, mode, b->stream);
+        const bool fork = !odd && b->Btot > 0 && b->concurrent_boundary;
+        if (fork) {
+            e = hipEventRecord(b->fork, b->stream);
+            if (e == hipSuccess) e = hipStreamWaitEvent(b->side, b->fork, 0);
+            if (e == hipSuccess) e = launch_batch_boundary<S>(d, b->bmap, b->Btot, b->t, mode, b->side);
+            if (e == hipSuccess) e = hipEventRecord(b->join, b->side);
+        }
+        if (e == hipSuccess)
+            e = launch_batch_step<S>(d, b->tile_map, b->Gtot, b->t, b->mem[0]->xcd_remap, odd, mode, b->stream);
+        if (e == hipSuccess && fork) e = hipStreamWaitEvent(b->stream, b->join, 0);
+        if (e == hipSuccess && !odd && !fork) e = launch_batch_boundary<S>(d, b->bmap, b->Btot, b->t, mode, b->stream);
         if (e == hipSuccess && b->any_squirmer) e = launch_batch_links<S>(d, b->lmap, b->Ltot, b->stream);
         if (e == hipSuccess && (extra_flags & LBM_STEP_WALL_FORCE))
             for (lbm_hip_ctx* c : b->mem)
                 if (e == hipSuccess)
                     e = launch_wall_force<S>((const S*)c->f, c->flag, c->red_part, c->st, c->NX, c->NY, c->NZ, c->N,
-                                             c->SS, b->stream);
+                                             c->SS, c->PX, b->stream);
         if (e == hipSuccess && b->any_squirmer) e = launch_batch_squirmer<S>(d, b->count, tnext, b->stream);
         return 0;
     });
@@ -1192,7 +1250,11 @@ int lbm_hip_batch_create(const lbm_hip_params* members, int count, lbm_hip_batch
             return bail(fail(LBM_HIP_EINVAL, "member " + std::to_string(m) + " too large for a batch"));
     }
     if (hipSetDevice(b->device) != hipSuccess) return bail(fail(LBM_HIP_EDEVICE, "hipSetDevice failed"));
-    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
+    if (const char* v = std::getenv("LBM_HIP_CONCURRENT_BOUNDARY")) b->concurrent_boundary = std::atoi(v) != 0;
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->join, hipEventDisableTiming) != hipSuccess)
         return bail(fail(LBM_HIP_EDEVICE, "hipStreamCreate failed"));
     for (hipEvent_t& e : b->ev)
         if (hipEventCreate(&e) != hipSuccess) return bail(fail(LBM_HIP_EDEVICE, "hipEventCreate failed"));

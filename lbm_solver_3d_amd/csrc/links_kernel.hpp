@@ -31,7 +31,7 @@ struct LinkArgs {
     DevState* st;
     double* part;        // LINK_MAX_BLOCKS x 8 partial sums
     int NX, NY, NZ;
-    uint32_t SS;
+    uint32_t SS, PX;  // slot stride, population row pitch
     int walls_y, walls_z;
 };
 
@@ -163,7 +163,7 @@ __device__ __forceinline__ void links_block(const LinkArgs<S>& A, int block, int
         const int xn = x + CX[i] < 0 ? NX - 1 : (x + CX[i] >= NX ? 0 : x + CX[i]);
         const int yn = y + CY[i] < 0 ? NY - 1 : (y + CY[i] >= NY ? 0 : y + CY[i]);
         const int zn = z + CZ[i] < 0 ? NZ - 1 : (z + CZ[i] >= NZ ? 0 : z + CZ[i]);
-        S* slot = A.f + (size_t)i * A.SS + node_of(xn, yn, zn);
+        S* slot = A.f + (size_t)i * A.SS + pop_index(xn, yn, zn, A.PX, NY);
         const double fpost = to_real<S>(*slot, i);
         double rb[3], ub[3];
         link_velocity(sq, NX, NY, NZ, A.walls_y, A.walls_z, x + 0.5 * CX[i], y + 0.5 * CY[i], z + 0.5 * CZ[i],
@@ -200,7 +200,7 @@ __global__ __launch_bounds__(LINK_BLOCK) void particle_links_kernel(const LinkAr
 template <class S>
 __global__ __launch_bounds__(256) void wall_force_kernel(const S* __restrict__ f, const uint8_t* __restrict__ flag,
                                                          double* part, DevState* st, int NX, int NY, int NZ,
-                                                         uint32_t N, uint32_t SS) {
+                                                         uint32_t N, uint32_t SS, uint32_t PX) {
     __shared__ double lds[4 * 3];
     const uint32_t n = blockIdx.x * 256u + threadIdx.x;
     double acc[3] = {0.0, 0.0, 0.0};
@@ -215,7 +215,7 @@ __global__ __launch_bounds__(256) void wall_force_kernel(const S* __restrict__ f
                 const int zn = z + CZ[i] < 0 ? NZ - 1 : (z + CZ[i] >= NZ ? 0 : z + CZ[i]);
                 const uint32_t m = (uint32_t)xn + (uint32_t)NX * ((uint32_t)yn + (uint32_t)NY * (uint32_t)zn);
                 if ((flag[m] & TYPE_MASK) != WALL) continue;
-                const double fpost = to_real<S>(f[(size_t)i * SS + m], i);
+                const double fpost = to_real<S>(f[(size_t)i * SS + pop_index(xn, yn, zn, PX, NY)], i);
                 const double s = fpost + fpost;
                 acc[0] += s * CX[i]; acc[1] += s * CY[i]; acc[2] += s * CZ[i];
             }

@@ -53,6 +53,7 @@ struct RemapArgs {
     int NX, NY, NZ;
     uint32_t N;
     uint32_t SS;     // slot stride (elements between populations i and i+1)
+    uint32_t PX;     // population row pitch
     int walls_y, walls_z;
     int shifted;     // layout after the last step: 1 after an even step
     int shift_slot;  // mass-shift slot the next step reads
@@ -341,11 +342,10 @@ __device__ void squirmer_block(const RemapArgs<S>& A) {
             const int x = boxc[0][ix], y = boxc[1][iy], z = boxc[2][iz];
             if (cover) {
                 // population f_i(n) after the last step, lbm.cpp:295-298
-                size_t slot = (size_t)i * SS + n;
+                size_t slot = (size_t)i * SS + pop_index(x, y, z, A.PX, NY);
                 if (i > 0) {
-                    const uint32_t m = (uint32_t)step1(x, -CX[i], NX) +
-                                       (uint32_t)NX * ((uint32_t)step1(y, -CY[i], NY) +
-                                                       (uint32_t)NY * (uint32_t)step1(z, -CZ[i], NZ));
+                    const int xm = step1(x, -CX[i], NX), ym = step1(y, -CY[i], NY), zm = step1(z, -CZ[i], NZ);
+                    const uint32_t m = (uint32_t)xm + (uint32_t)NX * ((uint32_t)ym + (uint32_t)NY * (uint32_t)zm);
                     bool up_fluid = false;
                     if (!A.shifted) {
                         const uint8_t fm = A.flag[m];
@@ -353,7 +353,7 @@ __device__ void squirmer_block(const RemapArgs<S>& A) {
                         if (fm & FLIPPED) tm = (tm == FLUID) ? PARTICLE : FLUID;
                         up_fluid = tm == FLUID;
                     }
-                    if (!up_fluid) slot = (size_t)OPP[i] * SS + m;
+                    if (!up_fluid) slot = (size_t)OPP[i] * SS + pop_index(xm, ym, zm, A.PX, NY);
                 }
                 double fv = to_real<S>(A.f[slot], i);
                 if (dsh != 0.0) fv += W[i] * dsh;
@@ -484,15 +484,16 @@ __device__ void squirmer_block(const RemapArgs<S>& A) {
         const uint32_t e = list[k];
         const uint32_t n = e & 0x7FFFFFFFu;
         const bool cover = (e >> 31) != 0;
-        if (i == 0) {
-            if (!cover) f[n] = from_real<S>(equilibrium(0, lrr[k], lub[k][0], lub[k][1], lub[k][2]), 0);
-            continue;
-        }
         const int x = (int)(n % (uint32_t)NX), y = (int)((n / (uint32_t)NX) % (uint32_t)NY),
                   z = (int)(n / ((uint32_t)NX * (uint32_t)NY));
-        const uint32_t m = (uint32_t)step1(x, -CX[i], NX) +
-                           (uint32_t)NX * ((uint32_t)step1(y, -CY[i], NY) + (uint32_t)NY * (uint32_t)step1(z, -CZ[i], NZ));
-        const size_t own = (size_t)i * SS + n, far = (size_t)OPP[i] * SS + m;
+        const uint32_t pn = pop_index(x, y, z, A.PX, NY);
+        if (i == 0) {
+            if (!cover) f[pn] = from_real<S>(equilibrium(0, lrr[k], lub[k][0], lub[k][1], lub[k][2]), 0);
+            continue;
+        }
+        const int xm = step1(x, -CX[i], NX), ym = step1(y, -CY[i], NY), zm = step1(z, -CZ[i], NZ);
+        const uint32_t m = (uint32_t)xm + (uint32_t)NX * ((uint32_t)ym + (uint32_t)NY * (uint32_t)zm);
+        const size_t own = (size_t)i * SS + pn, far = (size_t)OPP[i] * SS + pop_index(xm, ym, zm, A.PX, NY);
         if (A.shifted) {  // loc(n, i) = slot opp(i) at n - c_i: nothing moves
             if (!cover) f[far] = from_real<S>(equilibrium(i, lrr[k], lub[k][0], lub[k][1], lub[k][2]), i);
             continue;
@@ -565,7 +566,7 @@ __global__ __launch_bounds__(SQ_THREADS) void squirmer_kernel(const RemapArgs<S>
 template <class S>
 __global__ __launch_bounds__(256) void gather_kernel(const S* __restrict__ f, const uint8_t* __restrict__ flag,
                                                      double* __restrict__ out, int NX, int NY, int NZ, uint32_t N,
-                                                     uint32_t SS, int shifted, const DevState* st, int shift_slot) {
+                                                     uint32_t SS, uint32_t PX, int shifted, const DevState* st, int shift_slot) {
     const uint32_t n = blockIdx.x * 256u + threadIdx.x;
     if (n >= N) return;
     const int x = (int)(n % (uint32_t)NX), y = (int)((n / (uint32_t)NX) % (uint32_t)NY),
@@ -573,11 +574,11 @@ __global__ __launch_bounds__(256) void gather_kernel(const S* __restrict__ f, co
     const bool fluid = (flag[n] & TYPE_MASK) == FLUID;
     const double dsh = st->shift[shift_slot];
     for (int i = 0; i < Q; ++i) {
-        size_t slot = (size_t)i * SS + n;
+        size_t slot = (size_t)i * SS + pop_index(x, y, z, PX, NY);
         if (fluid && i > 0) {
-            const uint32_t m = (uint32_t)wrapi(x - CX[i], NX) +
-                               (uint32_t)NX * ((uint32_t)wrapi(y - CY[i], NY) + (uint32_t)NY * (uint32_t)wrapi(z - CZ[i], NZ));
-            if (shifted || (flag[m] & TYPE_MASK) != FLUID) slot = (size_t)OPP[i] * SS + m;
+            const int xm = wrapi(x - CX[i], NX), ym = wrapi(y - CY[i], NY), zm = wrapi(z - CZ[i], NZ);
+            const uint32_t m = (uint32_t)xm + (uint32_t)NX * ((uint32_t)ym + (uint32_t)NY * (uint32_t)zm);
+            if (shifted || (flag[m] & TYPE_MASK) != FLUID) slot = (size_t)OPP[i] * SS + pop_index(xm, ym, zm, PX, NY);
         }
         double v = to_real<S>(f[slot], i);
         if (fluid && dsh != 0.0) v += W[i] * dsh;
@@ -588,7 +589,7 @@ __global__ __launch_bounds__(256) void gather_kernel(const S* __restrict__ f, co
 template <class S>
 __global__ __launch_bounds__(256) void scatter_kernel(S* __restrict__ f, const uint8_t* __restrict__ flag,
                                                       const double* __restrict__ in, int NX, int NY, int NZ, uint32_t N,
-                                                      uint32_t SS, int shifted) {
+                                                      uint32_t SS, uint32_t PX, int shifted) {
     const uint32_t n = blockIdx.x * 256u + threadIdx.x;
     if (n >= N) return;
     const int x = (int)(n % (uint32_t)NX), y = (int)((n / (uint32_t)NX) % (uint32_t)NY),
@@ -598,11 +599,11 @@ __global__ __launch_bounds__(256) void scatter_kernel(S* __restrict__ f, const u
     const bool fluid = (flag[n] & TYPE_MASK) == FLUID;
     if (!fluid) return;
     for (int i = 0; i < Q; ++i) {
-        size_t slot = (size_t)i * SS + n;
+        size_t slot = (size_t)i * SS + pop_index(x, y, z, PX, NY);
         if (fluid && i > 0) {
-            const uint32_t m = (uint32_t)wrapi(x - CX[i], NX) +
-                               (uint32_t)NX * ((uint32_t)wrapi(y - CY[i], NY) + (uint32_t)NY * (uint32_t)wrapi(z - CZ[i], NZ));
-            if (shifted || (flag[m] & TYPE_MASK) != FLUID) slot = (size_t)OPP[i] * SS + m;
+            const int xm = wrapi(x - CX[i], NX), ym = wrapi(y - CY[i], NY), zm = wrapi(z - CZ[i], NZ);
+            const uint32_t m = (uint32_t)xm + (uint32_t)NX * ((uint32_t)ym + (uint32_t)NY * (uint32_t)zm);
+            if (shifted || (flag[m] & TYPE_MASK) != FLUID) slot = (size_t)OPP[i] * SS + pop_index(xm, ym, zm, PX, NY);
         }
         f[slot] = from_real<S>(in[(size_t)i * N + n], i);
     }
@@ -613,7 +614,7 @@ __global__ __launch_bounds__(256) void scatter_kernel(S* __restrict__ f, const u
 template <class S>
 __global__ __launch_bounds__(256) void mass_kernel(const S* __restrict__ f, const uint8_t* __restrict__ flag,
                                                    double* __restrict__ part, int NX, int NY, int NZ, uint32_t N,
-                                                   uint32_t SS, int shifted, const DevState* st, int shift_slot) {
+                                                   uint32_t SS, uint32_t PX, int shifted, const DevState* st, int shift_slot) {
     __shared__ double lds[4];
     const uint32_t n = blockIdx.x * 256u + threadIdx.x;
     double ex[1] = {0.0};
@@ -623,11 +624,11 @@ __global__ __launch_bounds__(256) void mass_kernel(const S* __restrict__ f, cons
         const double dsh = st->shift[shift_slot];
         double s = 0.0;
         for (int i = 0; i < Q; ++i) {
-            size_t slot = (size_t)i * SS + n;
+            size_t slot = (size_t)i * SS + pop_index(x, y, z, PX, NY);
             if (i > 0) {
-                const uint32_t m = (uint32_t)wrapi(x - CX[i], NX) +
-                                   (uint32_t)NX * ((uint32_t)wrapi(y - CY[i], NY) + (uint32_t)NY * (uint32_t)wrapi(z - CZ[i], NZ));
-                if (shifted || (flag[m] & TYPE_MASK) != FLUID) slot = (size_t)OPP[i] * SS + m;
+                const int xm = wrapi(x - CX[i], NX), ym = wrapi(y - CY[i], NY), zm = wrapi(z - CZ[i], NZ);
+                const uint32_t m = (uint32_t)xm + (uint32_t)NX * ((uint32_t)ym + (uint32_t)NY * (uint32_t)zm);
+                if (shifted || (flag[m] & TYPE_MASK) != FLUID) slot = (size_t)OPP[i] * SS + pop_index(xm, ym, zm, PX, NY);
             }
             double v = to_real<S>(f[slot], i);
             if (dsh != 0.0) v += W[i] * dsh;

@@ -298,21 +298,32 @@ __device__ __forceinline__ void block_reduce(double (&v)[K], double* lds) {
 #define LBM_STEP_WAVES_PER_EU 4
 #endif
 
+// Neighbour offsets of node (x, y, z): node-array rows (yo, zo, pitch NX)
+// and population rows (pyo, pzo, pitch PX); x offsets in nodes and in bytes.
 struct RowGeom {
-    uint32_t xo[3], xb[3], yo[3], zo[3], n;
+    uint32_t xo[3], xb[3], yo[3], zo[3], pyo[3], pzo[3], n;
 };
 
 template <class S>
 __device__ __forceinline__ RowGeom row_geom(uint32_t x, uint32_t y, uint32_t z, uint32_t NX, uint32_t NY,
-                                            uint32_t NZ) {
+                                            uint32_t NZ, uint32_t PX) {
     RowGeom g;
-    const uint32_t NXY = NX * NY;
-    g.yo[0] = (y == 0 ? NY - 1 : y - 1) * NX; g.yo[1] = y * NX; g.yo[2] = (y + 1 == NY ? 0u : y + 1) * NX;
-    g.zo[0] = (z == 0 ? NZ - 1 : z - 1) * NXY; g.zo[1] = z * NXY; g.zo[2] = (z + 1 == NZ ? 0u : z + 1) * NXY;
+    const uint32_t ym = y == 0 ? NY - 1 : y - 1, yp = y + 1 == NY ? 0u : y + 1;
+    const uint32_t zm = z == 0 ? NZ - 1 : z - 1, zp = z + 1 == NZ ? 0u : z + 1;
+    const uint32_t NXY = NX * NY, PXY = PX * NY;
+    g.yo[0] = ym * NX; g.yo[1] = y * NX; g.yo[2] = yp * NX;
+    g.zo[0] = zm * NXY; g.zo[1] = z * NXY; g.zo[2] = zp * NXY;
+    g.pyo[0] = ym * PX; g.pyo[1] = y * PX; g.pyo[2] = yp * PX;
+    g.pzo[0] = zm * PXY; g.pzo[1] = z * PXY; g.pzo[2] = zp * PXY;
     g.xo[0] = x == 0 ? NX - 1 : x - 1; g.xo[1] = x; g.xo[2] = x + 1 == NX ? 0u : x + 1;
     for (int k = 0; k < 3; ++k) g.xb[k] = g.xo[k] * (uint32_t)sizeof(S);
     g.n = x + g.yo[1] + g.zo[1];
     return g;
+}
+
+// Population index of node (x, y, z) within a slot.
+__device__ __forceinline__ uint32_t pop_index(int x, int y, int z, uint32_t PX, int NY) {
+    return (uint32_t)x + PX * ((uint32_t)y + (uint32_t)NY * (uint32_t)z);
 }
 
 // Loads and stores that bypass the caches' retention.  The even step's bulk
@@ -351,8 +362,8 @@ __device__ __forceinline__ double node_update(const StepArgs<S>& A, const FastCo
     auto base = [&](int s, uint32_t row) -> char* { return (char*)(f + ((size_t)s * SS + row)); };
     auto at = [](char* b, uint32_t off) -> S* { return (S*)(b + off); };
     const uint32_t* xb = rg.xb;
-    const uint32_t* yo = rg.yo;
-    const uint32_t* zo = rg.zo;
+    const uint32_t* yo = rg.pyo;  // population rows
+    const uint32_t* zo = rg.pzo;
 
     // ---- gather the 19 populations node n collides ----
     T g[Q];
@@ -369,7 +380,7 @@ __device__ __forceinline__ double node_update(const StepArgs<S>& A, const FastCo
         if (fl & BOUNDARY) {
             static_for<1, Q>([&](auto I) {
                 constexpr int i = decltype(I)::value;
-                const uint32_t m = rg.xo[1 + CX[i]] + yo[1 + CY[i]] + zo[1 + CZ[i]];
+                const uint32_t m = rg.xo[1 + CX[i]] + rg.yo[1 + CY[i]] + rg.zo[1 + CZ[i]];
                 solid |= (uint32_t)((A.flag[m] & TYPE_MASK) != FLUID) << i;
             });
             static_for<1, Q>([&](auto I) {
@@ -475,7 +486,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<S>& A, const FastCoef& 
     const uint32_t NX = (uint32_t)A.NX, NY = (uint32_t)A.NY, NZ = (uint32_t)A.NZ;
     const bool row_ok = (uint32_t)y < NY;  // wavefront-uniform
     const bool inb = row_ok && (uint32_t)x < NX;
-    const RowGeom rg = row_geom<S>(inb ? (uint32_t)x : 0u, row_ok ? (uint32_t)y : 0u, (uint32_t)z, NX, NY, NZ);
+    const RowGeom rg = row_geom<S>(inb ? (uint32_t)x : 0u, row_ok ? (uint32_t)y : 0u, (uint32_t)z, NX, NY, NZ, A.PX);
     const uint8_t fl = inb ? A.flag[rg.n] : (uint8_t)WALL;
     bool active = (fl & TYPE_MASK) == FLUID;
     if constexpr (!ODD) {
@@ -552,7 +563,7 @@ __device__ __forceinline__ void boundary_block(const StepArgs<S>& A, const FastC
     const int x = chunk * 64 + lane;
     const bool inb = y >= 0 && (uint32_t)x < NX;
     const RowGeom rg = row_geom<S>(inb ? (uint32_t)x : 0u, y >= 0 ? (uint32_t)y : 0u, y >= 0 ? (uint32_t)z : 0u,
-                                   NX, NY, NZ);
+                                   NX, NY, NZ, A.PX);
     const uint8_t fl = inb ? A.flag[rg.n] : (uint8_t)WALL;
     bool active = (fl & TYPE_MASK) == FLUID;
     if (__builtin_amdgcn_ballot_w64(active && (fl & BOUNDARY)) == 0) active = false;  // not a boundary chunk

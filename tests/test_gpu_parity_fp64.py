@@ -56,6 +56,19 @@ def test_duct_fp64_non_cubic_tiles(oracle_mod, lbm):
         _compare_exact(o, L)
 
 
+def test_duct_fp64_padded_rows(oracle_mod, lbm):
+    """Rows of 50 nodes are stored with a pitch of 64 (aligned wavefront
+    segments): still bit-identical to the oracle."""
+    kv = dict(nx=50, ny=9, nz=7, tau=0.7, fx=2e-5, fy=1e-5, walls_y=1, walls_z=1)
+    o = oracle_mod.Oracle(**kv)
+    with lbm.Lattice(**kv, precision="fp64") as L:
+        assert L.info().row_pitch == 64
+        for _ in range(33):
+            o.step()
+            L.step(lbm.STEP_ALL)
+        _compare_exact(o, L)
+
+
 def _run_pair(o, L, lbm, steps, mass_every=10):
     for t in range(1, steps + 1):
         o.step()
@@ -101,6 +114,34 @@ def test_channel_squirmer_near_wall_fp64(oracle_mod, lbm):
     with lbm.Lattice(**kv, precision="fp64") as L:
         _run_pair(o, L, lbm, 333)
         _compare_squirmer(o, L)
+
+
+def test_channel_squirmer_padded_rows_fp64(oracle_mod, lbm):
+    """Squirmer in a channel whose 100-node rows are stored with a pitch of
+    128: links, remap and mass correction address the padded rows."""
+    kv = dict(nx=100, ny=20, nz=18, walls_y=0, walls_z=1, squirmer=1, radius=4.5, b1=0.03, beta=-2,
+              angle=-30, planar=1, plane_xz=1, z0=7.5, hmin=0.5)
+    o = oracle_mod.Oracle(**kv)
+    with lbm.Lattice(**kv, precision="fp64") as L:
+        assert L.info().row_pitch == 128
+        _run_pair(o, L, lbm, 160)
+        _compare_squirmer(o, L)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp16"])
+def test_row_padding_changes_nothing(lbm, monkeypatch, prec):
+    kv = dict(nx=100, ny=20, nz=18, walls_y=0, walls_z=1, squirmer=1, radius=4.5, b1=0.03, beta=-2, angle=-30,
+              planar=1, plane_xz=1, z0=7.5, hmin=0.5, precision=prec)
+    out = []
+    for pad in ("1", "0"):
+        monkeypatch.setenv("LBM_HIP_ROW_PAD", pad)
+        with lbm.Lattice(**kv) as L:
+            assert L.info().row_pitch == (128 if pad == "1" else 100)
+            L.run(57, 10, lbm.STEP_ALL)
+            out.append((L.populations(), L.flags(), L.squirmer()["X"]))
+    assert np.array_equal(out[0][1], out[1][1])
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][2], out[1][2])
 
 
 def test_plane_xz_squirmer_fp64(oracle_mod, lbm):
