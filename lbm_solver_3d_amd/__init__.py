@@ -102,10 +102,12 @@ _lib = None
 
 
 def exported_symbols() -> list[str]:
-    """Entry points declared in include/lbm_hip.h."""
+    """Entry points declared in the C headers under include/."""
     import re
-    txt = HEADER.read_text()
-    return sorted(set(re.findall(r"\b(lbm_hip_\w+)\s*\(", txt)))
+    names = set()
+    for h in sorted(HEADER.parent.glob("*.h")):
+        names |= set(re.findall(r"\b(lbm_(?:hip|mlp)_\w+)\s*\(", h.read_text()))
+    return sorted(names)
 
 
 TUNE_CANDIDATES = 6
@@ -171,6 +173,17 @@ def load_library(path: Path | None = None):
     lib.lbm_hip_batch_info_get.argtypes = [bt, C.POINTER(BatchInfo)]
     lib.lbm_hip_batch_autotune.argtypes = [bt, C.c_int, C.c_int, C.POINTER(Tuning)]
     lib.lbm_hip_batch_time_run.argtypes = [bt, C.c_int, C.c_int, dp]
+    fp = C.POINTER(C.c_float)
+    mh = C.c_void_p
+    lib.lbm_mlp_create.argtypes = [C.c_int, C.c_int, C.c_ulonglong, C.c_int, C.POINTER(C.c_void_p)]
+    lib.lbm_mlp_destroy.argtypes = [mh]
+    lib.lbm_mlp_destroy.restype = None
+    lib.lbm_mlp_get_params.argtypes = [mh, C.c_int, fp]
+    lib.lbm_mlp_set_params.argtypes = [mh, C.c_int, fp]
+    lib.lbm_mlp_forward.argtypes = [mh, fp, C.c_int, fp]
+    lib.lbm_mlp_predict.argtypes = [mh, fp, C.c_int, fp, fp]
+    lib.lbm_mlp_loss_grad.argtypes = [mh, C.c_int, fp, fp, C.c_int, fp, fp]
+    lib.lbm_mlp_train.argtypes = [mh, fp, fp, C.c_int, C.c_int, C.c_int, C.c_float, fp]
     lib.lbm_hip_last_error.restype = C.c_char_p
     lib.lbm_hip_last_error.argtypes = []
     if path is None:
@@ -417,3 +430,84 @@ def ensemble_members(count: int = 64, seed: int = 0) -> list[dict]:
         out.append(dict(nx=4 * H, ny=2 * H, nz=H, tau=1.0, walls_y=0, walls_z=1, squirmer=1, radius=float(R),
                         b1=1e-3, beta=beta, angle=pitch, z0=z0, planar=1, plane_xz=1, eps=0.2, hc=2.0, hmin=1.0))
     return out
+
+
+# ---------------------------------------------------------------------------
+# Learned velocity surrogate (include/lbm_mlp.h)
+# ---------------------------------------------------------------------------
+MLP_IN, MLP_OUT, MLP_PARAMS = 5, 6, 34566
+MLP_MODES = {"fp32": 0, "bf16": 1}
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float)) if a is not None else None
+
+
+class MLP:
+    """An ensemble of 5 -> 128 -> 128 -> 128 -> 6 tanh nets trained and
+    evaluated by the library's HIP kernels."""
+
+    def __init__(self, nets: int = 4, mode: str = "bf16", seed: int = 0, device: int = 0):
+        self.lib = load_library()
+        h = C.c_void_p()
+        _check(self.lib, self.lib.lbm_mlp_create(nets, MLP_MODES[mode], seed, device, C.byref(h)))
+        self.h, self.nets, self.mode = h, nets, mode
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.lbm_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @staticmethod
+    def _f32(a, cols):
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        if a.ndim != 2 or a.shape[1] != cols:
+            raise ValueError(f"expected an (n, {cols}) array")
+        return a
+
+    def params(self, k: int) -> np.ndarray:
+        p = np.empty(MLP_PARAMS, np.float32)
+        _check(self.lib, self.lib.lbm_mlp_get_params(self.h, k, _fp(p)))
+        return p
+
+    def set_params(self, k: int, p):
+        p = np.ascontiguousarray(p, dtype=np.float32).reshape(-1)
+        if p.size != MLP_PARAMS:
+            raise ValueError("wrong parameter count")
+        _check(self.lib, self.lib.lbm_mlp_set_params(self.h, k, _fp(p)))
+
+    def forward(self, x) -> np.ndarray:
+        x = self._f32(x, MLP_IN)
+        y = np.empty((self.nets, x.shape[0], MLP_OUT), np.float32)
+        _check(self.lib, self.lib.lbm_mlp_forward(self.h, _fp(x), x.shape[0], _fp(y)))
+        return y
+
+    def predict(self, x) -> tuple[np.ndarray, np.ndarray]:
+        x = self._f32(x, MLP_IN)
+        mean = np.empty((x.shape[0], MLP_OUT), np.float32)
+        spread = np.empty(x.shape[0], np.float32)
+        _check(self.lib, self.lib.lbm_mlp_predict(self.h, _fp(x), x.shape[0], _fp(mean), _fp(spread)))
+        return mean, spread
+
+    def loss_grad(self, k: int, x, t) -> tuple[float, np.ndarray]:
+        x, t = self._f32(x, MLP_IN), self._f32(t, MLP_OUT)
+        loss = C.c_float()
+        g = np.empty(MLP_PARAMS, np.float32)
+        _check(self.lib, self.lib.lbm_mlp_loss_grad(self.h, k, _fp(x), _fp(t), x.shape[0], C.byref(loss), _fp(g)))
+        return loss.value, g
+
+    def train(self, x, t, epochs: int, batch: int = 256, lr: float = 1e-3) -> np.ndarray:
+        x, t = self._f32(x, MLP_IN), self._f32(t, MLP_OUT)
+        losses = np.empty(self.nets, np.float32)
+        _check(self.lib, self.lib.lbm_mlp_train(self.h, _fp(x), _fp(t), x.shape[0], epochs, batch, lr,
+                                                _fp(losses)))
+        return losses

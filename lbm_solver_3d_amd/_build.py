@@ -5,6 +5,7 @@ Three translation units:
                     -ffp-contract=off (reference rounding)
   kernels_fast.hip  FP32 / FP16 / BF16 step kernels
   lbm_hip.hip       the C ABI (include/lbm_hip.h)
+  mlp.hip           the learned velocity surrogate (include/lbm_mlp.h)
 """
 from __future__ import annotations
 
@@ -32,6 +33,7 @@ UNITS = {
     "kernels_fp64": ["-ffp-contract=off"],
     "kernels_fast": ["-ffp-contract=fast"],
     "lbm_hip": ["-ffp-contract=off"],
+    "mlp": ["-ffp-contract=fast"],
 }
 
 

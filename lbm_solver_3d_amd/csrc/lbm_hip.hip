@@ -97,6 +97,9 @@ struct lbm_hip_ctx {
     bool owned_by_batch = false;  // member of an lbm_hip_batch
 };
 
+// Error text of the other entry points of the library (the MLP surrogate).
+void lbmhip_set_error_text(const char* msg) { g_err = msg; }
+
 extern "C" {
 
 const char* lbm_hip_last_error(void) { return g_err.c_str(); }
