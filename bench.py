@@ -159,8 +159,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--precisions", default="fp16,fp32")
-    ap.add_argument("--workload", default="channel", choices=["channel", "ensemble"],
-                    help="channel: the 512x256x128 slit channel (headline); ensemble: 64 slit-channel runs batched")
+    ap.add_argument("--workload", default="channel", choices=["channel", "ensemble", "surrogate"],
+                    help="channel: the 512x256x128 slit channel (headline); ensemble: 64 slit-channel runs batched; "
+                         "surrogate: the learned velocity surrogate with LBM fallback, 1000 trajectories")
     ap.add_argument("--no-sequential", action="store_true",
                     help="ensemble: skip timing the runs one after another")
     ap.add_argument("--copy-probe", action="store_true",
@@ -197,6 +198,21 @@ def main():
                             "ms_per_step": round(e["sequential_ms_per_step"], 5)} if e["sequential_mlups"] else None),
             "batch_over_sequential": (round(e["mlups"] / e["sequential_mlups"], 3) if e["sequential_mlups"] else None),
             "tuning": e["tuning"], "device_gb": round(e["device_gb"], 2)}))
+        return
+    if args.workload == "surrogate":
+        from lbm_solver_3d_amd import surrogate as S
+        log = lambda m: print(m, file=sys.stderr, flush=True)
+        rep = S.study(log=log)
+        t = rep["trajectories"]
+        traj_s = t["count"] / (t["surrogate_seconds"] + t["fallback_seconds"])
+        print(json.dumps({
+            "metric": "trajectories/s, learned velocity surrogate with LBM fallback (1000 slit-channel swimmers)",
+            "value": round(traj_s, 2), "unit": "trajectories/s", "n_gpus": 1, "steps": t["steps"], "warmup": 0,
+            "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16",
+            "data": "synthetic", "config": {"workload": "MLP 5-128-128-128-6 x4, trained on batched LBM runs; "
+                                                        "slit channels H in [16,32], R in [3,6], B1=1e-2",
+                                            "parallelism": "single GPU"},
+            "report": rep}))
         return
     res = {}
     for prec in args.precisions.split(","):
