@@ -36,11 +36,14 @@ WORKLOAD = dict(nx=512, ny=256, nz=128, tau=1.0, walls_y=0, walls_z=1, squirmer=
 MASS_EVERY = 10
 
 
-def measure(lbm, precision: str, steps: int, warmup: int, tune: bool = True) -> dict:
+def measure(lbm, precision: str, steps: int, warmup: int, tiling=None) -> dict:
     with lbm.Lattice(**WORKLOAD, precision=precision) as L:
         info = L.info()
-        # tile shape chosen by timing the candidates (state left unchanged)
-        tuning = L.autotune(trial_steps=20, mass_every=MASS_EVERY) if tune else None
+        if tiling:  # a fixed tile shape (profiling runs)
+            L.set_tiling(*tiling)
+            tuning = dict(tile_width=tiling[0], xcd_remap=tiling[1], candidates=[])
+        else:       # tile shape chosen by timing the candidates (state left unchanged)
+            tuning = L.autotune(trial_steps=20, mass_every=MASS_EVERY)
         if warmup:
             L.time_run(warmup, MASS_EVERY)
         L.synchronize()
@@ -162,6 +165,8 @@ def main():
     ap.add_argument("--workload", default="channel", choices=["channel", "ensemble", "surrogate"],
                     help="channel: the 512x256x128 slit channel (headline); ensemble: 64 slit-channel runs batched; "
                          "surrogate: the learned velocity surrogate with LBM fallback, 1000 trajectories")
+    ap.add_argument("--tiling", default=None,
+                    help="WIDTH,XCD: use this collide-stream tile shape instead of autotuning (e.g. 256,0)")
     ap.add_argument("--no-sequential", action="store_true",
                     help="ensemble: skip timing the runs one after another")
     ap.add_argument("--copy-probe", action="store_true",
@@ -217,7 +222,8 @@ def main():
     res = {}
     for prec in args.precisions.split(","):
         print(f"[bench] {prec}: warmup {args.warmup}, timed {args.steps}", file=sys.stderr, flush=True)
-        res[prec] = measure(lbm, prec, args.steps, args.warmup)
+        tiling = tuple(int(v) for v in args.tiling.split(",")) if args.tiling else None
+        res[prec] = measure(lbm, prec, args.steps, args.warmup, tiling)
         print(f"[bench] {prec}: {res[prec]['mlups']:.0f} MLUPS, kernel {res[prec]['kernel_ms']:.4f} ms "
               f"({res[prec]['frac'] * 100:.1f}% of HBM peak)", file=sys.stderr, flush=True)
     head = res.get("fp16") or next(iter(res.values()))
