@@ -19,6 +19,8 @@ from __future__ import annotations
 
 import argparse
 import json
+
+import numpy as np
 import os
 import sys
 import time
@@ -65,6 +67,48 @@ def measure(lbm, precision: str, steps: int, warmup: int, tiling=None) -> dict:
         tile_width=tuning["tile_width"] if tuning else None, xcd_remap=tuning["xcd_remap"] if tuning else None,
         tuning=tuning,
     )
+
+
+def trajectory_errors(lbm, steps: int, every: int = 500, precisions=("fp64", "fp32", "fp16", "bf16"),
+                      log=None) -> dict:
+    """The 512x256x128 pusher run `steps` steps in each storage precision,
+    the squirmer state recorded every `every` steps; errors against the FP64
+    run: velocity |U - U64| / |U64| (mean and max over the samples after the
+    first 2000 steps), position |X - X64| relative to the FP64 distance
+    travelled, heading angle."""
+    runs = {}
+    for prec in precisions:
+        t0 = time.perf_counter()
+        rows = []
+        with lbm.Lattice(**WORKLOAD, precision=prec) as L:
+            done = 0
+            while done < steps:
+                L.run(every, MASS_EVERY, 0)
+                done += every
+                s = L.squirmer()
+                rows.append((done, s["X"], s["U"], s["e"], s["W"]))
+        runs[prec] = rows
+        if log:
+            log(f"[bench] trajectory {prec}: {steps} steps in {time.perf_counter() - t0:.1f} s")
+    ref = runs["fp64"]
+    X0 = ref[0][1] - ref[0][2] * every  # start (approximately: X before the first window)
+    out = {"steps": steps, "every": every,
+           "fp64_final": {"X": list(map(float, ref[-1][1])), "U": list(map(float, ref[-1][2]))},
+           "fp64_travel": float(np.linalg.norm(ref[-1][1] - X0))}
+    for prec, rows in runs.items():
+        if prec == "fp64":
+            continue
+        ve, pe, he = [], [], []
+        for (t, X, U, e, _), (_, X64, U64, e64, _) in zip(rows, ref):
+            if t < 2000:
+                continue
+            ve.append(np.linalg.norm(U - U64) / np.linalg.norm(U64))
+            pe.append(np.linalg.norm(X - X64))
+            he.append(np.degrees(np.arccos(np.clip(np.dot(e, e64), -1.0, 1.0))))
+        out[prec] = {"velocity_rel_err_mean": float(np.mean(ve)), "velocity_rel_err_max": float(np.max(ve)),
+                     "position_err_final": float(pe[-1]), "position_err_rel_travel": float(pe[-1] / out["fp64_travel"]),
+                     "heading_err_deg_final": float(he[-1])}
+    return out
 
 
 def measure_ensemble(lbm, steps: int, warmup: int, count: int = 64, sequential: bool = True) -> dict:
@@ -162,9 +206,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--precisions", default="fp16,fp32")
-    ap.add_argument("--workload", default="channel", choices=["channel", "ensemble", "surrogate"],
+    ap.add_argument("--workload", default="channel", choices=["channel", "ensemble", "surrogate", "trajectory"],
                     help="channel: the 512x256x128 slit channel (headline); ensemble: 64 slit-channel runs batched; "
-                         "surrogate: the learned velocity surrogate with LBM fallback, 1000 trajectories")
+                         "surrogate: the learned velocity surrogate with LBM fallback, 1000 trajectories; "
+                         "trajectory: the 512x256x128 pusher in FP32 / FP16 / BF16 storage against FP64 "
+                         "(--steps steps)")
     ap.add_argument("--tiling", default=None,
                     help="WIDTH,XCD: use this collide-stream tile shape instead of autotuning (e.g. 256,0)")
     ap.add_argument("--no-sequential", action="store_true",
@@ -203,6 +249,14 @@ def main():
                             "ms_per_step": round(e["sequential_ms_per_step"], 5)} if e["sequential_mlups"] else None),
             "batch_over_sequential": (round(e["mlups"] / e["sequential_mlups"], 3) if e["sequential_mlups"] else None),
             "tuning": e["tuning"], "device_gb": round(e["device_gb"], 2)}))
+        return
+    if args.workload == "trajectory":
+        log = lambda m: print(m, file=sys.stderr, flush=True)
+        rep = trajectory_errors(lbm, args.steps, log=log)
+        print(json.dumps({"metric": "squirmer velocity / trajectory error of FP32, FP16, BF16 storage against FP64, "
+                                    "512x256x128 pusher", "value": rep["fp16"]["velocity_rel_err_mean"],
+                          "unit": "relative velocity error (FP16 store, mean)", "n_gpus": 1, "steps": args.steps,
+                          "warmup": 0, "higher_is_better": False, "data": "synthetic", "report": rep}))
         return
     if args.workload == "surrogate":
         from lbm_solver_3d_amd import surrogate as S

@@ -245,3 +245,16 @@ def test_c2_bulk_128_speed(lbm):
     assert abs(u["fp32"] / u["fp64"] - 1) < 1e-4
     assert abs(u["fp16"] / u["fp64"] - 1) < 5e-3
     assert abs(u["bf16"] / u["fp64"] - 1) < 5e-2
+
+
+def test_c3_storage_precisions_against_fp64_trajectory():
+    """The 512x256x128 pusher (the bench workload) for 4000 steps in FP32,
+    FP16 and BF16 storage against the FP64 run: squirmer velocity within 1 %
+    (FP32, FP16; BF16 within 5 %), measured FP16 ~1e-4."""
+    import bench
+    import lbm_solver_3d_amd as lbm
+    rep = bench.trajectory_errors(lbm, 4000, every=500)
+    print(rep)
+    assert rep["fp32"]["velocity_rel_err_max"] < 1e-3, rep["fp32"]
+    assert rep["fp16"]["velocity_rel_err_max"] < 1e-2, rep["fp16"]
+    assert rep["bf16"]["velocity_rel_err_max"] < 5e-2, rep["bf16"]
