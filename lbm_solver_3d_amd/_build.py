@@ -1,9 +1,10 @@
 """Build liblbm_hip.so (HIP, gfx950) in-tree with hipcc.
 
-Three translation units:
+Four translation units:
   kernels_fp64.hip  FP64 step kernels, squirmer/remap and auxiliary kernels,
                     -ffp-contract=off (reference rounding)
-  kernels_fast.hip  FP32 / FP16 / BF16 step kernels
+  kernels_fast.hip  FP32 / FP16 / BF16 step kernels, -ffp-contract=off too
+                    (their multiply-adds are written as fmaf)
   lbm_hip.hip       the C ABI (include/lbm_hip.h)
   mlp.hip           the learned velocity surrogate (include/lbm_mlp.h)
 """
@@ -31,7 +32,7 @@ COMMON = [
 ]
 UNITS = {
     "kernels_fp64": ["-ffp-contract=off"],
-    "kernels_fast": ["-ffp-contract=fast"],
+    "kernels_fast": ["-ffp-contract=off"],  # multiply-adds are explicit fmaf (same rounding in every kernel)
     "lbm_hip": ["-ffp-contract=off"],
     "mlp": ["-ffp-contract=fast"],
 }

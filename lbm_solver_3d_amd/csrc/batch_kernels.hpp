@@ -28,7 +28,7 @@ struct BatchMember {
 
 // Per-member arguments of the mass-correction finalize.
 struct BatchFinalize {
-    const double* part;
+    const MassPart* part;
     DevState* st;
     int G, bblocks, has_squirmer;
 };
@@ -36,21 +36,7 @@ struct BatchFinalize {
 constexpr uint32_t BATCH_LOCAL_MASK = 0xFFFFFFu;
 
 // The member tables are not written during a launch: the kernels read them
-// through the constant address space, so the loads are scalar and may be
-// repeated where a value is needed, as with kernel arguments, instead of
-// holding copies in vector registers (which cost the collide-stream kernel a
-// third of its occupancy).
-template <class T>
-__device__ __forceinline__ const __attribute__((address_space(4))) T* as_constant(const T* p) {
-    return (const __attribute__((address_space(4))) T*)p;
-}
-
-template <class T>
-__device__ __forceinline__ T from_constant(const T* p) {
-    T out;
-    __builtin_memcpy(&out, as_constant(p), sizeof(T));
-    return out;
-}
+// with from_constant (scalar loads, see device_state.hpp).
 
 template <class S, bool ODD, int MODE>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(LBM_STEP_WAVES_PER_EU)))
@@ -94,6 +80,7 @@ __global__ __launch_bounds__(SQ_THREADS) void batch_squirmer_kernel(const BatchM
     RemapArgs<S> R = from_constant(&mb->R);
     R.shifted = (t_after > 0 && ((t_after - 1) & 1) == 0) ? 1 : 0;
     R.shift_slot = (int)(t_after & 1);
+    R.accumulate = 0;
     squirmer_block<S>(R);
 }
 

@@ -1,11 +1,31 @@
 // device_state.hpp -- device-resident squirmer state, coupling accumulators
 // and the argument blocks of the kernels.
 #pragma once
+#include <hip/hip_runtime.h>
+
+#include <cmath>
 #include <cstdint>
 
 #include "lattice.hpp"
 
 namespace lbmhip {
+
+// Data that no block writes while a kernel runs (the per-member argument
+// tables of an ensemble) is read through the constant address space, so
+// the loads are scalar and the values stay in scalar registers, as kernel
+// arguments do, instead of vector registers (which cost the collide-stream
+// kernel a third of its occupancy).
+template <class T>
+__device__ __forceinline__ const __attribute__((address_space(4))) T* as_constant(const T* p) {
+    return (const __attribute__((address_space(4))) T*)p;
+}
+
+template <class T>
+__device__ __forceinline__ T from_constant(const T* p) {
+    T out;
+    __builtin_memcpy(&out, as_constant(p), sizeof(T));
+    return out;
+}
 
 // struct Squirmer (reference squirmer.hpp:14-37), kept on the device so a
 // whole time step (fluid update + rigid-body update + remap) runs without a
@@ -18,6 +38,19 @@ struct DevSquirmer {
     long long contacts;
     int planar, plane_xz;
 };
+
+// Excess-mass partial sum in 128-bit fixed point: value hi + lo * 2^-64.
+// Integer addition is associative, so the total of the partials does not
+// depend on how the nodes were grouped into tiles, blocks and kernels (tile
+// shape, XCD mapping, bulk / boundary split, ensemble or single lattice);
+// each node's excess is truncated to a multiple of 2^-64 (an error below
+// 6e-20 per node).
+struct MassPart {
+    long long hi;
+    unsigned long long lo;
+};
+
+constexpr int FIN_MAX_BLOCKS = 128;  // blocks of the mass-correction finalize
 
 // Everything the kernels exchange through device memory besides the fields.
 // The mass shift is double-buffered by step parity: step t reads slot t&1
@@ -33,9 +66,10 @@ struct DevState {
     double link_dmass;     // mass change of the moving-boundary corrections of the last step
     unsigned link_ticket;  // last-block counters of the two link kernels
     unsigned wall_ticket;
-    // Mass change made by remaps since the last step, as excess over one
+    // Mass change made by the remaps since the last step, as excess over one
     // unit per fluid node (cover removes sum_i f_i - 1, uncover adds
-    // sum_i feq_i - 1).
+    // sum_i feq_i - 1).  The first remap after a step sets it, later ones
+    // add to it (the host tells the remap which).
     double remap_dmass;
     long long nfluid;   // number of FLUID nodes, maintained by the remap
     int band[6];        // x0, y0, z0 (wrapped) and widths of the rho band
@@ -47,6 +81,9 @@ struct DevState {
     // update leaves it alone).
     double xf[3];
     double rf;
+    // block sums of the mass-correction finalize and its last-block counter
+    MassPart fin_part[FIN_MAX_BLOCKS];
+    unsigned fin_ticket;
 };
 
 constexpr int ERR_REMAP_OVERFLOW = 1;
@@ -61,7 +98,7 @@ struct StepArgs {
     void* uy;
     void* uz;
     DevState* st;
-    double* mass_part;    // one excess-mass partial per block
+    MassPart* mass_part;  // one excess-mass partial per block
     int NX, NY, NZ;
     uint32_t N;
     uint32_t SS;          // slot stride: population i of node (x, y, z) is f[i * SS + x + PX (y + NY z)]

@@ -1,5 +1,6 @@
 // kernels_fast.hip -- FP32-arithmetic step kernels for FP32 / FP16 / BF16
-// population storage (compiled with floating-point contraction allowed).
+// population storage (compiled without contraction; the collision's
+// multiply-adds are explicit fmaf).
 #include "launch_step_impl.hpp"
 
 namespace lbmhip {

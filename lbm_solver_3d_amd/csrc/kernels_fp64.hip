@@ -43,7 +43,7 @@ hipError_t launch_scatter(S* f, const uint8_t* flag, const double* in, int NX, i
 }
 
 template <class S>
-hipError_t launch_mass(const S* f, const uint8_t* flag, double* part, int NX, int NY, int NZ, uint32_t N,
+hipError_t launch_mass(const S* f, const uint8_t* flag, MassPart* part, int NX, int NY, int NZ, uint32_t N,
                        uint32_t SS, uint32_t PX, int shifted, const DevState* st, int shift_slot, hipStream_t s) {
     hipLaunchKernelGGL((mass_kernel<S>), dim3((N + 255) / 256), dim3(256), 0, s, f, flag, part, NX, NY, NZ, N, SS,
                        PX, shifted, st, shift_slot);
@@ -91,7 +91,7 @@ hipError_t launch_batch_squirmer(const BatchMember<S>* m, int count, long long t
                                          uint32_t, int, const DevState*, int, hipStream_t);                                \
     template hipError_t launch_scatter<S>(S*, const uint8_t*, const double*, int, int, int, uint32_t, uint32_t,  \
                                           uint32_t, int, hipStream_t);                                                     \
-    template hipError_t launch_mass<S>(const S*, const uint8_t*, double*, int, int, int, uint32_t, uint32_t,     \
+    template hipError_t launch_mass<S>(const S*, const uint8_t*, MassPart*, int, int, int, uint32_t, uint32_t,   \
                                        uint32_t, int, const DevState*, int, hipStream_t);                                  \
     template hipError_t launch_fill_populations<S>(S*, uint32_t, hipStream_t);
 
@@ -112,36 +112,85 @@ template hipError_t launch_batch_boundary<double>(const BatchMember<double>*, co
 // ---------------------------------------------------------------------------
 // Mass correction (lbm.cpp:324-339): d = 1 - m / n_fluid with
 // m = n_fluid + excess, so d = -excess / n_fluid; the shift is added to the
-// slot the next step reads.  One block, fixed summation order.
+// slot the next step reads.  One block; the partials are fixed-point
+// (MassPart), so their total is exact and independent of the grouping.
+// `terms` (MASS_TERM_*): the mass changes made after the partials were
+// taken that the excess includes.
 // ---------------------------------------------------------------------------
-__device__ void mass_finalize_block(const double* part, int nparts, DevState* st, int use_remap_dmass,
-                                    int shift_slot) {
-    __shared__ double lds[16];
-    // eight independent partial sums per thread keep eight loads in flight;
-    // the combination order is fixed, so the result is deterministic
-    double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    int k = threadIdx.x;
-    for (; k + 7 * 1024 < nparts; k += 8 * 1024) {
-#pragma unroll
-        for (int j = 0; j < 8; ++j) a[j] += part[k + j * 1024];
+// Block sum of fixed-point values (blockDim.x a multiple of 64, at most
+// 1024); thread 0 gets the total.
+__device__ unsigned __int128 block_sum_fixed(unsigned __int128 v) {
+    __shared__ unsigned long long lds[2 * 16];
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long h = __shfl_xor((unsigned long long)(v >> 64), off, 64);
+        const unsigned long long l = __shfl_xor((unsigned long long)v, off, 64);
+        v += mass_join(h, l);
     }
-    for (int j = 0; k < nparts; k += 1024, ++j) a[j & 7] += part[k];
-    double s = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = s;
+    if ((threadIdx.x & 63) == 0) {
+        lds[2 * (threadIdx.x >> 6)] = (unsigned long long)(v >> 64);
+        lds[2 * (threadIdx.x >> 6) + 1] = (unsigned long long)v;
+    }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        double e = 0.0;
-        for (int w = 0; w < 16; ++w) e += lds[w];
-        if (use_remap_dmass) e += st->remap_dmass + st->link_dmass;
-        const long long nf = st->nfluid;
-        if (nf > 0) st->shift[shift_slot] += -e / (double)nf;
-    }
+    unsigned __int128 s = 0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += mass_join(lds[2 * w], lds[2 * w + 1]);
+    return s;
 }
 
-__global__ __launch_bounds__(1024) void mass_finalize_kernel(const double* part, int nparts, DevState* st,
-                                                             int use_remap_dmass, int shift_slot) {
-    mass_finalize_block(part, nparts, st, use_remap_dmass, shift_slot);
+// The shift d = -excess / n_fluid from the fixed-point excess total.
+__device__ void apply_mass_total(unsigned __int128 s, DevState* st, int terms, int shift_slot) {
+    // hi + lo 2^-64 as a double (|hi| < 2^53)
+    double e = (double)(long long)(unsigned long long)(s >> 64) + (double)(unsigned long long)s * 0x1p-64;
+    if (terms & MASS_TERM_REMAP) e += st->remap_dmass;
+    if (terms & MASS_TERM_LINKS) e += st->link_dmass;
+    const long long nf = st->nfluid;
+    if (nf > 0) st->shift[shift_slot] += -e / (double)nf;
+}
+
+__device__ __forceinline__ unsigned __int128 load_part(const MassPart* p) {
+    return mass_join((unsigned long long)p->hi, p->lo);
+}
+
+// One block of 1024 threads over all partials (an ensemble member's).
+__device__ void mass_finalize_block(const MassPart* part, int nparts, DevState* st, int terms, int shift_slot) {
+    unsigned __int128 a[4] = {0, 0, 0, 0};  // four loads in flight per thread
+    int k = threadIdx.x;
+    for (; k + 3 * 1024 < nparts; k += 4 * 1024) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) a[j] += load_part(&part[k + j * 1024]);
+    }
+    for (; k < nparts; k += 1024) a[0] += load_part(&part[k]);
+    const unsigned __int128 s = block_sum_fixed((a[0] + a[1]) + (a[2] + a[3]));
+    if (threadIdx.x == 0) apply_mass_total(s, st, terms, shift_slot);
+}
+
+// Single lattice: FIN blocks each sum a strided share of the partials into
+// st->fin_part; the last block to finish adds those and sets the shift.
+constexpr int FIN_BLOCK = 256;
+__global__ __launch_bounds__(FIN_BLOCK) void mass_finalize_kernel(const MassPart* part, int nparts, DevState* st,
+                                                                  int terms, int shift_slot) {
+    __shared__ bool is_last;
+    unsigned __int128 v = 0;
+    for (int k = blockIdx.x * FIN_BLOCK + threadIdx.x; k < nparts; k += gridDim.x * FIN_BLOCK) v += load_part(&part[k]);
+    v = block_sum_fixed(v);
+    if (threadIdx.x == 0) {
+        st->fin_part[blockIdx.x].hi = (long long)(unsigned long long)(v >> 64);
+        st->fin_part[blockIdx.x].lo = (unsigned long long)v;
+        __threadfence();
+        is_last = atomicAdd(&st->fin_ticket, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!is_last) return;
+    __threadfence();
+    unsigned __int128 w = 0;
+    for (unsigned b = threadIdx.x; b < gridDim.x; b += FIN_BLOCK)
+        w += mass_join((unsigned long long)__builtin_nontemporal_load(&st->fin_part[b].hi),
+                       __builtin_nontemporal_load(&st->fin_part[b].lo));
+    w = block_sum_fixed(w);
+    if (threadIdx.x == 0) {
+        apply_mass_total(w, st, terms, shift_slot);
+        st->fin_ticket = 0u;  // ready for the next launch
+    }
 }
 
 // One block per ensemble member; members without a squirmer take no mass
@@ -151,7 +200,8 @@ __global__ __launch_bounds__(1024) void batch_finalize_kernel(const BatchFinaliz
     const BatchFinalize& f = fa[blockIdx.x];
     if (!f.has_squirmer) return;
     const bool last_odd = ((t_after - 1) & 1) != 0;
-    mass_finalize_block(f.part, f.G + (last_odd ? 0 : f.bblocks), f.st, 1, (int)(t_after & 1));
+    mass_finalize_block(f.part, f.G + (last_odd ? 0 : f.bblocks), f.st, MASS_TERM_LINKS | MASS_TERM_REMAP,
+                        (int)(t_after & 1));
 }
 
 hipError_t launch_batch_finalize(const BatchFinalize* fa, int count, long long t_after, hipStream_t s) {
@@ -160,9 +210,11 @@ hipError_t launch_batch_finalize(const BatchFinalize* fa, int count, long long t
     return hipGetLastError();
 }
 
-hipError_t launch_mass_finalize(const double* part, int nparts, DevState* st, int use_remap_dmass, int shift_slot,
+hipError_t launch_mass_finalize(const MassPart* part, int nparts, DevState* st, int terms, int shift_slot,
                                 hipStream_t s) {
-    hipLaunchKernelGGL(mass_finalize_kernel, dim3(1), dim3(1024), 0, s, part, nparts, st, use_remap_dmass, shift_slot);
+    int blocks = (nparts + 4 * FIN_BLOCK - 1) / (4 * FIN_BLOCK);  // about four partials per thread
+    blocks = blocks < 1 ? 1 : (blocks > FIN_MAX_BLOCKS ? FIN_MAX_BLOCKS : blocks);
+    hipLaunchKernelGGL(mass_finalize_kernel, dim3(blocks), dim3(FIN_BLOCK), 0, s, part, nparts, st, terms, shift_slot);
     return hipGetLastError();
 }
 

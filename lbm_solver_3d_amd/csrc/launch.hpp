@@ -1,7 +1,9 @@
 // launch.hpp -- host-side launchers of the device kernels.  The FP64 kernels
 // and the squirmer/remap kernels are compiled in kernels_fp64.hip with
 // floating-point contraction off (reference operation order); the FP32 /
-// FP16 / BF16 step kernels in kernels_fast.hip with contraction on.
+// FP16 / BF16 step kernels in kernels_fast.hip, also without contraction:
+// their multiply-adds are explicit fmaf, so every kernel that updates a node
+// (bulk, boundary, ensemble) rounds the same way.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -40,7 +42,7 @@ hipError_t launch_scatter(S* f, const uint8_t* flag, const double* in, int NX, i
                           uint32_t SS, uint32_t PX, int shifted, hipStream_t s);
 
 template <class S>
-hipError_t launch_mass(const S* f, const uint8_t* flag, double* part, int NX, int NY, int NZ, uint32_t N,
+hipError_t launch_mass(const S* f, const uint8_t* flag, MassPart* part, int NX, int NY, int NZ, uint32_t N,
                        uint32_t SS, uint32_t PX, int shifted, const DevState* st, int shift_slot, hipStream_t s);
 
 template <class S>
@@ -60,7 +62,9 @@ hipError_t launch_batch_squirmer(const BatchMember<S>* m, int count, long long t
 hipError_t launch_batch_finalize(const BatchFinalize* fa, int count, long long t_after, hipStream_t s);
 
 // Storage-independent kernels (kernels_fp64.hip).
-hipError_t launch_mass_finalize(const double* part, int nparts, DevState* st, int use_remap_dmass, int shift_slot,
+constexpr int MASS_TERM_LINKS = 1;  // moving-boundary corrections of the last step (link_dmass)
+constexpr int MASS_TERM_REMAP = 2;  // remaps since the last step (remap_dmass)
+hipError_t launch_mass_finalize(const MassPart* part, int nparts, DevState* st, int terms, int shift_slot,
                                 hipStream_t s);
 hipError_t launch_diag(const uint8_t* flag, const void* rho, const void* ux, const void* uy, const void* uz,
                        int is_double, uint32_t N, double* part, int nparts, double* out4, hipStream_t s);
@@ -79,7 +83,7 @@ hipError_t launch_macro_to_double(const void* src, int is_double, double* dst, u
                                                 uint32_t, uint32_t, int, const DevState*, int, hipStream_t);              \
     extern template hipError_t launch_scatter<S>(S*, const uint8_t*, const double*, int, int, int, uint32_t,     \
                                                  uint32_t, uint32_t, int, hipStream_t);                                   \
-    extern template hipError_t launch_mass<S>(const S*, const uint8_t*, double*, int, int, int, uint32_t,        \
+    extern template hipError_t launch_mass<S>(const S*, const uint8_t*, MassPart*, int, int, int, uint32_t,      \
                                               uint32_t, uint32_t, int, const DevState*, int, hipStream_t);                \
     extern template hipError_t launch_fill_populations<S>(S*, uint32_t, hipStream_t);                         \
     extern template hipError_t launch_batch_step<S>(const BatchMember<S>*, const uint32_t*, int, long long, int, \

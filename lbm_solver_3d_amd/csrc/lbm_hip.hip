@@ -70,7 +70,7 @@ struct lbm_hip_ctx {
     void* uy = nullptr;
     void* uz = nullptr;
     DevState* st = nullptr;
-    double* mass_part = nullptr;  // >= max(G, ceil(N/256)) doubles
+    MassPart* mass_part = nullptr;  // >= max(G, ceil(N/256)) fixed-point partials
     double* diag_part = nullptr;  // 4 * ceil(N/256) doubles
     double* diag_out = nullptr;   // 4 doubles
     double* red_part = nullptr;   // 8 * max(LINK_MAX_BLOCKS, ceil(N/256)) doubles (link / wall kernels)
@@ -95,6 +95,8 @@ struct lbm_hip_ctx {
     hipGraphExec_t graph = nullptr;
     int graph_cycle = 0, graph_mass_every = -1;
     bool owned_by_batch = false;  // member of an lbm_hip_batch
+    double radius_max = 0.0;      // radius the boundary work list is sized for
+    int remaps_since_step = 0;    // lbm_hip_advance_squirmer calls since the last lbm_hip_step
 };
 
 // Error text of the other entry points of the library (the MLP surrogate).
@@ -233,6 +235,7 @@ RemapArgs<S> remap_args(const lbm_hip_ctx* c) {
     a.walls_y = c->p.walls_y; a.walls_z = c->p.walls_z;
     a.shifted = (c->t > 0 && ((c->t - 1) & 1) == 0) ? 1 : 0;
     a.shift_slot = (int)(c->t & 1);
+    a.accumulate = c->remaps_since_step > 0 ? 1 : 0;
     return a;
 }
 
@@ -381,6 +384,7 @@ int lbm_hip_create(const lbm_hip_params* pin, lbm_hip_ctx** out) {
             const int w = (int)std::ceil(2.0 * pin->radius) + 5;
             b.box_rows_max = std::min(w, pin->ny) * std::min(w, pin->nz);
             b.box_chunks = std::min(b.nchx, (w + 63) / 64 + 2);
+            c->radius_max = pin->radius;
         }
         const long long waves = (long long)b.n_wall_rows * b.nchx + (long long)b.box_rows_max * b.box_chunks;
         c->bblocks = (int)((waves + (BLOCK / 64) - 1) / (BLOCK / 64));
@@ -410,7 +414,7 @@ int lbm_hip_create(const lbm_hip_params* pin, lbm_hip_ctx** out) {
     if (!dmalloc(&c->f, (size_t)Q * c->SS * c->ssize) || !dmalloc((void**)&c->flag, N) ||
         !dmalloc(&c->rho, N * c->msize) || !dmalloc(&c->ux, N * c->msize) || !dmalloc(&c->uy, N * c->msize) ||
         !dmalloc(&c->uz, N * c->msize) || !dmalloc((void**)&c->st, sizeof(DevState)) ||
-        !dmalloc((void**)&c->mass_part, sizeof(double) * nparts) ||
+        !dmalloc((void**)&c->mass_part, sizeof(MassPart) * nparts) ||
         !dmalloc((void**)&c->diag_part, sizeof(double) * 4 * c->nb256) ||
         !dmalloc((void**)&c->diag_out, sizeof(double) * 4) ||
         !dmalloc((void**)&c->red_part, sizeof(double) * 8 * std::max(LINK_MAX_BLOCKS, c->nb256)) ||
@@ -495,7 +499,7 @@ int lbm_hip_create(const lbm_hip_params* pin, lbm_hip_ctx** out) {
             using S = std::remove_pointer_t<decltype(tag)>;
             return launch_fill_populations<S>((S*)c->f, c->SS, c->stream) == hipSuccess ? 0 : -1;
         }) != 0 ||
-        hipMemsetAsync(c->mass_part, 0, sizeof(double) * nparts, c->stream) != hipSuccess ||
+        hipMemsetAsync(c->mass_part, 0, sizeof(MassPart) * nparts, c->stream) != hipSuccess ||
         hipStreamSynchronize(c->stream) != hipSuccess)
         return cleanup(fail(LBM_HIP_EDEVICE, "context initialisation failed"));
     *out = c;
@@ -540,6 +544,7 @@ int lbm_hip_step(lbm_hip_ctx* c, int flags) {
     if (e != hipSuccess) return fail(LBM_HIP_EDEVICE, std::string("step launch: ") + hipGetErrorString(e));
     c->last_bparts = odd ? 0 : c->bblocks;
     ++c->t;
+    c->remaps_since_step = 0;
     c->macro_valid = (flags & LBM_STEP_MACRO) != 0;
     c->wallf_valid = (flags & LBM_STEP_WALL_FORCE) != 0;
     c->mass_valid = (flags & LBM_STEP_MASS) != 0;
@@ -556,6 +561,7 @@ int lbm_hip_advance_squirmer(lbm_hip_ctx* c) {
         return 0;
     });
     if (e != hipSuccess) return fail(LBM_HIP_EDEVICE, std::string("squirmer launch: ") + hipGetErrorString(e));
+    ++c->remaps_since_step;
     return LBM_HIP_OK;
 }
 
@@ -564,7 +570,8 @@ int lbm_hip_correct_mass(lbm_hip_ctx* c) {
     hipError_t e;
     const int shift_slot = (int)(c->t & 1);
     if (c->mass_valid) {
-        e = launch_mass_finalize(c->mass_part, c->G + c->last_bparts, c->st, 1, shift_slot, c->stream);
+        const int terms = MASS_TERM_LINKS | (c->remaps_since_step > 0 ? MASS_TERM_REMAP : 0);
+        e = launch_mass_finalize(c->mass_part, c->G + c->last_bparts, c->st, terms, shift_slot, c->stream);
     } else {
         const int shifted = layout_shifted(c) ? 1 : 0;
         e = hipSuccess;
@@ -641,6 +648,7 @@ static int run_loop(lbm_hip_ctx* c, int steps, int mass_every, int last_flags, b
             left -= reps * C;
             c->macro_valid = c->wallf_valid = false;
             c->mass_valid = false;
+            c->remaps_since_step = c->has_squirmer ? 1 : 0;
         }
     }
     while (left > 0) {
@@ -692,6 +700,9 @@ int lbm_hip_set_squirmer(lbm_hip_ctx* c, const lbm_squirmer_state* in) {
     s.contacts = in->contacts;
     s.planar = in->planar; s.plane_xz = in->plane_xz;
     if (c->has_squirmer) {
+        if (!(s.R > 0.0) || s.R > c->radius_max)
+            return fail(LBM_HIP_EINVAL, "squirmer radius must be positive and at most the radius the context was "
+                                        "created with (the boundary work list is sized for it)");
         const int lb = link_blocks_for(s.R, c->NX, c->NY, c->NZ);
         if (lb > LINK_MAX_BLOCKS) return fail(LBM_HIP_EINVAL, "squirmer radius too large");
         c->link_blocks = lb;

@@ -57,6 +57,7 @@ struct RemapArgs {
     int walls_y, walls_z;
     int shifted;     // layout after the last step: 1 after an even step
     int shift_slot;  // mass-shift slot the next step reads
+    int accumulate;  // 1: another remap ran since the last step (add to remap_dmass, do not set it)
 };
 
 // squirmer.cpp:73-77
@@ -464,7 +465,7 @@ __device__ void squirmer_block(const RemapArgs<S>& A) {
     if (wave == 0) {
         if (lane == 0) {
             for (int d = 0; d < 3; ++d) { sq.Fc[d] += Fc[d]; sq.Tc[d] += Tc[d]; }
-            st->remap_dmass += dmass;
+            st->remap_dmass = (A.accumulate ? st->remap_dmass : 0.0) + dmass;
             st->nfluid += dnf;
             st->changed = K;
         }
@@ -613,9 +614,9 @@ __global__ __launch_bounds__(256) void scatter_kernel(S* __restrict__ f, const u
 // one partial per block (used when the last step emitted no mass partials).
 template <class S>
 __global__ __launch_bounds__(256) void mass_kernel(const S* __restrict__ f, const uint8_t* __restrict__ flag,
-                                                   double* __restrict__ part, int NX, int NY, int NZ, uint32_t N,
+                                                   MassPart* __restrict__ part, int NX, int NY, int NZ, uint32_t N,
                                                    uint32_t SS, uint32_t PX, int shifted, const DevState* st, int shift_slot) {
-    __shared__ double lds[4];
+    static_assert(BLOCK == 256, "one mass partial per block of 256 nodes");
     const uint32_t n = blockIdx.x * 256u + threadIdx.x;
     double ex[1] = {0.0};
     if (n < N && (flag[n] & TYPE_MASK) == FLUID) {
@@ -636,8 +637,7 @@ __global__ __launch_bounds__(256) void mass_kernel(const S* __restrict__ f, cons
         }
         ex[0] = s - 1.0;
     }
-    block_reduce<1>(ex, lds);
-    if (threadIdx.x == 0) part[blockIdx.x] = ex[0];
+    block_mass_sum(ex[0], &part[blockIdx.x]);
 }
 
 }  // namespace lbmhip

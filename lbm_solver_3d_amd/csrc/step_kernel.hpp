@@ -127,6 +127,9 @@ struct FastCoef {
 template <bool FORCE>
 __device__ __forceinline__ void collide_fast(float (&g)[Q], const FastCoef& K, float& rho, float& jx, float& jy,
                                              float& jz, float& drho_s) {
+    // Every multiply-add is an explicit fmaf and the unit is compiled
+    // without contraction, so each kernel that inlines this (bulk, boundary,
+    // ensemble) rounds identically.
     // pair sums and differences feed both the moments and the collision
     float sp[9], df[9];
     static_for<0, 9>([&](auto P) {
@@ -142,20 +145,20 @@ __device__ __forceinline__ void collide_fast(float (&g)[Q], const FastCoef& K, f
     jy = ((df[1] + df[3]) - (df[4] - df[7])) + df[8];
     jz = ((df[2] + df[5]) - (df[6] - df[7])) - df[8];
     if constexpr (FORCE) { jx += K.hFx; jy += K.hFy; jz += K.hFz; }
-    rho = 1.0f + d * K.inv_s;
+    rho = fmaf(d, K.inv_s, 1.0f);
     const float isr = K.inv_s / rho;  // 1 / (s rho)
     const float q = 4.5f * isr;
-    const float jj = jx * jx + jy * jy + jz * jz;
-    const float t0 = d - 1.5f * isr * jj;
+    const float jj = fmaf(jz, jz, fmaf(jy, jy, jx * jx));
+    const float t0 = fmaf(-1.5f * isr, jj, d);
     float uF = 0.f, irho = 0.f;
     if constexpr (FORCE) {
         irho = 1.0f / rho;
-        uF = (jx * K.Fx + jy * K.Fy + jz * K.Fz);  // = s rho (u . F)
+        uF = fmaf(jz, K.Fz, fmaf(jy, K.Fy, jx * K.Fx));  // = s rho (u . F)
     }
     // rest population
     {
-        float r0 = K.one_m_omp * g[0] + K.kp_rest * t0;
-        if constexpr (FORCE) r0 -= K.gPw3[2] * uF * irho;
+        float r0 = fmaf(K.one_m_omp, g[0], K.kp_rest * t0);
+        if constexpr (FORCE) r0 = fmaf(-K.gPw3[2] * uF, irho, r0);
         g[0] = r0;
     }
     static_for<0, 9>([&](auto P) {
@@ -173,9 +176,9 @@ __device__ __forceinline__ void collide_fast(float (&g)[Q], const FastCoef& K, f
         else pcu = jy - jz;
         const float kp = diag ? K.kp_diag : K.kp_axis;
         const float km = diag ? K.km_diag : K.km_axis;
-        const float e = (q * pcu) * pcu + t0;
-        float Pp = kp * e - K.a * sp[p];
-        float Mm = km * pcu - K.b * df[p];
+        const float e = fmaf(q * pcu, pcu, t0);
+        float Pp = fmaf(kp, e, -K.a * sp[p]);
+        float Mm = fmaf(km, pcu, -K.b * df[p]);
         if constexpr (FORCE) {
             // c_i . F for this direction
             float cF;
@@ -189,8 +192,8 @@ __device__ __forceinline__ void collide_fast(float (&g)[Q], const FastCoef& K, f
             else if constexpr (p == 7) cF = K.Fy + K.Fz;
             else cF = K.Fy - K.Fz;
             const int k = diag ? 1 : 0;
-            Pp += (K.gPw9[k] * cF * pcu - K.gPw3[k] * uF) * irho;
-            Mm += (diag ? K.sm_diag : K.sm_axis) * cF;
+            Pp = fmaf(fmaf(K.gPw9[k] * cF, pcu, -K.gPw3[k] * uF), irho, Pp);
+            Mm = fmaf(diag ? K.sm_diag : K.sm_axis, cF, Mm);
         }
         const float gi = g[i], gio = g[io];
         g[i] = gi + Pp + Mm;
@@ -234,11 +237,11 @@ __device__ inline int wrap_coord(int a, int n) {
     a %= n;
     return a < 0 ? a + n : a;
 }
-__device__ inline LinkBoxDims link_box_dims(const DevSquirmer& sq, int NX, int NY, int NZ) {
+__device__ inline LinkBoxDims link_box_dims(const double X[3], double R, int NX, int NY, int NZ) {
     LinkBoxDims b;
-    const int x0 = (int)floor(sq.X[0] - sq.R) - 1, x1 = (int)ceil(sq.X[0] + sq.R) + 1;
-    const int y0 = (int)floor(sq.X[1] - sq.R) - 1, y1 = (int)ceil(sq.X[1] + sq.R) + 1;
-    const int z0 = (int)floor(sq.X[2] - sq.R) - 1, z1 = (int)ceil(sq.X[2] + sq.R) + 1;
+    const int x0 = (int)floor(X[0] - R) - 1, x1 = (int)ceil(X[0] + R) + 1;
+    const int y0 = (int)floor(X[1] - R) - 1, y1 = (int)ceil(X[1] + R) + 1;
+    const int z0 = (int)floor(X[2] - R) - 1, z1 = (int)ceil(X[2] + R) + 1;
     b.y0 = y0; b.z0 = z0;
     b.wy = (y1 - y0 + 1) < NY ? (y1 - y0 + 1) : NY;
     b.wz = (z1 - z0 + 1) < NZ ? (z1 - z0 + 1) : NZ;
@@ -252,6 +255,51 @@ __device__ inline LinkBoxDims link_box_dims(const DevSquirmer& sq, int NX, int N
         b.nck = (xs + wx - 1) / 64 - b.c0 + 1;
     if (b.nck > nchx) b.nck = nchx;
     return b;
+}
+__device__ inline LinkBoxDims link_box_dims(const DevSquirmer& sq, int NX, int NY, int NZ) {
+    return link_box_dims(sq.X, sq.R, NX, NY, NZ);
+}
+// Whether chunk c of row (y, z) lies in box b.
+__device__ inline bool in_link_box(const LinkBoxDims& b, int c, int y, int z, int nchx, int NY, int NZ) {
+    int dy = y - wrap_coord(b.y0, NY); dy += dy < 0 ? NY : 0;
+    int dz = z - wrap_coord(b.z0, NZ); dz += dz < 0 ? NZ : 0;
+    int dc = c - b.c0; dc += dc < 0 ? nchx : 0;
+    return dy < b.wy && dz < b.wz && dc < b.nck;
+}
+
+// Fixed-point value of one node's excess mass (see MassPart).
+__device__ __forceinline__ unsigned __int128 mass_fixed(double e) {
+    const double fl = floor(e);
+    const unsigned long long lo = (unsigned long long)((e - fl) * 0x1p64);  // e - fl in [0, 1), exact
+    return ((unsigned __int128)(unsigned long long)(long long)fl << 64) | lo;
+}
+
+__device__ __forceinline__ unsigned __int128 mass_join(unsigned long long hi, unsigned long long lo) {
+    return ((unsigned __int128)hi << 64) | lo;
+}
+
+// Block sum (blocks of BLOCK threads) of the fixed-point excess; thread 0
+// stores it.
+__device__ __forceinline__ void block_mass_sum(double excess, MassPart* out) {
+    __shared__ unsigned long long lds[2 * (BLOCK / 64)];
+    unsigned __int128 v = mass_fixed(excess);
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long h = __shfl_xor((unsigned long long)(v >> 64), off, 64);
+        const unsigned long long l = __shfl_xor((unsigned long long)v, off, 64);
+        v += mass_join(h, l);
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        lds[2 * w] = (unsigned long long)(v >> 64);
+        lds[2 * w + 1] = (unsigned long long)v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned __int128 s = 0;
+        for (int k = 0; k < BLOCK / 64; ++k) s += mass_join(lds[2 * k], lds[2 * k + 1]);
+        out->hi = (long long)(unsigned long long)(s >> 64);
+        out->lo = (unsigned long long)s;
+    }
 }
 
 // Wavefront + block reduction of K doubles; thread 0 of the block gets the sums.
@@ -349,9 +397,12 @@ __device__ __forceinline__ void store_stream(S* p, S v) {
 
 // Update one fluid node; returns its excess mass sum_i f*_i - 1 when MASS.
 // BPATH: even-step node of a boundary chunk (bounce-back redirections).
+// `row_solid` (BPATH): the solid directions, the same for every node of the
+// wavefront's row chunk (a row next to a wall, away from the squirmer), or
+// -1 to read them from the neighbours' flags.
 template <class S, bool ODD, int MODE, bool BPATH>
 __device__ __forceinline__ double node_update(const StepArgs<S>& A, const FastCoef& K, const RowGeom& rg, int x,
-                                              int y, int z, uint8_t fl) {
+                                              int y, int z, uint8_t fl, int row_solid = -1) {
     using St = Store<S>;
     using T = typename St::C;
     constexpr bool EXACT = std::is_same<S, double>::value;
@@ -377,7 +428,14 @@ __device__ __forceinline__ double node_update(const StepArgs<S>& A, const FastCo
     // come back from that node's slot opp(j) (bounce-back)
     uint32_t solid = 0;
     if constexpr (BPATH) {
-        if (fl & BOUNDARY) {
+        if (row_solid >= 0) {  // wavefront-uniform: no flag loads, uniform branches
+            solid = (uint32_t)row_solid;
+            static_for<1, Q>([&](auto I) {
+                constexpr int j = decltype(I)::value;
+                if ((solid >> OPP[j]) & 1u)
+                    g[j] = St::load(*at(base(OPP[j], yo[1 - CY[j]] + zo[1 - CZ[j]]), xb[1 - CX[j]]));
+            });
+        } else if (fl & BOUNDARY) {
             static_for<1, Q>([&](auto I) {
                 constexpr int i = decltype(I)::value;
                 const uint32_t m = rg.xo[1 + CX[i]] + rg.yo[1 + CY[i]] + rg.zo[1 + CZ[i]];
@@ -494,22 +552,13 @@ __device__ __forceinline__ void step_tile(const StepArgs<S>& A, const FastCoef& 
         if (__builtin_amdgcn_ballot_w64(active && (fl & BOUNDARY)) != 0) active = false;
     }
 
-    if (tile == 0 && threadIdx.x == 0) {
-        // clear the slot step t+1 reads its mass shift from, and the remap's
-        // mass bookkeeping of the previous step
-        A.st->shift[(A.t + 1) & 1] = 0.0;
-        A.st->remap_dmass = 0.0;
-    }
+    // clear the slot step t+1 reads its mass shift from
+    if (tile == 0 && threadIdx.x == 0) A.st->shift[(A.t + 1) & 1] = 0.0;
 
     double excess = 0.0;
     if (active) excess = node_update<S, ODD, MODE, false>(A, K, rg, x, y, z, fl);
 
-    if constexpr (MASS) {
-        __shared__ double lds[BLOCK / 64];
-        double m[1] = {excess};
-        block_reduce<1>(m, lds);
-        if (threadIdx.x == 0) A.mass_part[tile] = m[0];
-    }
+    if constexpr (MASS) block_mass_sum(excess, &A.mass_part[tile]);
 }
 
 template <class S, bool ODD, int MODE>
@@ -528,7 +577,7 @@ struct BoundaryArgs {
     int nchx;                   // 64-node chunks per row
     int box_rows_max, box_chunks;  // bound of box rows and chunks per box row
     int wy_lo, wy_hi, wz_lo, wz_hi;  // wall-adjacent coordinates (-1: none)
-    double* mass_part;          // partials after the step kernel's
+    MassPart* mass_part;        // partials after the step kernel's
 };
 
 template <class S, int MODE>
@@ -539,11 +588,32 @@ __device__ __forceinline__ void boundary_block(const StepArgs<S>& A, const FastC
     const int wave = block * (BLOCK / 64) + (int)(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     int y = -1, z = -1, chunk = 0;
+    int row_solid = -1;
     const int nwall = B.n_wall_rows * B.nchx;
     if (wave < nwall) {
         const uint32_t r = B.wall_rows[wave / B.nchx];
         y = (int)(r % NY); z = (int)(r / NY);
         chunk = wave % B.nchx;
+        // A wall-row chunk outside the squirmer's link box (around its
+        // centre and around the centre its PARTICLE flags were classified
+        // at) has the walls as its only solid neighbours: the same
+        // directions for every node.
+        bool near_sphere = false;
+        if (A.has_squirmer) {
+            const DevState* st = A.st;
+            near_sphere = in_link_box(link_box_dims(st->sq, A.NX, A.NY, A.NZ), chunk, y, z, B.nchx, A.NY, A.NZ) ||
+                          in_link_box(link_box_dims(st->xf, st->rf, A.NX, A.NY, A.NZ), chunk, y, z, B.nchx, A.NY,
+                                      A.NZ);
+        }
+        if (!near_sphere) {
+            uint32_t m = 0;
+            for (int i = 1; i < Q; ++i) {
+                const int yn = y + CY[i], zn = z + CZ[i];
+                const bool wall = (A.walls_y && (yn == 0 || yn == A.NY - 1)) || (A.walls_z && (zn == 0 || zn == A.NZ - 1));
+                m |= (uint32_t)wall << i;
+            }
+            row_solid = (int)m;
+        }
     } else if (A.has_squirmer) {
         const int w = wave - nwall;
         const DevSquirmer& sq = A.st->sq;
@@ -560,6 +630,7 @@ __device__ __forceinline__ void boundary_block(const StepArgs<S>& A, const FastC
     y = __builtin_amdgcn_readfirstlane(y);
     z = __builtin_amdgcn_readfirstlane(z);
     chunk = __builtin_amdgcn_readfirstlane(chunk);
+    row_solid = __builtin_amdgcn_readfirstlane(row_solid);
     const int x = chunk * 64 + lane;
     const bool inb = y >= 0 && (uint32_t)x < NX;
     const RowGeom rg = row_geom<S>(inb ? (uint32_t)x : 0u, y >= 0 ? (uint32_t)y : 0u, y >= 0 ? (uint32_t)z : 0u,
@@ -568,13 +639,8 @@ __device__ __forceinline__ void boundary_block(const StepArgs<S>& A, const FastC
     bool active = (fl & TYPE_MASK) == FLUID;
     if (__builtin_amdgcn_ballot_w64(active && (fl & BOUNDARY)) == 0) active = false;  // not a boundary chunk
     double excess = 0.0;
-    if (active) excess = node_update<S, false, MODE, true>(A, K, rg, x, y, z, fl);
-    if constexpr (MASS) {
-        __shared__ double lds[BLOCK / 64];
-        double m[1] = {excess};
-        block_reduce<1>(m, lds);
-        if (threadIdx.x == 0) B.mass_part[block] = m[0];
-    }
+    if (active) excess = node_update<S, false, MODE, true>(A, K, rg, x, y, z, fl, row_solid);
+    if constexpr (MASS) block_mass_sum(excess, &B.mass_part[block]);
 }
 
 template <class S, int MODE>
