@@ -139,7 +139,9 @@ int lbm_hip_run(lbm_hip_ctx* ctx, int steps, int mass_every, int last_flags);
 
 /* Read / write the public struct Squirmer member LBM::sq (lbm.hpp:78). */
 int lbm_hip_get_squirmer(lbm_hip_ctx* ctx, lbm_squirmer_state* out);
-/* The radius may not exceed the one the context was created with. */
+/* The radius may not exceed the one the context was created with, and the
+ * centre may move by less than one node per axis (the flags stay those of
+ * the old centre until the next remap, as in the reference's loop). */
 int lbm_hip_set_squirmer(lbm_hip_ctx* ctx, const lbm_squirmer_state* in);
 
 /* LBM::wallForce (lbm.hpp:75) after the last step; LBM_HIP_ESTATE if that

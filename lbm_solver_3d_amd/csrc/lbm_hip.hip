@@ -248,9 +248,10 @@ LinkArgs<S> link_args(const lbm_hip_ctx* c) {
     return a;
 }
 
-// Blocks of particle_links_kernel: its box is at most ceil(2R) + 5 wide per axis.
+// Blocks of particle_links_kernel: its box (around the covering sphere, of
+// radius < R + 1) is at most ceil(2R) + 7 wide per axis.
 int link_blocks_for(double R, int NX, int NY, int NZ) {
-    const int w = (int)std::ceil(2.0 * R) + 5;
+    const int w = (int)std::ceil(2.0 * R) + 7;
     const long long M = (long long)std::min(w, NX) * std::min(w, NY) * std::min(w, NZ);
     return (int)((M + LINK_BLOCK - 1) / LINK_BLOCK);
 }
@@ -381,7 +382,8 @@ int lbm_hip_create(const lbm_hip_params* pin, lbm_hip_ctx** out) {
         b.wy_lo = pin->walls_y ? 1 : -1; b.wy_hi = pin->walls_y ? pin->ny - 2 : -1;
         b.wz_lo = pin->walls_z ? 1 : -1; b.wz_hi = pin->walls_z ? pin->nz - 2 : -1;
         if (pin->squirmer) {
-            const int w = (int)std::ceil(2.0 * pin->radius) + 5;
+            // boundary box: at most ceil(2r) + 4 rows per axis for the covering sphere, r < R + 1
+            const int w = (int)std::ceil(2.0 * pin->radius) + 7;
             b.box_rows_max = std::min(w, pin->ny) * std::min(w, pin->nz);
             b.box_chunks = std::min(b.nchx, (w + 63) / 64 + 2);
             c->radius_max = pin->radius;
@@ -705,6 +707,22 @@ int lbm_hip_set_squirmer(lbm_hip_ctx* c, const lbm_squirmer_state* in) {
                                         "created with (the boundary work list is sized for it)");
         const int lb = link_blocks_for(s.R, c->NX, c->NY, c->NZ);
         if (lb > LINK_MAX_BLOCKS) return fail(LBM_HIP_EINVAL, "squirmer radius too large");
+        // The PARTICLE flags stay those of the centre they were classified
+        // at until the next remap, which only reclassifies around the new
+        // centre (as lbm.cpp:271-291 does): the centre may move by less than
+        // a node per axis, as in a time step.
+        double xf[3];
+        HIP_TRY(hipMemcpyAsync(xf, c->st->xf, sizeof xf, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        const int Nd[3] = {c->NX, c->NY, c->NZ};
+        const bool per[3] = {true, !c->p.walls_y, !c->p.walls_z};
+        for (int d = 0; d < 3; ++d) {
+            double dx = s.X[d] - xf[d];
+            if (per[d]) dx -= Nd[d] * std::round(dx / Nd[d]);
+            if (!(std::fabs(dx) < 1.0))
+                return fail(LBM_HIP_EINVAL, "the squirmer centre may move by less than one node per axis from "
+                                            "where its flags were classified");
+        }
         c->link_blocks = lb;
     }
     HIP_TRY(hipMemcpyAsync(&c->st->sq, &s, sizeof s, hipMemcpyHostToDevice, c->stream));

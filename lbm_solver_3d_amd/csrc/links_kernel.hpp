@@ -40,13 +40,13 @@ struct LinkArgs {
 struct LinkBox {
     int x0, y0, z0, wx, wy, wz;
 };
-__device__ inline LinkBox link_box(const DevSquirmer& sq, int NX, int NY, int NZ) {
+__device__ inline LinkBox link_box(const double X[3], double R, int NX, int NY, int NZ) {
     LinkBox b;
     const int Nd[3] = {NX, NY, NZ};
     int lo[3], wd[3];
     for (int d = 0; d < 3; ++d) {
-        lo[d] = (int)floor(sq.X[d] - sq.R) - 1;
-        const int hi = (int)ceil(sq.X[d] + sq.R) + 1;
+        lo[d] = (int)floor(X[d] - R) - 1;
+        const int hi = (int)ceil(X[d] + R) + 1;
         int w = hi - lo[d] + 1;
         wd[d] = w > Nd[d] ? Nd[d] : w;
     }
@@ -105,7 +105,9 @@ __device__ __forceinline__ void links_block(const LinkArgs<S>& A, int block, int
     DevState* st = A.st;
     const DevSquirmer& sq = st->sq;
     const int NX = A.NX, NY = A.NY, NZ = A.NZ;
-    const LinkBox b = link_box(sq, NX, NY, NZ);
+    double cs[3];
+    const double rs = covering_sphere(st, NX, NY, NZ, cs);
+    const LinkBox b = link_box(cs, rs, NX, NY, NZ);
     const int M = b.wx * b.wy * b.wz;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int k = block * LINK_BLOCK + tid;

@@ -259,6 +259,29 @@ __device__ inline LinkBoxDims link_box_dims(const double X[3], double R, int NX,
 __device__ inline LinkBoxDims link_box_dims(const DevSquirmer& sq, int NX, int NY, int NZ) {
     return link_box_dims(sq.X, sq.R, NX, NY, NZ);
 }
+// A sphere (centre c, radius r) covering the squirmer at its centre X and
+// at the centre xf its flags were classified at: the two are the same after
+// every remap, and a host update moves X by less than a node.  The link and
+// boundary boxes are built around it, so they hold every link of the flags.
+__device__ inline double covering_sphere(const DevState* st, int NX, int NY, int NZ, double c[3]) {
+    const DevSquirmer& sq = st->sq;
+    const int Nd[3] = {NX, NY, NZ};
+    const double r0 = sq.R > st->rf ? sq.R : st->rf;
+    double r = r0;
+    for (int d = 0; d < 3; ++d) {
+        double dx = st->xf[d] - sq.X[d];
+        dx -= Nd[d] * round(dx / Nd[d]);  // nearest image (a walled axis never wraps this far)
+        c[d] = sq.X[d] + 0.5 * dx;
+        r = fmax(r, r0 + 0.5 * fabs(dx));
+    }
+    return r;
+}
+// Rows of the even-step boundary chunks near the squirmer.
+__device__ inline LinkBoxDims boundary_box_dims(const DevState* st, int NX, int NY, int NZ) {
+    double c[3];
+    const double r = covering_sphere(st, NX, NY, NZ, c);
+    return link_box_dims(c, r, NX, NY, NZ);
+}
 // Whether chunk c of row (y, z) lies in box b.
 __device__ inline bool in_link_box(const LinkBoxDims& b, int c, int y, int z, int nchx, int NY, int NZ) {
     int dy = y - wrap_coord(b.y0, NY); dy += dy < 0 ? NY : 0;
@@ -594,17 +617,12 @@ __device__ __forceinline__ void boundary_block(const StepArgs<S>& A, const FastC
         const uint32_t r = B.wall_rows[wave / B.nchx];
         y = (int)(r % NY); z = (int)(r / NY);
         chunk = wave % B.nchx;
-        // A wall-row chunk outside the squirmer's link box (around its
-        // centre and around the centre its PARTICLE flags were classified
-        // at) has the walls as its only solid neighbours: the same
-        // directions for every node.
+        // A wall-row chunk outside the box of the squirmer's links has the
+        // walls as its only solid neighbours: the same directions for every
+        // node.
         bool near_sphere = false;
-        if (A.has_squirmer) {
-            const DevState* st = A.st;
-            near_sphere = in_link_box(link_box_dims(st->sq, A.NX, A.NY, A.NZ), chunk, y, z, B.nchx, A.NY, A.NZ) ||
-                          in_link_box(link_box_dims(st->xf, st->rf, A.NX, A.NY, A.NZ), chunk, y, z, B.nchx, A.NY,
-                                      A.NZ);
-        }
+        if (A.has_squirmer)
+            near_sphere = in_link_box(boundary_box_dims(A.st, A.NX, A.NY, A.NZ), chunk, y, z, B.nchx, A.NY, A.NZ);
         if (!near_sphere) {
             uint32_t m = 0;
             for (int i = 1; i < Q; ++i) {
@@ -616,8 +634,7 @@ __device__ __forceinline__ void boundary_block(const StepArgs<S>& A, const FastC
         }
     } else if (A.has_squirmer) {
         const int w = wave - nwall;
-        const DevSquirmer& sq = A.st->sq;
-        const LinkBoxDims bd = link_box_dims(sq, A.NX, A.NY, A.NZ);
+        const LinkBoxDims bd = boundary_box_dims(A.st, A.NX, A.NY, A.NZ);
         const int row = w / B.box_chunks, k = w % B.box_chunks;
         if (row < bd.wy * bd.wz && k < bd.nck) {
             const int iy = row % bd.wy, iz = row / bd.wy;

@@ -203,3 +203,24 @@ def test_graph_replay_equals_eager(lbm, prec):
         sa, sb = A_.squirmer(), B_.squirmer()
         for k in ("X", "e", "U", "W"):
             assert np.array_equal(sa[k], sb[k]), k
+
+
+def test_host_nudged_squirmer_fp64(oracle_mod, lbm):
+    """The squirmer centre moved by the host between steps (by less than a
+    node, as lbm_hip_set_squirmer allows): the next step still sees the old
+    flags, whose links may lie outside the box of the new centre, and the
+    remap reclassifies around the new one, as the reference does when its
+    public member sq is changed."""
+    kv = dict(nx=40, ny=24, nz=22, walls_y=0, walls_z=1, squirmer=1, radius=4.5, b1=0.02, beta=-2.0, angle=10.0,
+              planar=1, plane_xz=1, z0=11.0)
+    o = oracle_mod.Oracle(**kv)
+    with lbm.Lattice(**kv, precision="fp64") as L:
+        _run_pair(o, L, lbm, 30)
+        X = np.array(o.state()["X"]) + np.array([0.85, 0.0, -0.7])
+        for d in range(3):
+            o.L.sq.X[d] = X[d]
+        L.set_squirmer(X=X)
+        with pytest.raises(lbm.LbmHipError):
+            L.set_squirmer(X=X + np.array([1.2, 0.0, 0.0]))
+        _run_pair(o, L, lbm, 40)
+        _compare_squirmer(o, L)
