@@ -332,7 +332,7 @@ def main():
                                                   for kk, vv in v.items() if kk not in ("U", "X", "tuning")}
         if v.get("tuning"):
             out["per_precision"][k]["tuning_ms_per_step"] = {
-                f"{c['tile_width']}{'x' if c['xcd_remap'] else ''}": round(c["ms_per_step"], 5)
+                f"{c['tile_width']}/order{c['xcd_remap']}": round(c["ms_per_step"], 5)
                 for c in v["tuning"]["candidates"]}
     print(json.dumps(out))
 

@@ -95,7 +95,7 @@ typedef struct lbm_hip_info {
     int64_t count_fluid;    /* FLUID nodes now (LBM::countFluid, lbm.cpp:342) */
     int row_pitch;          /* nodes per stored population row (nx, or nx rounded up to 64) */
     int tile_width;         /* collide-stream tile width in x (64 / 128 / 256) */
-    int xcd_remap;          /* 1: contiguous tile range per XCD */
+    int xcd_remap;          /* tile order over the XCDs (lbm_hip_set_tiling) */
 } lbm_hip_info;
 
 typedef struct lbm_hip_ctx lbm_hip_ctx;
@@ -179,15 +179,17 @@ int lbm_hip_time_run(lbm_hip_ctx* ctx, int steps, int mass_every, double* total_
 /* ---- tile-shape autotuning ----
  * No counterpart in the reference (its OpenMP loop has no launch shape).
  * Candidates: collide-stream tiles of 64 x 4, 128 x 2 and 256 x 1 nodes
- * (tile_width 64 / 128 / 256), each with and without the XCD-contiguous
- * tile order.  Each candidate runs `trial_steps` driver-loop steps (graph
+ * (tile_width 64 / 128 / 256), each in three orders of the tiles over the
+ * 8 XCDs (xcd_remap): 0 the hardware's round robin of consecutive tiles,
+ * 1 a contiguous range of tiles per XCD, 2 whole tile rows per XCD (the
+ * tiles that share lines at their x edges in one L2).  Each candidate runs `trial_steps` driver-loop steps (graph
  * replayed, after one untimed cycle) from the same saved state; the state is
  * restored afterwards, so tuning does not advance the simulation.  The
  * fastest candidate stays selected. */
-#define LBM_HIP_TUNE_CANDIDATES 6
+#define LBM_HIP_TUNE_CANDIDATES 9
 typedef struct lbm_hip_tuning {
     int tile_width;       /* selected: 64, 128 or 256 nodes along x */
-    int xcd_remap;        /* selected: 1 = contiguous tile range per XCD */
+    int xcd_remap;        /* selected tile order: 0, 1 or 2 (above) */
     double ms_per_step;   /* selected candidate's time per step */
     int cand_tile_width[LBM_HIP_TUNE_CANDIDATES];
     int cand_xcd_remap[LBM_HIP_TUNE_CANDIDATES];
@@ -196,7 +198,7 @@ typedef struct lbm_hip_tuning {
 
 int lbm_hip_autotune(lbm_hip_ctx* ctx, int trial_steps, int mass_every, lbm_hip_tuning* out);
 
-/* Select a tile shape directly (tile_width 64 / 128 / 256, xcd_remap 0 / 1). */
+/* Select a tile shape directly (tile_width 64 / 128 / 256, xcd_remap 0 / 1 / 2). */
 int lbm_hip_set_tiling(lbm_hip_ctx* ctx, int tile_width, int xcd_remap);
 
 /* ---- ensemble of lattices ----
@@ -236,6 +238,8 @@ int lbm_hip_batch_member(lbm_hip_batch* b, int i, lbm_hip_ctx** out);
 int lbm_hip_batch_info_get(lbm_hip_batch* b, lbm_hip_batch_info* out);
 /* Autotuning as lbm_hip_autotune, one tile shape for all members. */
 int lbm_hip_batch_autotune(lbm_hip_batch* b, int trial_steps, int mass_every, lbm_hip_tuning* out);
+/* One tile shape for all members, as lbm_hip_set_tiling. */
+int lbm_hip_batch_set_tiling(lbm_hip_batch* b, int tile_width, int xcd_remap);
 /* Bench support: time `steps` graph-replayed steps with HIP events. */
 int lbm_hip_batch_time_run(lbm_hip_batch* b, int steps, int mass_every, double* total_ms);
 

@@ -110,7 +110,7 @@ def exported_symbols() -> list[str]:
     return sorted(names)
 
 
-TUNE_CANDIDATES = 6
+TUNE_CANDIDATES = 9
 
 
 class Tuning(C.Structure):
@@ -172,6 +172,7 @@ def load_library(path: Path | None = None):
     lib.lbm_hip_batch_member.argtypes = [bt, C.c_int, C.POINTER(C.c_void_p)]
     lib.lbm_hip_batch_info_get.argtypes = [bt, C.POINTER(BatchInfo)]
     lib.lbm_hip_batch_autotune.argtypes = [bt, C.c_int, C.c_int, C.POINTER(Tuning)]
+    lib.lbm_hip_batch_set_tiling.argtypes = [bt, C.c_int, C.c_int]
     lib.lbm_hip_batch_time_run.argtypes = [bt, C.c_int, C.c_int, dp]
     fp = C.POINTER(C.c_float)
     mh = C.c_void_p
@@ -405,6 +406,9 @@ class Batch:
         t = Tuning()
         _check(self.lib, self.lib.lbm_hip_batch_autotune(self.h, trial_steps, mass_every, C.byref(t)))
         return t.as_dict()
+
+    def set_tiling(self, tile_width: int, xcd_remap: int = 1):
+        _check(self.lib, self.lib.lbm_hip_batch_set_tiling(self.h, tile_width, xcd_remap))
 
     def time_run(self, steps: int, mass_every: int = 10) -> float:
         tot = C.c_double()

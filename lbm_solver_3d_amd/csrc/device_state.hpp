@@ -107,7 +107,7 @@ struct StepArgs {
     int gx, gy, gz;       // grid in tiles
     int tx_shift;         // log2 of the tile width in x
     int ty;               // tile height in y
-    int xcd_remap;        // 1: give each XCD a contiguous range of tiles
+    int xcd_remap;        // 1: give each XCD a contiguous range of tiles; 2: whole tile rows per XCD
     double omPlus, omMinus, Fx, Fy, Fz;
     long long t;          // step index (selects double-buffer slots)
 };

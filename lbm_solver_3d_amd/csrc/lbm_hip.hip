@@ -954,8 +954,8 @@ int snapshot_restore(lbm_hip_ctx* c, const Snapshot& s, hipStream_t stream) {
 
 int tile_shift_of(int width) { return width == 64 ? 6 : (width == 128 ? 7 : (width == 256 ? 8 : -1)); }
 
-const int kTuneWidth[LBM_HIP_TUNE_CANDIDATES] = {64, 128, 256, 64, 128, 256};
-const int kTuneXcd[LBM_HIP_TUNE_CANDIDATES] = {1, 1, 1, 0, 0, 0};
+const int kTuneWidth[LBM_HIP_TUNE_CANDIDATES] = {64, 128, 256, 64, 128, 256, 64, 128, 256};
+const int kTuneXcd[LBM_HIP_TUNE_CANDIDATES] = {1, 1, 1, 0, 0, 0, 2, 2, 2};
 
 int graph_cycle_of(bool squirmer, int mass_every) {
     return (squirmer && mass_every > 0) ? ((mass_every % 2 == 0) ? mass_every : 2 * mass_every) : 2;
@@ -970,7 +970,8 @@ int lbm_hip_set_tiling(lbm_hip_ctx* c, int tile_width, int xcd_remap) {
     const int ts = tile_shift_of(tile_width);
     if (ts < 0) return fail(LBM_HIP_EINVAL, "tile_width must be 64, 128 or 256");
     set_tile_shape(c, ts);
-    c->xcd_remap = xcd_remap ? 1 : 0;
+    if (xcd_remap < 0 || xcd_remap > 2) return fail(LBM_HIP_EINVAL, "xcd_remap must be 0, 1 or 2");
+    c->xcd_remap = xcd_remap;
     c->mass_valid = false;  // partials of a pending mass step follow the old tiles
     return LBM_HIP_OK;
 }
@@ -1077,14 +1078,34 @@ void free_batch(lbm_hip_batch* b) {
 int batch_tables(lbm_hip_batch* b) {
     std::vector<uint32_t> tmap, bm, lm;
     std::vector<BatchFinalize> fin((size_t)b->count);
+    const bool row_order = b->mem[0]->xcd_remap == 2;
+    std::vector<uint32_t> queue[8];  // tile order 2: whole tile rows per XCD
+    size_t qlen[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int m = 0; m < b->count; ++m) {
         lbm_hip_ctx* c = b->mem[(size_t)m];
         const uint32_t tag = (uint32_t)m << 24;
-        for (int k = 0; k < c->G; ++k) tmap.push_back(tag | (uint32_t)k);
+        if (row_order) {
+            // each tile row to the XCD queue with the fewest tiles so far
+            for (int r = 0; r < c->G / c->gx; ++r) {
+                const int q = (int)(std::min_element(qlen, qlen + 8) - qlen);
+                for (int k = 0; k < c->gx; ++k) queue[q].push_back(tag | (uint32_t)(r * c->gx + k));
+                qlen[q] += (size_t)c->gx;
+            }
+        } else {
+            for (int k = 0; k < c->G; ++k) tmap.push_back(tag | (uint32_t)k);
+        }
         for (int k = 0; k < c->bblocks; ++k) bm.push_back(tag | (uint32_t)k);
         if (c->has_squirmer)
             for (int k = 0; k < c->link_blocks; ++k) lm.push_back(tag | (uint32_t)k);
         fin[(size_t)m] = BatchFinalize{c->mass_part, c->st, c->G, c->bblocks, c->has_squirmer ? 1 : 0};
+    }
+    if (row_order) {
+        // block i runs on XCD i mod 8: interleave the queues
+        for (size_t j = 0, left = 1; left; ++j) {
+            left = 0;
+            for (int q = 0; q < 8; ++q)
+                if (j < queue[q].size()) { tmap.push_back(queue[q][j]); ++left; }
+        }
     }
     b->Gtot = (int)tmap.size(); b->Btot = (int)bm.size(); b->Ltot = (int)lm.size();
     HIP_TRY(hipMemcpyAsync(b->tile_map, tmap.data(), tmap.size() * sizeof(uint32_t), hipMemcpyHostToDevice, b->stream));
@@ -1141,7 +1162,7 @@ int batch_step(lbm_hip_batch* b, int mass_every, int extra_flags) {
             if (e == hipSuccess) e = hipEventRecord(b->join, b->side);
         }
         if (e == hipSuccess)
-            e = launch_batch_step<S>(d, b->tile_map, b->Gtot, b->t, b->mem[0]->xcd_remap, odd, mode, b->stream);
+            e = launch_batch_step<S>(d, b->tile_map, b->Gtot, b->t, b->mem[0]->xcd_remap == 1, odd, mode, b->stream);
         if (e == hipSuccess && fork) e = hipStreamWaitEvent(b->stream, b->join, 0);
         if (e == hipSuccess && !odd && !fork) e = launch_batch_boundary<S>(d, b->bmap, b->Btot, b->t, mode, b->stream);
         if (e == hipSuccess && b->any_squirmer) e = launch_batch_links<S>(d, b->lmap, b->Ltot, b->stream);
@@ -1352,6 +1373,15 @@ int lbm_hip_batch_info_get(lbm_hip_batch* b, lbm_hip_batch_info* o) {
     o->xcd_remap = b->mem[0]->xcd_remap;
     o->device_bytes = b->device_bytes;
     return LBM_HIP_OK;
+}
+
+int lbm_hip_batch_set_tiling(lbm_hip_batch* b, int tile_width, int xcd_remap) {
+    if (!b) return fail(LBM_HIP_EINVAL, "null batch");
+    int rc = batch_check(b);
+    if (rc) return rc;
+    for (lbm_hip_ctx* c : b->mem)
+        if ((rc = lbm_hip_set_tiling(c, tile_width, xcd_remap))) return rc;
+    return batch_tables(b);
 }
 
 int lbm_hip_batch_autotune(lbm_hip_batch* b, int trial_steps, int mass_every, lbm_hip_tuning* out) {

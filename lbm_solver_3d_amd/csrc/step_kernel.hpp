@@ -553,6 +553,17 @@ __device__ __forceinline__ int xcd_tile(int lin, int G) {
     return xcd * q8 + (xcd < r8 ? xcd : r8) + (lin >> 3);
 }
 
+// The tiles of one tile row (gx tiles across x) on one XCD, consecutive tile
+// rows dealt to the XCDs round robin: the lines a row's tiles share at
+// their x edges (the odd step's shifted accesses) stay in one L2.  The
+// rows past the last multiple of 8 keep the identity order.
+__device__ __forceinline__ int xcd_tile_rows(int lin, int G, int gx) {
+    const int rows8 = (G / gx) & ~7;
+    if (lin >= rows8 * gx) return lin;
+    const int xcd = lin & 7, k = lin >> 3;
+    return ((k / gx) * 8 + xcd) * gx + k % gx;
+}
+
 // One tile (one block of BLOCK threads, 2^tx_shift x ty nodes of a z plane)
 // of the collide-stream update.
 template <class S, bool ODD, int MODE>
@@ -588,7 +599,8 @@ template <class S, bool ODD, int MODE>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(LBM_STEP_WAVES_PER_EU)))
 void aa_step_kernel(const StepArgs<S> A, const FastCoef K) {
     const int G = A.gx * A.gy * A.gz;
-    step_tile<S, ODD, MODE>(A, K, A.xcd_remap ? xcd_tile(blockIdx.x, G) : (int)blockIdx.x);
+    const int lin = (int)blockIdx.x;
+    step_tile<S, ODD, MODE>(A, K, A.xcd_remap == 1 ? xcd_tile(lin, G) : (A.xcd_remap == 2 ? xcd_tile_rows(lin, G, A.gx) : lin));
 }
 
 // Boundary chunks of an even step.  Work list: every 64-node chunk of the
