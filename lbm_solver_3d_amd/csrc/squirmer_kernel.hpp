@@ -336,7 +336,6 @@ __device__ void squirmer_block(const RemapArgs<S>& A) {
         for (int w = tid; w < nc * Q; w += SQ_THREADS) {
             const int kk = w / Q, i = w - kk * Q, k = c0 + kk;
             const uint32_t e = list[k];
-            const uint32_t n = e & 0x7FFFFFFFu;
             const bool cover = (e >> 31) != 0;
             const int p = lpos[k];
             const int ix = p % bwx, iy = (p / bwx) % bwy, iz = p / (bwx * bwy);

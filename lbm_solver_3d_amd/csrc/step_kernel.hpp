@@ -418,60 +418,17 @@ __device__ __forceinline__ void store_stream(S* p, S v) {
     else __builtin_nontemporal_store(v, p);
 }
 
-// Update one fluid node; returns its excess mass sum_i f*_i - 1 when MASS.
-// BPATH: even-step node of a boundary chunk (bounce-back redirections).
-// `row_solid` (BPATH): the solid directions, the same for every node of the
-// wavefront's row chunk (a row next to a wall, away from the squirmer), or
-// -1 to read them from the neighbours' flags.
-template <class S, bool ODD, int MODE, bool BPATH>
-__device__ __forceinline__ double node_update(const StepArgs<S>& A, const FastCoef& K, const RowGeom& rg, int x,
-                                              int y, int z, uint8_t fl, int row_solid = -1) {
+// Mass shift, collision and macroscopic outputs of one node whose 19
+// populations are in g (all update paths share it); returns the excess mass
+// sum_i f*_i - 1 when MASS.
+template <class S, int MODE>
+__device__ __forceinline__ double collide_node(const StepArgs<S>& A, const FastCoef& K, typename Store<S>::C (&g)[Q],
+                                               uint32_t n, int x, int y, int z) {
     using St = Store<S>;
     using T = typename St::C;
     constexpr bool EXACT = std::is_same<S, double>::value;
     constexpr bool MACRO = (MODE & STEP_MACRO) != 0;
     constexpr bool MASS = (MODE & STEP_MASS) != 0;
-    const uint32_t SS = A.SS, n = rg.n;
-    S* __restrict__ f = A.f;
-    auto base = [&](int s, uint32_t row) -> char* { return (char*)(f + ((size_t)s * SS + row)); };
-    auto at = [](char* b, uint32_t off) -> S* { return (S*)(b + off); };
-    const uint32_t* xb = rg.xb;
-    const uint32_t* yo = rg.pyo;  // population rows
-    const uint32_t* zo = rg.pzo;
-
-    // ---- gather the 19 populations node n collides ----
-    T g[Q];
-    static_for<0, Q>([&](auto I) {
-        constexpr int i = decltype(I)::value;
-        if constexpr (!ODD && !BPATH) g[i] = St::load(load_stream(at(base(i, yo[1] + zo[1]), xb[1])));
-        else if constexpr (!ODD) g[i] = St::load(*at(base(i, yo[1] + zo[1]), xb[1]));
-        else g[i] = St::load(*at(base(OPP[i], yo[1 - CY[i]] + zo[1 - CZ[i]]), xb[1 - CX[i]]));
-    });
-    // even step, boundary node: populations whose upstream node is solid
-    // come back from that node's slot opp(j) (bounce-back)
-    uint32_t solid = 0;
-    if constexpr (BPATH) {
-        if (row_solid >= 0) {  // wavefront-uniform: no flag loads, uniform branches
-            solid = (uint32_t)row_solid;
-            static_for<1, Q>([&](auto I) {
-                constexpr int j = decltype(I)::value;
-                if ((solid >> OPP[j]) & 1u)
-                    g[j] = St::load(*at(base(OPP[j], yo[1 - CY[j]] + zo[1 - CZ[j]]), xb[1 - CX[j]]));
-            });
-        } else if (fl & BOUNDARY) {
-            static_for<1, Q>([&](auto I) {
-                constexpr int i = decltype(I)::value;
-                const uint32_t m = rg.xo[1 + CX[i]] + rg.yo[1 + CY[i]] + rg.zo[1 + CZ[i]];
-                solid |= (uint32_t)((A.flag[m] & TYPE_MASK) != FLUID) << i;
-            });
-            static_for<1, Q>([&](auto I) {
-                constexpr int j = decltype(I)::value;
-                if ((solid >> OPP[j]) & 1u)
-                    g[j] = St::load(*at(base(OPP[j], yo[1 - CY[j]] + zo[1 - CZ[j]]), xb[1 - CX[j]]));
-            });
-        }
-    }
-
     // pending mass-correction shift (lbm.cpp:337), applied on load
     const double dsh = A.st->shift[A.t & 1];
     if (dsh != 0.0) {
@@ -525,6 +482,62 @@ __device__ __forceinline__ double node_update(const StepArgs<S>& A, const FastCo
             if (dx < b[3] && dy < b[4] && dz < b[5]) static_cast<T*>(A.rho)[n] = rho_t;
         }
     }
+
+    return excess;
+}
+
+// Update one fluid node; returns its excess mass sum_i f*_i - 1 when MASS.
+// BPATH: even-step node of a boundary chunk (bounce-back redirections).
+// `row_solid` (BPATH): the solid directions, the same for every node of the
+// wavefront's row chunk (a row next to a wall, away from the squirmer), or
+// -1 to read them from the neighbours' flags.
+template <class S, bool ODD, int MODE, bool BPATH>
+__device__ __forceinline__ double node_update(const StepArgs<S>& A, const FastCoef& K, const RowGeom& rg, int x,
+                                              int y, int z, uint8_t fl, int row_solid = -1) {
+    using St = Store<S>;
+    using T = typename St::C;
+    const uint32_t SS = A.SS, n = rg.n;
+    S* __restrict__ f = A.f;
+    auto base = [&](int s, uint32_t row) -> char* { return (char*)(f + ((size_t)s * SS + row)); };
+    auto at = [](char* b, uint32_t off) -> S* { return (S*)(b + off); };
+    const uint32_t* xb = rg.xb;
+    const uint32_t* yo = rg.pyo;  // population rows
+    const uint32_t* zo = rg.pzo;
+
+    // ---- gather the 19 populations node n collides ----
+    T g[Q];
+    static_for<0, Q>([&](auto I) {
+        constexpr int i = decltype(I)::value;
+        if constexpr (!ODD && !BPATH) g[i] = St::load(load_stream(at(base(i, yo[1] + zo[1]), xb[1])));
+        else if constexpr (!ODD) g[i] = St::load(*at(base(i, yo[1] + zo[1]), xb[1]));
+        else g[i] = St::load(*at(base(OPP[i], yo[1 - CY[i]] + zo[1 - CZ[i]]), xb[1 - CX[i]]));
+    });
+    // even step, boundary node: populations whose upstream node is solid
+    // come back from that node's slot opp(j) (bounce-back)
+    uint32_t solid = 0;
+    if constexpr (BPATH) {
+        if (row_solid >= 0) {  // wavefront-uniform: no flag loads, uniform branches
+            solid = (uint32_t)row_solid;
+            static_for<1, Q>([&](auto I) {
+                constexpr int j = decltype(I)::value;
+                if ((solid >> OPP[j]) & 1u)
+                    g[j] = St::load(*at(base(OPP[j], yo[1 - CY[j]] + zo[1 - CZ[j]]), xb[1 - CX[j]]));
+            });
+        } else if (fl & BOUNDARY) {
+            static_for<1, Q>([&](auto I) {
+                constexpr int i = decltype(I)::value;
+                const uint32_t m = rg.xo[1 + CX[i]] + rg.yo[1 + CY[i]] + rg.zo[1 + CZ[i]];
+                solid |= (uint32_t)((A.flag[m] & TYPE_MASK) != FLUID) << i;
+            });
+            static_for<1, Q>([&](auto I) {
+                constexpr int j = decltype(I)::value;
+                if ((solid >> OPP[j]) & 1u)
+                    g[j] = St::load(*at(base(OPP[j], yo[1 - CY[j]] + zo[1 - CZ[j]]), xb[1 - CX[j]]));
+            });
+        }
+    }
+
+    const double excess = collide_node<S, MODE>(A, K, g, n, x, y, z);
 
     // ---- stream ----
     static_for<0, Q>([&](auto I) {
@@ -600,7 +613,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(LBM_STEP_
 void aa_step_kernel(const StepArgs<S> A, const FastCoef K) {
     const int G = A.gx * A.gy * A.gz;
     const int lin = (int)blockIdx.x;
-    step_tile<S, ODD, MODE>(A, K, A.xcd_remap == 1 ? xcd_tile(lin, G) : (A.xcd_remap == 2 ? xcd_tile_rows(lin, G, A.gx) : lin));
+    step_tile<S, ODD, MODE>(A, K,
+                            A.xcd_remap == 1 ? xcd_tile(lin, G) : (A.xcd_remap == 2 ? xcd_tile_rows(lin, G, A.gx) : lin));
 }
 
 // Boundary chunks of an even step.  Work list: every 64-node chunk of the
