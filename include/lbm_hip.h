@@ -243,6 +243,15 @@ int lbm_hip_batch_set_tiling(lbm_hip_batch* b, int tile_width, int xcd_remap);
 /* Bench support: time `steps` graph-replayed steps with HIP events. */
 int lbm_hip_batch_time_run(lbm_hip_batch* b, int steps, int mass_every, double* total_ms);
 
+/* ---- bandwidth probe ----
+ * No counterpart in the reference: a measured ceiling for the roofline of
+ * the collide-stream kernel beside the published HBM peak.  Copies
+ * bytes / 2 bytes between two device buffers of `device` (16-byte
+ * nontemporal loads and stores) `reps` times after two untimed copies and
+ * reports the bandwidth in GB/s counting bytes read plus written.
+ * Allocates the two buffers and frees them before returning. */
+int lbm_hip_copy_bandwidth(int device, size_t bytes, int reps, double* gbs);
+
 /* Message of the last failed call on this thread ("" if none). */
 const char* lbm_hip_last_error(void);
 

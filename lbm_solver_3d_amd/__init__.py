@@ -174,6 +174,7 @@ def load_library(path: Path | None = None):
     lib.lbm_hip_batch_autotune.argtypes = [bt, C.c_int, C.c_int, C.POINTER(Tuning)]
     lib.lbm_hip_batch_set_tiling.argtypes = [bt, C.c_int, C.c_int]
     lib.lbm_hip_batch_time_run.argtypes = [bt, C.c_int, C.c_int, dp]
+    lib.lbm_hip_copy_bandwidth.argtypes = [C.c_int, C.c_size_t, C.c_int, dp]
     fp = C.POINTER(C.c_float)
     mh = C.c_void_p
     lib.lbm_mlp_create.argtypes = [C.c_int, C.c_int, C.c_ulonglong, C.c_int, C.POINTER(C.c_void_p)]
@@ -195,6 +196,15 @@ def load_library(path: Path | None = None):
 def _check(lib, rc: int):
     if rc != 0:
         raise LbmHipError(rc, lib.lbm_hip_last_error().decode())
+
+
+def copy_bandwidth(nbytes: int, reps: int = 20, device: int = 0) -> float:
+    """Device streaming-copy bandwidth in GB/s (bytes read + written) over two
+    buffers of nbytes / 2 bytes (lbm_hip_copy_bandwidth)."""
+    lib = load_library()
+    out = C.c_double()
+    _check(lib, lib.lbm_hip_copy_bandwidth(int(device), int(nbytes), int(reps), C.byref(out)))
+    return out.value
 
 
 def make_params(**kv) -> Params:

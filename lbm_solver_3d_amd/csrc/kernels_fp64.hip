@@ -330,4 +330,21 @@ hipError_t launch_macro_to_double(const void* src, int is_double, double* dst, u
     return hipGetLastError();
 }
 
+// Streaming copy of n16 16-byte words with nontemporal loads and stores
+// (the bandwidth probe).
+typedef uint32_t word4 __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void copy_probe_kernel(const word4* __restrict__ a, word4* __restrict__ b,
+                                                         size_t n16) {
+    const size_t stride = (size_t)gridDim.x * 256u;
+    for (size_t i = (size_t)blockIdx.x * 256u + threadIdx.x; i < n16; i += stride)
+        __builtin_nontemporal_store(__builtin_nontemporal_load(a + i), b + i);
+}
+
+hipError_t launch_copy_probe(const void* a, void* b, size_t n16, hipStream_t s) {
+    const size_t want = (n16 + 255) / 256;
+    const unsigned grid = (unsigned)(want < 65536 ? (want ? want : 1) : 65536);
+    hipLaunchKernelGGL(copy_probe_kernel, dim3(grid), dim3(256), 0, s, (const word4*)a, (word4*)b, n16);
+    return hipGetLastError();
+}
+
 }  // namespace lbmhip

@@ -70,6 +70,7 @@ hipError_t launch_diag(const uint8_t* flag, const void* rho, const void* ux, con
                        int is_double, uint32_t N, double* part, int nparts, double* out4, hipStream_t s);
 hipError_t launch_fill_macro(void* rho, void* ux, void* uy, void* uz, int is_double, uint32_t N, hipStream_t s);
 hipError_t launch_macro_to_double(const void* src, int is_double, double* dst, uint32_t N, hipStream_t s);
+hipError_t launch_copy_probe(const void* a, void* b, size_t n16, hipStream_t s);
 
 #define LBMHIP_DECLARE(S)                                                                                          \
     extern template hipError_t launch_step<S>(const StepArgs<S>&, const FastCoef&, bool, int, hipStream_t);     \

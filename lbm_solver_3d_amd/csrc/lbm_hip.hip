@@ -50,13 +50,6 @@ struct lbm_hip_ctx {
     hipEvent_t kev[2] = {nullptr, nullptr};
     int device = 0;
     hipStream_t stream = nullptr;
-    // even steps, when `concurrent_boundary`: the boundary-chunk kernel runs
-    // on `side`, concurrently with the bulk kernel on `stream` (they touch
-    // disjoint nodes), forked and joined by events.  Off by default: on one
-    // lattice the bulk kernel fills the GPU and the overlap gains nothing.
-    hipStream_t side = nullptr;
-    hipEvent_t fork = nullptr, join = nullptr;
-    bool concurrent_boundary = false;
     int NX = 0, NY = 0, NZ = 0;
     uint32_t N = 0;
     uint32_t SS = 0;  // slot stride in elements (PX * NY * NZ plus padding)
@@ -306,16 +299,12 @@ void free_ctx(lbm_hip_ctx* c) {
     if (!c) return;
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
-    if (c->side) hipStreamSynchronize(c->side);
     for (void* q : {c->f, (void*)c->flag, c->rho, c->ux, c->uy, c->uz, (void*)c->st, (void*)c->mass_part,
                     (void*)c->diag_part, (void*)c->diag_out, (void*)c->red_part, (void*)c->wall_rows})
         if (q) hipFree(q);
     for (hipEvent_t e : c->events) hipEventDestroy(e);
-    for (hipEvent_t e : {c->fork, c->join})
-        if (e) hipEventDestroy(e);
     if (c->graph) hipGraphExecDestroy(c->graph);
     if (c->stream) hipStreamDestroy(c->stream);
-    if (c->side) hipStreamDestroy(c->side);
     delete c;
 }
 
@@ -399,11 +388,7 @@ int lbm_hip_create(const lbm_hip_params* pin, lbm_hip_ctx** out) {
         return code;
     };
     if (hipSetDevice(c->device) != hipSuccess) return cleanup(fail(LBM_HIP_EDEVICE, "hipSetDevice failed"));
-    if (const char* v = std::getenv("LBM_HIP_CONCURRENT_BOUNDARY")) c->concurrent_boundary = std::atoi(v) != 0;
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c->join, hipEventDisableTiming) != hipSuccess)
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
         return cleanup(fail(LBM_HIP_EDEVICE, "hipStreamCreate failed"));
 
     const size_t N = c->N;
@@ -523,19 +508,10 @@ int lbm_hip_step(lbm_hip_ctx* c, int flags) {
     dispatch(c->prec, [&](auto* tag) -> int {
         using S = std::remove_pointer_t<decltype(tag)>;
         const StepArgs<S> sa = step_args<S>(c);
-        const bool fork = !odd && c->bblocks > 0 && c->concurrent_boundary;
-        if (fork) {
-            e = hipEventRecord(c->fork, c->stream);
-            if (e == hipSuccess) e = hipStreamWaitEvent(c->side, c->fork, 0);
-            if (e == hipSuccess) e = launch_boundary<S>(sa, c->K, c->bargs, c->bblocks, mode, c->side);
-            if (e == hipSuccess) e = hipEventRecord(c->join, c->side);
-        }
-        if (e == hipSuccess && c->kev[0]) e = hipEventRecord(c->kev[0], c->stream);
+        if (c->kev[0]) e = hipEventRecord(c->kev[0], c->stream);
         if (e == hipSuccess) e = launch_step<S>(sa, c->K, odd, mode, c->stream);
         if (e == hipSuccess && c->kev[1]) e = hipEventRecord(c->kev[1], c->stream);
-        if (e == hipSuccess && fork) e = hipStreamWaitEvent(c->stream, c->join, 0);
-        if (e == hipSuccess && !odd && !fork)
-            e = launch_boundary<S>(sa, c->K, c->bargs, c->bblocks, mode, c->stream);
+        if (e == hipSuccess && !odd) e = launch_boundary<S>(sa, c->K, c->bargs, c->bblocks, mode, c->stream);
         // moving-boundary bounce-back and force/torque on the squirmer
         if (e == hipSuccess && c->has_squirmer) e = launch_links<S>(link_args<S>(c), c->link_blocks, c->stream);
         if (e == hipSuccess && (flags & LBM_STEP_WALL_FORCE))
@@ -1046,9 +1022,6 @@ struct lbm_hip_batch {
     int graph_cycle = 0, graph_mass_every = -1;
     size_t device_bytes = 0;
     hipEvent_t ev[2] = {nullptr, nullptr};
-    hipStream_t side = nullptr;  // even-step boundary chunks, concurrent with the bulk launch
-    hipEvent_t fork = nullptr, join = nullptr;
-    bool concurrent_boundary = false;
 };
 
 namespace {
@@ -1059,11 +1032,9 @@ void free_batch(lbm_hip_batch* b) {
     if (b->stream) hipStreamSynchronize(b->stream);
     for (void* q : {b->desc, (void*)b->fin, (void*)b->tile_map, (void*)b->bmap, (void*)b->lmap})
         if (q) hipFree(q);
-    if (b->side) hipStreamSynchronize(b->side);
     if (b->graph) hipGraphExecDestroy(b->graph);
-    for (hipEvent_t e : {b->ev[0], b->ev[1], b->fork, b->join})
+    for (hipEvent_t e : {b->ev[0], b->ev[1]})
         if (e) hipEventDestroy(e);
-    if (b->side) hipStreamDestroy(b->side);
     for (lbm_hip_ctx* c : b->mem) {
         if (!c) continue;
         c->owned_by_batch = false;
@@ -1154,17 +1125,8 @@ int batch_step(lbm_hip_batch* b, int mass_every, int extra_flags) {
     dispatch(b->prec, [&](auto* tag) -> int {
         using S = std::remove_pointer_t<decltype(tag)>;
         const auto* d = static_cast<const BatchMember<S>*>(b->desc);
-        const bool fork = !odd && b->Btot > 0 && b->concurrent_boundary;
-        if (fork) {
-            e = hipEventRecord(b->fork, b->stream);
-            if (e == hipSuccess) e = hipStreamWaitEvent(b->side, b->fork, 0);
-            if (e == hipSuccess) e = launch_batch_boundary<S>(d, b->bmap, b->Btot, b->t, mode, b->side);
-            if (e == hipSuccess) e = hipEventRecord(b->join, b->side);
-        }
-        if (e == hipSuccess)
-            e = launch_batch_step<S>(d, b->tile_map, b->Gtot, b->t, b->mem[0]->xcd_remap == 1, odd, mode, b->stream);
-        if (e == hipSuccess && fork) e = hipStreamWaitEvent(b->stream, b->join, 0);
-        if (e == hipSuccess && !odd && !fork) e = launch_batch_boundary<S>(d, b->bmap, b->Btot, b->t, mode, b->stream);
+        e = launch_batch_step<S>(d, b->tile_map, b->Gtot, b->t, b->mem[0]->xcd_remap == 1, odd, mode, b->stream);
+        if (e == hipSuccess && !odd) e = launch_batch_boundary<S>(d, b->bmap, b->Btot, b->t, mode, b->stream);
         if (e == hipSuccess && b->any_squirmer) e = launch_batch_links<S>(d, b->lmap, b->Ltot, b->stream);
         if (e == hipSuccess && (extra_flags & LBM_STEP_WALL_FORCE))
             for (lbm_hip_ctx* c : b->mem)
@@ -1303,11 +1265,7 @@ int lbm_hip_batch_create(const lbm_hip_params* members, int count, lbm_hip_batch
             return bail(fail(LBM_HIP_EINVAL, "member " + std::to_string(m) + " too large for a batch"));
     }
     if (hipSetDevice(b->device) != hipSuccess) return bail(fail(LBM_HIP_EDEVICE, "hipSetDevice failed"));
-    if (const char* v = std::getenv("LBM_HIP_CONCURRENT_BOUNDARY")) b->concurrent_boundary = std::atoi(v) != 0;
-    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&b->join, hipEventDisableTiming) != hipSuccess)
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(LBM_HIP_EDEVICE, "hipStreamCreate failed"));
     for (hipEvent_t& e : b->ev)
         if (hipEventCreate(&e) != hipSuccess) return bail(fail(LBM_HIP_EDEVICE, "hipEventCreate failed"));
@@ -1457,6 +1415,38 @@ int lbm_hip_batch_time_run(lbm_hip_batch* b, int steps, int mass_every, double* 
     HIP_TRY(hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
     if (total_ms) *total_ms = ms;
     return batch_check(b);
+}
+
+int lbm_hip_copy_bandwidth(int device, size_t bytes, int reps, double* gbs) {
+    if (!gbs || reps <= 0 || bytes < 512) return fail(LBM_HIP_EINVAL, "copy_bandwidth: bytes >= 512, reps > 0, gbs");
+    const size_t n16 = bytes / 2 / 16;
+    HIP_TRY(hipSetDevice(device));
+    void* a = nullptr;
+    void* b = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    float ms = 0.f;
+    hipError_t e = hipMalloc(&a, n16 * 16);
+    if (e == hipSuccess) e = hipMalloc(&b, n16 * 16);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&ev[0]);
+    if (e == hipSuccess) e = hipEventCreate(&ev[1]);
+    if (e == hipSuccess) e = hipMemsetAsync(a, 0, n16 * 16, s);
+    for (int r = 0; r < 2 && e == hipSuccess; ++r) e = launch_copy_probe(a, b, n16, s);
+    if (e == hipSuccess) e = hipEventRecord(ev[0], s);
+    for (int r = 0; r < reps && e == hipSuccess; ++r) e = launch_copy_probe(a, b, n16, s);
+    if (e == hipSuccess) e = hipEventRecord(ev[1], s);
+    if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev[0], ev[1]);
+    if (s) hipStreamSynchronize(s);
+    for (hipEvent_t v : ev)
+        if (v) hipEventDestroy(v);
+    if (s) hipStreamDestroy(s);
+    if (a) hipFree(a);
+    if (b) hipFree(b);
+    if (e != hipSuccess) return fail(LBM_HIP_EDEVICE, std::string("copy_bandwidth: ") + hipGetErrorString(e));
+    *gbs = 2.0 * (double)(n16 * 16) * reps / (ms * 1e-3) / 1e9;
+    return LBM_HIP_OK;
 }
 
 }  // extern "C"

@@ -76,3 +76,11 @@ def test_ensemble_members_are_deterministic_and_valid(lbm):
         assert -5 <= m["beta"] <= 5 and 0 <= m["angle"] < 45
         assert 2 * R <= m["z0"] <= H - 2 * R
         lbm.validate(**m)
+
+
+def test_copy_bandwidth_rejects_bad_arguments(lbm):
+    lib = lbm.load_library()
+    out = C.c_double()
+    assert lib.lbm_hip_copy_bandwidth(0, 100, 1, C.byref(out)) == -1
+    assert lib.lbm_hip_copy_bandwidth(0, 1 << 20, 0, C.byref(out)) == -1
+    assert lib.lbm_hip_copy_bandwidth(0, 1 << 20, 1, None) == -1
