@@ -6,7 +6,8 @@ Restated from the mathematics the reference uses:
 * plane Poiseuille profile (analytic.cpp:23-25);
 * relative L2 error of u_x on the cross-section x = 0 (analytic.cpp:27-46);
 * Blake's squirmer flow field (postprocessing/lbm_post.py:153-170, Blake 1971)
-  and the near-field error metric of lbm_post.py:350-361.
+  and the near-field error metric of lbm_post.py:350-361;
+* lateral extrema of a slit-channel trajectory (lbm_post.py:224-238).
 """
 from __future__ import annotations
 
@@ -121,3 +122,23 @@ def read_vtk(path):
             out["velocity"] = np.frombuffer(data, ">f8", 3 * n, pos).astype(float).reshape(n, 3)
             pos += 24 * n + 1
     return out
+
+
+def oscillation_extrema(t, Y, ny, R, U0, t_skip=0.0, prominence=0.02):
+    """Lateral extrema of a swimmer in a slit channel with walls in y
+    (postprocessing/lbm_post.py:224-238).  The channel width is H = ny - 2
+    (wall nodes at y = 0 and ny - 1), its centre line y = (ny - 1) / 2; time
+    is scaled as T = t U0 / H and the lateral position as
+    yn = (Y - centre) / (H / 2 - R), so |yn| = 1 is a sphere touching a wall.
+    Returns (T, yn, idx), idx the sorted indices of the local maxima and
+    minima of yn with at least the given prominence (scipy find_peaks) and
+    T > t_skip."""
+    from scipy.signal import find_peaks
+    t, Y = np.asarray(t, float), np.asarray(Y, float)
+    H = ny - 2
+    T = t * U0 / H
+    yn = (Y - 0.5 * (ny - 1)) / (0.5 * H - R)
+    hi, _ = find_peaks(yn, prominence=prominence)
+    lo, _ = find_peaks(-yn, prominence=prominence)
+    idx = np.sort(np.concatenate([hi, lo]))
+    return T, yn, idx[T[idx] > t_skip]
