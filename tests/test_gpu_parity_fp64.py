@@ -116,6 +116,22 @@ def test_channel_squirmer_near_wall_fp64(oracle_mod, lbm):
         _compare_squirmer(o, L)
 
 
+@pytest.mark.parametrize("kv,steps", [
+    # walls in y and z, odd NX: the link, boundary and remap boxes wrap around x = 0
+    (dict(nx=37, ny=23, nz=21, walls_y=1, walls_z=1, squirmer=1, radius=4.2, b1=0.03, beta=-1.5, angle=-25,
+          planar=1, x0=1.3, hmin=0.5), 220),
+    # fully periodic, the sphere over the corner of the box: every box wraps in x, y and z
+    (dict(nx=30, ny=26, nz=28, walls_y=0, walls_z=0, squirmer=1, radius=4.0, b1=0.02, beta=2.0, angle=30,
+          planar=0, x0=0.4, y0=25.7, z0=0.6), 180),
+])
+def test_squirmer_across_periodic_boundaries_fp64(oracle_mod, lbm, kv, steps):
+    o = oracle_mod.Oracle(**kv)
+    with lbm.Lattice(**kv, precision="fp64") as L:
+        assert np.array_equal(L.flags(), o.flags())
+        _run_pair(o, L, lbm, steps)
+        _compare_squirmer(o, L)
+
+
 def test_channel_squirmer_padded_rows_fp64(oracle_mod, lbm):
     """Squirmer in a channel whose 100-node rows are stored with a pitch of
     128: links, remap and mass correction address the padded rows."""
