@@ -104,7 +104,9 @@ typedef struct lbm_hip_ctx lbm_hip_ctx;
 void lbm_hip_default_params(lbm_hip_params* p);
 
 /* Params::validate (params.cpp:120-140): 0 if valid, else LBM_HIP_EINVAL with
- * the reference's message in lbm_hip_last_error(). */
+ * the reference's message in lbm_hip_last_error().  Beyond the reference's
+ * rules: nx * ny * nz < 2^32 - 65536 (32-bit node indices; population
+ * offsets are 64-bit), and a known precision. */
 int lbm_hip_validate(const lbm_hip_params* p);
 
 /* LBM::LBM(const Params&) (lbm.cpp:60-94): allocate, set wall flags, f = w_i,

@@ -322,13 +322,15 @@ class Lattice:
         _check(self.lib, self.lib.lbm_hip_download(self.h, _dp(f), None, None, None, None, None))
         return f.reshape(Q, self.N)
 
-    def fields(self) -> dict:
+    def fields(self, names=("rho", "ux", "uy", "uz", "flag")) -> dict:
+        """Moments of the last step that wrote them (as doubles) and the flags;
+        only the named ones are transferred."""
         N = self.N
-        rho, ux, uy, uz = (np.empty(N) for _ in range(4))
-        flag = np.empty(N, np.uint8)
-        _check(self.lib, self.lib.lbm_hip_download(self.h, None, _dp(rho), _dp(ux), _dp(uy), _dp(uz),
-                                                   flag.ctypes.data_as(C.POINTER(C.c_uint8))))
-        return dict(rho=rho, ux=ux, uy=uy, uz=uz, flag=flag)
+        out = {k: (np.empty(N, np.uint8) if k == "flag" else np.empty(N)) for k in names}
+        g = lambda k: _dp(out[k]) if k in out else None
+        fl = out["flag"].ctypes.data_as(C.POINTER(C.c_uint8)) if "flag" in out else None
+        _check(self.lib, self.lib.lbm_hip_download(self.h, None, g("rho"), g("ux"), g("uy"), g("uz"), fl))
+        return out
 
     def flags(self) -> np.ndarray:
         flag = np.empty(self.N, np.uint8)

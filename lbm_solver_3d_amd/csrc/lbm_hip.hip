@@ -117,7 +117,7 @@ void lbm_hip_default_params(lbm_hip_params* p) {
 }
 
 // Params::validate, params.cpp:120-140 (same rules and messages), plus the
-// 32-bit indexing limit of this implementation and a precision check.
+// 32-bit node indexing limit of this implementation and a precision check.
 int lbm_hip_validate(const lbm_hip_params* p) {
     if (!p) return fail(LBM_HIP_EINVAL, "null parameters");
     auto bad = [](const char* m) { return fail(LBM_HIP_EINVAL, m); };
@@ -127,8 +127,10 @@ int lbm_hip_validate(const lbm_hip_params* p) {
     if (p->tau <= 0.5) return bad("tau must be > 0.5");
     if (p->lambda <= 0.0) return bad("lambda must be > 0");
     if (p->precision < LBM_HIP_FP64 || p->precision > LBM_HIP_BF16) return bad("unknown precision");
+    // node and in-slot indices are 32-bit (population i of a node sits at
+    // i * SS + that index, formed in 64 bits)
     const double N = (double)p->nx * p->ny * p->nz + 65536.0;
-    if (N * 19.0 >= 4294967296.0) return bad("grid too large for 32-bit population indexing (19*N must be < 2^32)");
+    if (N >= 4294967296.0) return bad("grid too large for 32-bit node indexing (nx*ny*nz must be < 2^32 - 65536)");
     if (!p->squirmer) return LBM_HIP_OK;
     if (p->radius <= 1.0) return bad("squirmer radius must be > 1 lattice unit");
     if (std::fabs(p->angle) >= 45.0) return bad("initial angle must satisfy |angle| < 45 deg from the x-axis");
@@ -260,7 +262,7 @@ uint32_t row_pitch(int nx, int ny, int nz) {
     if (const char* e = std::getenv("LBM_HIP_ROW_PAD"))
         if (std::atoi(e) == 0) return (uint32_t)nx;
     if (px == (uint32_t)nx || 2 * px > 3 * (uint32_t)nx) return (uint32_t)nx;
-    if (19.0 * ((double)px * ny * nz + 65536.0) >= 4294967296.0) return (uint32_t)nx;
+    if ((double)px * ny * nz + 65536.0 >= 4294967296.0) return (uint32_t)nx;
     return px;
 }
 
@@ -352,7 +354,7 @@ int lbm_hip_create(const lbm_hip_params* pin, lbm_hip_ctx** out) {
     // tiles: one wavefront row-chunk along x, block of 256 threads
     for (int ts = 6; ts <= 8; ++ts) c->G_max = std::max(c->G_max, tiles_for(pin->nx, pin->ny, pin->nz, ts));
     set_tile_shape(c, pin->nx <= 64 ? 6 : (pin->nx <= 128 ? 7 : 8));
-    c->nb256 = (int)((c->N + 255) / 256);
+    c->nb256 = (int)(((size_t)c->N + 255) / 256);
 
     // even-step boundary chunks: rows next to a wall, and the squirmer's box rows
     std::vector<uint32_t> wall_rows;

@@ -38,6 +38,7 @@ def test_default_params_match_reference(lbm):
     (dict(squirmer=1, y0=4.0), "overlaps (or is too close to) a y-wall"),
     (dict(squirmer=1, nx=9), "domain too small for the squirmer"),
     (dict(squirmer=1, b1=0.09), "swimming speed U0"),
+    (dict(nx=2048, ny=2048, nz=1024), "grid too large for 32-bit node indexing"),
 ])
 def test_validate_rejects_like_reference(lbm, kv, msg):
     lib = lbm.load_library()
@@ -53,7 +54,8 @@ def test_validate_accepts_reference_decks(lbm):
     lib = lbm.load_library()
     for kv in [dict(nx=4, ny=34, nz=34, tau=0.8, fx=5e-5, walls_y=1, walls_z=1),
                dict(nx=48, ny=48, nz=48, walls_y=0, squirmer=1, angle=0),
-               dict(nx=64, ny=34, nz=32, squirmer=1, angle=30)]:
+               dict(nx=64, ny=34, nz=32, squirmer=1, angle=30),
+               dict(nx=2048, ny=512, nz=256, walls_y=1, walls_z=1)]:  # 19 N > 2^32: 64-bit slot offsets
         p = lbm.make_params(**kv)
         assert lib.lbm_hip_validate(C.byref(p)) == 0, lib.lbm_hip_last_error()
 
