@@ -25,7 +25,7 @@ double ductVelocity(double y, double z, double a, double b, double G, double mu)
 
 double planeVelocity(double y, double a, double G, double mu) { return G / (2.0 * mu) * (a * a - y * y); }
 
-double poiseuilleError(LBM& L, const std::string& base) {
+double poiseuilleError(const LBM& L, const std::string& base) {
     if (!L.wallsY && !L.wallsZ) return -1.0;
     L.syncHost(false);
     const double mu = L.nu, G = L.F[0];

@@ -13,6 +13,6 @@ double ductVelocity(double y, double z, double a, double b, double G, double mu)
 double planeVelocity(double y, double a, double G, double mu);
 // Relative L2 error vs the exact solution; writes <base>_profile_{y,z}.dat.
 // Calls L.syncHost(false) first.  Negative if there are no walls.
-double poiseuilleError(LBM& L, const std::string& base);
+double poiseuilleError(const LBM& L, const std::string& base);
 
 }  // namespace lbm

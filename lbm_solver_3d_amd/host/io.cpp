@@ -28,7 +28,7 @@ void ensureDirectory(const std::string& dir) {
     if (ec) throw std::runtime_error("cannot create output directory '" + dir + "': " + ec.message());
 }
 
-void writeVTK(LBM& L, const std::string& path) {
+void writeVTK(const LBM& L, const std::string& path) {
     L.syncHost(false);
     std::FILE* fp = std::fopen(path.c_str(), "wb");
     if (!fp) throw std::runtime_error("cannot write '" + path + "'");

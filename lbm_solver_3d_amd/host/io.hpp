@@ -11,7 +11,7 @@ namespace lbm {
 
 // STRUCTURED_POINTS, binary big-endian: flag (uint8), density (float64),
 // velocity (3 x float64).  Calls L.syncHost(false) first.
-void writeVTK(LBM& L, const std::string& path);
+void writeVTK(const LBM& L, const std::string& path);
 
 // One row every traj_every steps:  t X Y Z ex ey ez Ux Uy Uz Wx Wy Wz Fwall_y Fwall_z
 class TrajectoryWriter {

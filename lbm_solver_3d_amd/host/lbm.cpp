@@ -103,7 +103,7 @@ void LBM::pushSquirmer() {
     check(lbm_hip_set_squirmer(ctx_, &s));
 }
 
-void LBM::syncHost(bool populations) {
+void LBM::syncHost(bool populations) const {
     rho.resize(N); ux.resize(N); uy.resize(N); uz.resize(N); flag.resize(N);
     if (populations) f.resize((size_t)d3q19::Q * N);
     check(lbm_hip_download(ctx_, populations ? f.data() : nullptr, rho.data(), ux.data(), uy.data(), uz.data(),

@@ -57,9 +57,11 @@ public:
     double F[3];
     bool wallsY, wallsZ;
 
-    std::vector<double> f, fnew;
-    std::vector<double> rho, ux, uy, uz;
-    std::vector<uint8_t> flag;
+    // mutable: they cache device data, refreshed by syncHost() also through
+    // a const LBM& (writeVTK and poiseuilleError take one, as in the reference)
+    mutable std::vector<double> f, fnew;
+    mutable std::vector<double> rho, ux, uy, uz;
+    mutable std::vector<uint8_t> flag;
 
     double wallForce[3] = {0, 0, 0};
 
@@ -68,7 +70,7 @@ public:
 
     // ---- this build ----
     // Refresh the host mirrors f, rho, ux, uy, uz, flag (fnew mirrors f).
-    void syncHost(bool populations = true);
+    void syncHost(bool populations = true) const;
     // Upload host-side changes of sq to the device.
     void pushSquirmer();
     // Upload f (reference layout) to the device.

@@ -15,13 +15,19 @@ import analytic as A
 
 ROOT = Path(__file__).resolve().parents[1]
 DRIVER = ROOT / "lbm_solver_3d_amd" / "bin" / "lbm_squirmer"
+# the reference's own src/main.cpp, unchanged, compiled against the C++ facade
+# (oracle/Makefile; built only where the reference tree is present)
+REF_MAIN = ROOT / "oracle" / "_ref" / "lbm_squirmer_facade"
 REFOUT = ROOT / "tests" / "golden" / "reference_outputs"
+DRIVERS = [pytest.param(DRIVER, id="driver"), pytest.param(REF_MAIN, id="reference-main-on-facade")]
 
 
-def run(args, cwd=None, timeout=600):
-    if not DRIVER.exists():
+def run(args, cwd=None, timeout=600, driver=DRIVER):
+    if not driver.exists():
+        if driver == REF_MAIN:
+            pytest.skip("the reference's main.cpp was not built against the facade (reference tree absent)")
         pytest.skip("driver not built")
-    return subprocess.run([str(DRIVER), *map(str, args)], capture_output=True, text=True, cwd=cwd, timeout=timeout)
+    return subprocess.run([str(driver), *map(str, args)], capture_output=True, text=True, cwd=cwd, timeout=timeout)
 
 
 def read_kv(path):
@@ -47,26 +53,30 @@ def read_kv(path):
     ([REFOUT / "channel_neutral.in", "b1=0.1"], "swimming speed U0 = 2 B1 / 3 too large"),
     ([REFOUT / "duct.in", "print_every=0"], "error: print_every and traj_every must be >= 1"),
 ])
-def test_driver_rejects_like_reference(args, msg, tmp_path):
-    r = run(args, cwd=tmp_path)
+@pytest.mark.parametrize("driver", DRIVERS)
+def test_driver_rejects_like_reference(args, msg, tmp_path, driver):
+    r = run(args, cwd=tmp_path, driver=driver)
     assert r.returncode == 1
     assert msg in r.stderr
 
 
-def test_driver_bad_deck_line(tmp_path):
+@pytest.mark.parametrize("driver", DRIVERS)
+def test_driver_bad_deck_line(tmp_path, driver):
     deck = tmp_path / "bad.in"
     deck.write_text("nx = 8\nthis line has no equals sign\n")
-    r = run([deck], cwd=tmp_path)
+    r = run([deck], cwd=tmp_path, driver=driver)
     assert r.returncode == 1
     assert f"error: {deck}:2: expected 'key = value'" in r.stderr
 
 
 # ------------------------------------------------------------------ GPU
 @pytest.mark.gpu
-def test_duct_deck_matches_reference_outputs(tmp_path):
+@pytest.mark.parametrize("driver", DRIVERS)
+def test_duct_deck_matches_reference_outputs(tmp_path, driver):
     """inputs/duct.in through the driver (FP64 storage): outputs/duct_summary.txt,
-    duct_profile_{y,z}.dat and duct_final.vtk."""
-    r = run([REFOUT / "duct.in", f"output_dir={tmp_path}", "precision=fp64"])
+    duct_profile_{y,z}.dat and duct_final.vtk.  Also through the reference's
+    own main.cpp compiled against the facade."""
+    r = run([REFOUT / "duct.in", f"output_dir={tmp_path}", "precision=fp64"], driver=driver)
     assert r.returncode == 0, r.stderr
     s = read_kv(tmp_path / "duct_summary.txt")
     ref = read_kv(REFOUT / "duct_summary.txt")
@@ -91,11 +101,12 @@ def test_duct_deck_matches_reference_outputs(tmp_path):
 
 
 @pytest.mark.gpu
-def test_puller_deck_trajectory_rows(tmp_path):
+@pytest.mark.parametrize("driver", DRIVERS)
+def test_puller_deck_trajectory_rows(tmp_path, driver):
     """inputs/channel_puller.in for 1000 steps: trajectory rows equal the first
     rows of outputs/channel_puller_traj.dat to the printed digits."""
     r = run([REFOUT / "channel_puller.in", f"output_dir={tmp_path}", "steps=1000", "print_every=500",
-             "precision=fp64"])
+             "precision=fp64"], driver=driver)
     assert r.returncode == 0, r.stderr
     mine = np.loadtxt(tmp_path / "channel_puller_traj.dat")
     ref = np.loadtxt(REFOUT / "channel_puller_traj.dat")[:len(mine)]
