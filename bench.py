@@ -229,7 +229,7 @@ def main():
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; the multi-GPU x-slab decomposition is a design "
-                                   "only (see DESIGN.md), so --gpus must be 1"}))
+                                   "only, so --gpus must be 1"}))
         sys.exit(2)
 
     import lbm_solver_3d_amd as lbm

@@ -6,7 +6,7 @@
  * No counterpart in the reference: it stands in for the fluid solve of
  * LBM::step / advanceSquirmer (src/utils/lbm.cpp:128-263) by predicting the
  * squirmer's translational and angular velocity from its configuration; the
- * full lattice-Boltzmann step stays the fallback and the judge of its error.
+ * full lattice-Boltzmann step stays the fallback and the measure of its error.
  *
  * Parameters of one net, float, in this order: W1[128][5], b1[128],
  * W2[128][128], b2[128], W3[128][128], b3[128], W4[6][128], b4[6]

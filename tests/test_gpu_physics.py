@@ -1,5 +1,6 @@
-"""The reference's own acceptance tests and the BASELINE validation cases,
-run on the GPU path in every storage precision.
+"""The reference's own acceptance tests and the validation workloads (duct
+64x32x32, bulk 128^3 squirmer), run on the GPU path in every storage
+precision.
 
 * duct flow: L2 error against the series solution, second-order convergence,
   wall force = body force, mass (reference tests/test_duct_flow.py);
@@ -98,7 +99,7 @@ def test_plane_channel_exact(lbm, prec):
     assert err < (1e-6 if prec == "fp64" else 1e-4), err
 
 
-def test_c1_duct_second_order_fp64(lbm):
+def test_duct_64x32x32_second_order_fp64(lbm):
     """64x32x32-class duct, tau = 1, Fx = 1e-6, f = feq, 20000 steps; cross-sections 16/32/64."""
     errs = {}
     for H in (16, 32, 64):
@@ -113,8 +114,8 @@ def test_c1_duct_second_order_fp64(lbm):
     assert errs[64] < 1e-4
 
 
-def test_c1_duct_64x32x32_precisions(lbm):
-    """The BASELINE C1 grid itself: FP32 / FP16 / BF16 against FP64 after 20000 steps."""
+def test_duct_64x32x32_precisions(lbm):
+    """The 64x32x32 duct (tau = 1, Fx = 1e-6) itself: FP32 / FP16 / BF16 against FP64 after 20000 steps."""
     kv = dict(nx=64, ny=32, nz=32, tau=1.0, fx=1e-6, walls_y=1, walls_z=1)
     ux = {}
     for prec in PRECS:
@@ -228,7 +229,7 @@ def test_bulk_near_field_blake(bulk_runs, prec):
         assert e == pytest.approx({-3.0: 0.21, 0.0: 0.16, 3.0: 0.22}[beta], abs=0.02), (beta, e)
 
 
-def test_c2_bulk_128_speed(lbm):
+def test_bulk_128_squirmer_speed(lbm):
     """128^3 periodic box, R = 6, B1 = 1.5e-3 (U0 = 1e-3), beta = 0, 5000 steps:
     FP32 / FP16 speed within 0.5 % of FP64; FP64 near Blake's 2 B1 / 3 up to
     the finite-box correction."""
@@ -240,14 +241,14 @@ def test_c2_bulk_128_speed(lbm):
             tr = swim(L, 5000, every=50, lbm=lbm)
         u[prec] = tr[tr[:, 0] > 2500, 1].mean()
     u0 = 2 * 1.5e-3 / 3
-    print("C2 speeds / U0:", {k: v / u0 for k, v in u.items()})
+    print("bulk 128^3 speeds / U0:", {k: v / u0 for k, v in u.items()})
     assert 0.93 < u["fp64"] / u0 < 1.03, u["fp64"] / u0
     assert abs(u["fp32"] / u["fp64"] - 1) < 1e-4
     assert abs(u["fp16"] / u["fp64"] - 1) < 5e-3
     assert abs(u["bf16"] / u["fp64"] - 1) < 5e-2
 
 
-def test_c3_storage_precisions_against_fp64_trajectory():
+def test_slit_512x256x128_storage_precisions_against_fp64():
     """The 512x256x128 pusher (the bench workload) for 4000 steps in FP32,
     FP16 and BF16 storage against the FP64 run: squirmer velocity within 1 %
     (FP32, FP16; BF16 within 5 %), measured FP16 ~1e-4."""
