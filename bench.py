@@ -338,7 +338,7 @@ def main():
                                                   for kk, vv in v.items() if kk not in ("U", "X", "tuning")}
         if v.get("tuning"):
             out["per_precision"][k]["tuning_ms_per_step"] = {
-                f"{c['tile_width']}/order{c['xcd_remap']}": round(c["ms_per_step"], 5)
+                f"{c['tile_width'] or 'run'}/order{c['xcd_remap']}": round(c["ms_per_step"], 5)
                 for c in v["tuning"]["candidates"]}
     print(json.dumps(out))
 

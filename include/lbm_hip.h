@@ -94,7 +94,7 @@ typedef struct lbm_hip_info {
     int has_squirmer, walls_y, walls_z;
     int64_t count_fluid;    /* FLUID nodes now (LBM::countFluid, lbm.cpp:342) */
     int row_pitch;          /* nodes per stored population row (nx, or nx rounded up to 64) */
-    int tile_width;         /* collide-stream tile width in x (64 / 128 / 256) */
+    int tile_width;         /* collide-stream tile width in x (64 / 128 / 256; 0: row runs) */
     int xcd_remap;          /* tile order over the XCDs (lbm_hip_set_tiling) */
 } lbm_hip_info;
 
@@ -184,13 +184,16 @@ int lbm_hip_time_run(lbm_hip_ctx* ctx, int steps, int mass_every, double* total_
  * (tile_width 64 / 128 / 256), each in three orders of the tiles over the
  * 8 XCDs (xcd_remap): 0 the hardware's round robin of consecutive tiles,
  * 1 a contiguous range of tiles per XCD, 2 whole tile rows per XCD (the
- * tiles that share lines at their x edges in one L2).  Each candidate runs `trial_steps` driver-loop steps (graph
- * replayed, after one untimed cycle) from the same saved state; the state is
- * restored afterwards, so tuning does not advance the simulation.  The
- * fastest candidate stays selected. */
-#define LBM_HIP_TUNE_CANDIDATES 9
+ * tiles that share lines at their x edges in one L2); and row runs
+ * (tile_width 0: four consecutive 64-node chunks of the rows in row-major
+ * order, so rows narrower than a tile leave no lanes idle) in orders 1 and 0.
+ * Each candidate runs `trial_steps` driver-loop steps (graph replayed, after
+ * one untimed cycle) from the same saved state; the state is restored
+ * afterwards, so tuning does not advance the simulation.  The fastest
+ * candidate stays selected. */
+#define LBM_HIP_TUNE_CANDIDATES 11
 typedef struct lbm_hip_tuning {
-    int tile_width;       /* selected: 64, 128 or 256 nodes along x */
+    int tile_width;       /* selected: 64, 128 or 256 nodes along x, or 0 (row runs) */
     int xcd_remap;        /* selected tile order: 0, 1 or 2 (above) */
     double ms_per_step;   /* selected candidate's time per step */
     int cand_tile_width[LBM_HIP_TUNE_CANDIDATES];
@@ -200,7 +203,8 @@ typedef struct lbm_hip_tuning {
 
 int lbm_hip_autotune(lbm_hip_ctx* ctx, int trial_steps, int mass_every, lbm_hip_tuning* out);
 
-/* Select a tile shape directly (tile_width 64 / 128 / 256, xcd_remap 0 / 1 / 2). */
+/* Select a tile shape directly (tile_width 64 / 128 / 256 with xcd_remap
+ * 0 / 1 / 2, or tile_width 0 with xcd_remap 0 / 1). */
 int lbm_hip_set_tiling(lbm_hip_ctx* ctx, int tile_width, int xcd_remap);
 
 /* ---- ensemble of lattices ----

@@ -110,7 +110,7 @@ def exported_symbols() -> list[str]:
     return sorted(names)
 
 
-TUNE_CANDIDATES = 9
+TUNE_CANDIDATES = 11
 
 
 class Tuning(C.Structure):

@@ -105,7 +105,8 @@ struct StepArgs {
     uint32_t PX;          // population row pitch (NX, or NX rounded up to 64 nodes)
     int walls_y, walls_z, has_squirmer;
     int gx, gy, gz;       // grid in tiles
-    int tx_shift;         // log2 of the tile width in x
+    int tx_shift;         // log2 of the tile width in x; 0: row-run tiles (see step_tile)
+    int nchx;             // 64-node chunks per row
     int ty;               // tile height in y
     int xcd_remap;        // 1: give each XCD a contiguous range of tiles; 2: whole tile rows per XCD
     double omPlus, omMinus, Fx, Fy, Fz;

@@ -213,6 +213,7 @@ StepArgs<S> step_args(const lbm_hip_ctx* c) {
     a.NX = c->NX; a.NY = c->NY; a.NZ = c->NZ; a.N = c->N; a.SS = c->SS; a.PX = c->PX;
     a.walls_y = c->p.walls_y; a.walls_z = c->p.walls_z; a.has_squirmer = c->has_squirmer;
     a.gx = c->gx; a.gy = c->gy; a.gz = c->gz; a.tx_shift = c->tx_shift; a.ty = c->ty;
+    a.nchx = (c->NX + 63) / 64;
     a.xcd_remap = c->xcd_remap;
     a.omPlus = c->omPlus; a.omMinus = c->omMinus;
     a.Fx = c->p.fx; a.Fy = c->p.fy; a.Fz = c->p.fz;
@@ -278,20 +279,32 @@ uint32_t slot_stride(uint32_t N, size_t ssize) {
 
 bool layout_shifted(const lbm_hip_ctx* c) { return c->t > 0 && ((c->t - 1) & 1) == 0; }
 
-// Tiles of a block of BLOCK threads covering 2^ts x (BLOCK >> ts) nodes.
+// Tiles of a block of BLOCK threads covering 2^ts x (BLOCK >> ts) nodes, or
+// (ts = 0) BLOCK / 64 consecutive 64-node row chunks.
 int tiles_for(int nx, int ny, int nz, int ts) {
+    if (ts == 0) {
+        const long long chunks = (long long)((nx + 63) / 64) * ny * nz;
+        return (int)((chunks + BLOCK / 64 - 1) / (BLOCK / 64));
+    }
     const int ty = BLOCK >> ts;
     return ((nx + (1 << ts) - 1) >> ts) * ((ny + ty - 1) / ty) * nz;
 }
 
-// Tile shape of the collide-stream kernel (ts = log2 of the tile width, 6..8).
-// The boundary kernel's mass partials follow the step kernel's.
+// Tile shape of the collide-stream kernel (ts = log2 of the tile width, 6..8,
+// or 0 for row runs).  The boundary kernel's mass partials follow the step
+// kernel's.
 void set_tile_shape(lbm_hip_ctx* c, int ts) {
     c->tx_shift = ts;
-    c->ty = BLOCK >> ts;
-    c->gx = (c->p.nx + (1 << ts) - 1) >> ts;
-    c->gy = (c->p.ny + c->ty - 1) / c->ty;
-    c->gz = c->p.nz;
+    if (ts == 0) {
+        c->ty = 1;
+        c->gx = tiles_for(c->p.nx, c->p.ny, c->p.nz, 0);
+        c->gy = c->gz = 1;
+    } else {
+        c->ty = BLOCK >> ts;
+        c->gx = (c->p.nx + (1 << ts) - 1) >> ts;
+        c->gy = (c->p.ny + c->ty - 1) / c->ty;
+        c->gz = c->p.nz;
+    }
     c->G = c->gx * c->gy * c->gz;
     c->bargs.mass_part = c->mass_part ? c->mass_part + c->G : nullptr;
     if (c->graph) { hipGraphExecDestroy(c->graph); c->graph = nullptr; }
@@ -352,7 +365,7 @@ int lbm_hip_create(const lbm_hip_params* pin, lbm_hip_ctx** out) {
     c->K = make_coef(c->omPlus, c->omMinus, pin->fx, pin->fy, pin->fz, scale);
 
     // tiles: one wavefront row-chunk along x, block of 256 threads
-    for (int ts = 6; ts <= 8; ++ts) c->G_max = std::max(c->G_max, tiles_for(pin->nx, pin->ny, pin->nz, ts));
+    for (int ts : {0, 6, 7, 8}) c->G_max = std::max(c->G_max, tiles_for(pin->nx, pin->ny, pin->nz, ts));
     set_tile_shape(c, pin->nx <= 64 ? 6 : (pin->nx <= 128 ? 7 : 8));
     c->nb256 = (int)(((size_t)c->N + 255) / 256);
 
@@ -820,7 +833,7 @@ int lbm_hip_get_info(lbm_hip_ctx* c, lbm_hip_info* o) {
     HIP_TRY(hipStreamSynchronize(c->stream));
     o->count_fluid = nf;
     o->row_pitch = (int)c->PX;
-    o->tile_width = 1 << c->tx_shift;
+    o->tile_width = c->tx_shift ? 1 << c->tx_shift : 0;
     o->xcd_remap = c->xcd_remap;
     return LBM_HIP_OK;
 }
@@ -930,10 +943,12 @@ int snapshot_restore(lbm_hip_ctx* c, const Snapshot& s, hipStream_t stream) {
     return LBM_HIP_OK;
 }
 
-int tile_shift_of(int width) { return width == 64 ? 6 : (width == 128 ? 7 : (width == 256 ? 8 : -1)); }
+int tile_shift_of(int width) {
+    return width == 0 ? 0 : (width == 64 ? 6 : (width == 128 ? 7 : (width == 256 ? 8 : -1)));
+}
 
-const int kTuneWidth[LBM_HIP_TUNE_CANDIDATES] = {64, 128, 256, 64, 128, 256, 64, 128, 256};
-const int kTuneXcd[LBM_HIP_TUNE_CANDIDATES] = {1, 1, 1, 0, 0, 0, 2, 2, 2};
+const int kTuneWidth[LBM_HIP_TUNE_CANDIDATES] = {64, 128, 256, 64, 128, 256, 64, 128, 256, 0, 0};
+const int kTuneXcd[LBM_HIP_TUNE_CANDIDATES] = {1, 1, 1, 0, 0, 0, 2, 2, 2, 1, 0};
 
 int graph_cycle_of(bool squirmer, int mass_every) {
     return (squirmer && mass_every > 0) ? ((mass_every % 2 == 0) ? mass_every : 2 * mass_every) : 2;
@@ -946,9 +961,10 @@ extern "C" {
 int lbm_hip_set_tiling(lbm_hip_ctx* c, int tile_width, int xcd_remap) {
     if (!c) return fail(LBM_HIP_EINVAL, "null context");
     const int ts = tile_shift_of(tile_width);
-    if (ts < 0) return fail(LBM_HIP_EINVAL, "tile_width must be 64, 128 or 256");
-    set_tile_shape(c, ts);
+    if (ts < 0) return fail(LBM_HIP_EINVAL, "tile_width must be 0 (row runs), 64, 128 or 256");
     if (xcd_remap < 0 || xcd_remap > 2) return fail(LBM_HIP_EINVAL, "xcd_remap must be 0, 1 or 2");
+    if (ts == 0 && xcd_remap == 2) return fail(LBM_HIP_EINVAL, "row-run tiles (tile_width 0) take xcd_remap 0 or 1");
+    set_tile_shape(c, ts);
     c->xcd_remap = xcd_remap;
     c->mass_valid = false;  // partials of a pending mass step follow the old tiles
     return LBM_HIP_OK;
@@ -1329,7 +1345,7 @@ int lbm_hip_batch_info_get(lbm_hip_batch* b, lbm_hip_batch_info* o) {
     o->total_tiles = b->Gtot;
     o->boundary_blocks = b->Btot;
     o->link_blocks = b->Ltot;
-    o->tile_width = 1 << b->mem[0]->tx_shift;
+    o->tile_width = b->mem[0]->tx_shift ? 1 << b->mem[0]->tx_shift : 0;
     o->xcd_remap = b->mem[0]->xcd_remap;
     o->device_bytes = b->device_bytes;
     return LBM_HIP_OK;

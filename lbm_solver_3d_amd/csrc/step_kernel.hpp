@@ -577,16 +577,29 @@ __device__ __forceinline__ int xcd_tile_rows(int lin, int G, int gx) {
     return ((k / gx) * 8 + xcd) * gx + k % gx;
 }
 
-// One tile (one block of BLOCK threads, 2^tx_shift x ty nodes of a z plane)
-// of the collide-stream update.
+// One tile (one block of BLOCK threads) of the collide-stream update: either
+// 2^tx_shift x ty nodes of a z plane, or (tx_shift == 0, a "row run")
+// BLOCK / 64 consecutive 64-node chunks of the rows in row-major order,
+// which may end one row and start the next (no idle lanes on rows narrower
+// than a tile).  Either way a wavefront is one 64-node chunk of one row.
 template <class S, bool ODD, int MODE>
 __device__ __forceinline__ void step_tile(const StepArgs<S>& A, const FastCoef& K, int tile) {
     constexpr bool MASS = (MODE & STEP_MASS) != 0;
-    const int tgx = tile % A.gx, trest = tile / A.gx;
-    const int tgy = trest % A.gy, tgz = trest / A.gy;
-    const int lx = threadIdx.x & ((1 << A.tx_shift) - 1);
-    const int ly = __builtin_amdgcn_readfirstlane(threadIdx.x >> A.tx_shift);  // one row per wavefront
-    const int x = (tgx << A.tx_shift) + lx, y = tgy * A.ty + ly, z = tgz;
+    int x, y, z;
+    if (A.tx_shift == 0) {
+        const int chunk = __builtin_amdgcn_readfirstlane(tile * (BLOCK / 64) + (int)(threadIdx.x >> 6));
+        const int row = chunk / A.nchx;
+        x = (chunk - row * A.nchx) * 64 + (int)(threadIdx.x & 63);
+        y = row % A.NY;
+        z = row / A.NY;
+        if (z >= A.NZ) { y = -1; z = 0; }  // past the last row of the lattice
+    } else {
+        const int tgx = tile % A.gx, trest = tile / A.gx;
+        const int tgy = trest % A.gy, tgz = trest / A.gy;
+        const int lx = threadIdx.x & ((1 << A.tx_shift) - 1);
+        const int ly = __builtin_amdgcn_readfirstlane(threadIdx.x >> A.tx_shift);  // one row per wavefront
+        x = (tgx << A.tx_shift) + lx; y = tgy * A.ty + ly; z = tgz;
+    }
 
     const uint32_t NX = (uint32_t)A.NX, NY = (uint32_t)A.NY, NZ = (uint32_t)A.NZ;
     const bool row_ok = (uint32_t)y < NY;  // wavefront-uniform

@@ -45,13 +45,15 @@ def _same(a, b, rtol=0.0):
         assert a["sq"]["contacts"] == b["sq"]["contacts"]
 
 
-@pytest.mark.parametrize("prec,order", [("fp64", 1), ("fp16", 1), ("fp16", 2), ("fp32", 0)])
-def test_batch_members_equal_separate_runs(lbm, prec, order):
+@pytest.mark.parametrize("prec,width,order", [("fp64", 128, 1), ("fp16", 128, 1), ("fp16", 128, 2),
+                                              ("fp32", 128, 0), ("fp16", 0, 1), ("fp64", 0, 0)])
+def test_batch_members_equal_separate_runs(lbm, prec, width, order):
     """Tile order 2 permutes the ensemble's tile map on the host (whole tile
-    rows per XCD); the members' own runs use the default tiles."""
+    rows per XCD); width 0 is row runs; the members' own runs use the
+    default tiles."""
     steps = 37  # eager head, graph-replayed cycles, eager tail with moments
     with lbm.Batch(MEMBERS, precision=prec) as B:
-        B.set_tiling(128, order)
+        B.set_tiling(width, order)
         info = B.info()
         assert info["count"] == len(MEMBERS)
         assert info["total_nodes"] == sum(m["nx"] * m["ny"] * m["nz"] for m in MEMBERS)
@@ -87,7 +89,8 @@ def test_batch_autotune_keeps_state_and_picks_a_candidate(lbm):
         B.run(6, 10, 0)
         before = [_state(B.member(i)) for i in range(len(MEMBERS))]
         t = B.autotune(trial_steps=4, mass_every=10)
-        assert t["tile_width"] in (64, 128, 256) and t["xcd_remap"] in (0, 1, 2)
+        assert t["tile_width"] in (0, 64, 128, 256) and t["xcd_remap"] in (0, 1, 2)
+        assert len(t["candidates"]) == lbm.TUNE_CANDIDATES
         assert t["ms_per_step"] == min(c["ms_per_step"] for c in t["candidates"])
         info = B.info()
         assert (info["tile_width"], info["xcd_remap"]) == (t["tile_width"], t["xcd_remap"])
@@ -105,7 +108,7 @@ def test_batch_autotune_keeps_state_and_picks_a_candidate(lbm):
         _same(a, b)
 
 
-@pytest.mark.parametrize("width,order", [(64, 0), (128, 2), (256, 2), (64, 2)])
+@pytest.mark.parametrize("width,order", [(64, 0), (128, 2), (256, 2), (64, 2), (0, 1), (0, 0)])
 def test_tile_shapes_agree_without_squirmer(lbm, width, order):
     kv = dict(nx=70, ny=18, nz=9, tau=0.8, fx=2e-5, walls_y=1, walls_z=1, precision="fp64")
     with lbm.Lattice(**kv) as A, lbm.Lattice(**kv) as B:
@@ -121,5 +124,5 @@ def test_single_autotune_keeps_state(lbm):
         L.run(5, 10, 0)
         a = _state(L)
         t = L.autotune(trial_steps=4, mass_every=10)
-        assert t["tile_width"] in (64, 128, 256)
+        assert t["tile_width"] in (0, 64, 128, 256)
         _same(a, _state(L))

@@ -34,7 +34,7 @@ def _assert_same(a, b):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32", "fp16"])
-@pytest.mark.parametrize("width,xcd", [(64, 0), (128, 1), (256, 0), (256, 2), (64, 2)])
+@pytest.mark.parametrize("width,xcd", [(64, 0), (128, 1), (256, 0), (256, 2), (64, 2), (0, 1), (0, 0)])
 def test_results_independent_of_tile_shape(lbm, prec, width, xcd):
     """Other tiles, other per-tile mass partials: the same state bit for bit
     after graph-replayed steps with mass corrections."""
