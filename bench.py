@@ -140,6 +140,26 @@ def measure_ensemble(lbm, steps: int, warmup: int, count: int = 64, sequential: 
                 device_gb=info["device_bytes"] / 1e9)
 
 
+def measure_mlp(lbm, samples: int = 4096, epochs: int = 20, batch: int = 256) -> dict:
+    """Training throughput of the surrogate's networks alone: 4 nets
+    5-128-128-128-6 (BF16 operands, FP32 accumulation) on synthetic data."""
+    rng = np.random.default_rng(0)
+    x = rng.uniform(-1, 1, size=(samples, 5)).astype(np.float32)
+    t = np.sin(x @ rng.normal(size=(5, 6))).astype(np.float32)
+    with lbm.MLP(nets=4, mode="bf16", seed=1) as m:
+        m.train(x, t, epochs=1, batch=batch, lr=2e-3)  # warm-up (the first launch loads the code objects)
+        t0 = time.perf_counter()
+        loss = m.train(x, t, epochs=epochs, batch=batch, lr=2e-3)
+        dt = time.perf_counter() - t0
+        m.forward(x)
+    return {"metric": "surrogate MLP training throughput, 4 nets 5-128-128-128-6, BF16 MFMA",
+            "value": round(4 * samples * epochs / dt, 1), "unit": "sample-updates/s (all 4 nets)", "n_gpus": 1,
+            "steps": epochs * (samples // batch), "warmup": samples // batch, "ms_per_step": round(dt * 1e3 / (epochs * (samples // batch)), 4),
+            "higher_is_better": True, "dtype": "bf16", "data": "synthetic",
+            "config": {"workload": f"{samples} samples, {epochs} epochs, minibatch {batch}"},
+            "final_loss": [round(float(v), 6) for v in loss]}
+
+
 def cpu_baseline(budget_s: float = 12.0) -> dict | None:
     """Reference step on this host's cores, bounded sample of the workload."""
     try:
@@ -206,11 +226,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=500)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--precisions", default="fp16,fp32")
-    ap.add_argument("--workload", default="channel", choices=["channel", "ensemble", "surrogate", "trajectory"],
+    ap.add_argument("--workload", default="channel", choices=["channel", "ensemble", "surrogate", "trajectory", "mlp"],
                     help="channel: the 512x256x128 slit channel (headline); ensemble: 64 slit-channel runs batched; "
                          "surrogate: the learned velocity surrogate with LBM fallback, 1000 trajectories; "
                          "trajectory: the 512x256x128 pusher in FP32 / FP16 / BF16 storage against FP64 "
-                         "(--steps steps)")
+                         "(--steps steps); mlp: training of the surrogate's 4-net ensemble alone on 4096 "
+                         "synthetic samples, 20 epochs of 256-sample minibatches")
     ap.add_argument("--tiling", default=None,
                     help="WIDTH,XCD: use this collide-stream tile shape instead of autotuning (e.g. 256,0)")
     ap.add_argument("--no-sequential", action="store_true",
@@ -257,6 +278,9 @@ def main():
                                     "512x256x128 pusher", "value": rep["fp16"]["velocity_rel_err_mean"],
                           "unit": "relative velocity error (FP16 store, mean)", "n_gpus": 1, "steps": args.steps,
                           "warmup": 0, "higher_is_better": False, "data": "synthetic", "report": rep}))
+        return
+    if args.workload == "mlp":
+        print(json.dumps(measure_mlp(lbm)))
         return
     if args.workload == "surrogate":
         from lbm_solver_3d_amd import surrogate as S
