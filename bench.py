@@ -111,15 +111,19 @@ def trajectory_errors(lbm, steps: int, every: int = 500, precisions=("fp64", "fp
     return out
 
 
-def measure_ensemble(lbm, steps: int, warmup: int, count: int = 64, sequential: bool = True) -> dict:
+def measure_ensemble(lbm, steps: int, warmup: int, count: int = 64, sequential: bool = True, tiling=None) -> dict:
     """The 64-run slit-channel ensemble (seed 0, FP16 store): every phase of a
     step is one launch for all runs, the step cycle graph-replayed, tile shape
-    autotuned.  Beside it, the same runs advanced one after another, each on
-    its own (graph-replayed, autotuned) lattice."""
+    autotuned (or fixed by `tiling`).  Beside it, the same runs advanced one
+    after another, each on its own (graph-replayed, autotuned) lattice."""
     members = lbm.ensemble_members(count, 0)
     with lbm.Batch(members, precision="fp16") as B:
         info = B.info()
-        tuning = B.autotune(trial_steps=10, mass_every=MASS_EVERY)
+        if tiling:
+            B.set_tiling(*tiling)
+            tuning = dict(tile_width=tiling[0], xcd_remap=tiling[1], candidates=[])
+        else:
+            tuning = B.autotune(trial_steps=10, mass_every=MASS_EVERY)
         if warmup:
             B.time_run(warmup, MASS_EVERY)
         t0 = time.perf_counter()
@@ -233,7 +237,8 @@ def main():
                          "(--steps steps); mlp: training of the surrogate's 4-net ensemble alone on 4096 "
                          "synthetic samples, 20 epochs of 256-sample minibatches")
     ap.add_argument("--tiling", default=None,
-                    help="WIDTH,XCD: use this collide-stream tile shape instead of autotuning (e.g. 256,0)")
+                    help="WIDTH,XCD: use this collide-stream tile shape instead of autotuning (e.g. 256,0; "
+                         "width 0: row runs); channel and ensemble workloads")
     ap.add_argument("--no-sequential", action="store_true",
                     help="ensemble: skip timing the runs one after another")
     ap.add_argument("--copy-probe", action="store_true",
@@ -257,7 +262,8 @@ def main():
     lbm.load_library()
     if args.workload == "ensemble":
         print(f"[bench] ensemble: warmup {args.warmup}, timed {args.steps}", file=sys.stderr, flush=True)
-        e = measure_ensemble(lbm, args.steps, args.warmup, sequential=not args.no_sequential)
+        tiling = tuple(int(v) for v in args.tiling.split(",")) if args.tiling else None
+        e = measure_ensemble(lbm, args.steps, args.warmup, sequential=not args.no_sequential, tiling=tiling)
         print(json.dumps({
             "metric": "MLUPS, batched ensemble of 64 slit-channel squirmer runs (seed 0), FP16 store",
             "value": round(e["mlups"], 1), "unit": "MLUPS", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
