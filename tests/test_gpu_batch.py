@@ -46,7 +46,7 @@ def _same(a, b, rtol=0.0):
 
 
 @pytest.mark.parametrize("prec,width,order", [("fp64", 128, 1), ("fp16", 128, 1), ("fp16", 128, 2),
-                                              ("fp32", 128, 0), ("fp16", 0, 1), ("fp64", 0, 0)])
+                                              ("fp32", 128, 0), ("fp16", 0, 1), ("fp64", 0, 0), ("bf16", 64, 2)])
 def test_batch_members_equal_separate_runs(lbm, prec, width, order):
     """Tile order 2 permutes the ensemble's tile map on the host (whole tile
     rows per XCD); width 0 is row runs; the members' own runs use the

@@ -33,7 +33,7 @@ def _assert_same(a, b):
         assert np.array_equal(a["sq"][k], b["sq"][k]), k
 
 
-@pytest.mark.parametrize("prec", ["fp64", "fp32", "fp16"])
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "fp16", "bf16"])
 @pytest.mark.parametrize("width,xcd", [(64, 0), (128, 1), (256, 0), (256, 2), (64, 2), (0, 1), (0, 0)])
 def test_results_independent_of_tile_shape(lbm, prec, width, xcd):
     """Other tiles, other per-tile mass partials: the same state bit for bit
