@@ -2,6 +2,8 @@
 // squirmer / remap kernel for every storage format, and the auxiliary
 // kernels.  Compiled with -ffp-contract=off: no multiply-add is fused, so the
 // FP64 results round exactly like the reference's scalar code.
+#include <algorithm>
+
 #include "launch_step_impl.hpp"
 
 namespace lbmhip {
@@ -19,10 +21,12 @@ hipError_t launch_links(const LinkArgs<S>& a, int blocks, hipStream_t s) {
 }
 
 template <class S>
-hipError_t launch_wall_force(const S* f, const uint8_t* flag, double* part, DevState* st, int NX, int NY, int NZ,
-                             uint32_t N, uint32_t SS, uint32_t PX, hipStream_t s) {
-    hipLaunchKernelGGL((wall_force_kernel<S>), dim3((N + 255) / 256), dim3(256), 0, s, f, flag, part, st, NX, NY, NZ,
-                       N, SS, PX);
+hipError_t launch_wall_force(const S* f, const uint8_t* flag, const WallForceArgs& w, double* part, DevState* st,
+                             int NX, int NY, int NZ, uint32_t SS, uint32_t PX, hipStream_t s) {
+    // at least one block, so that a lattice without walls publishes a zero force
+    const long long waves = (long long)w.n_wall_rows * w.nchx;
+    const unsigned blocks = (unsigned)std::max(1LL, (waves + 3) / 4);
+    hipLaunchKernelGGL((wall_force_kernel<S>), dim3(blocks), dim3(256), 0, s, f, flag, w, part, st, NX, NY, NZ, SS, PX);
     return hipGetLastError();
 }
 
@@ -85,8 +89,8 @@ hipError_t launch_batch_squirmer(const BatchMember<S>* m, int count, long long t
     template hipError_t launch_batch_squirmer<S>(const BatchMember<S>*, int, long long, hipStream_t);            \
     template hipError_t launch_squirmer<S>(const RemapArgs<S>&, hipStream_t);                                    \
     template hipError_t launch_links<S>(const LinkArgs<S>&, int, hipStream_t);                                   \
-    template hipError_t launch_wall_force<S>(const S*, const uint8_t*, double*, DevState*, int, int, int,       \
-                                             uint32_t, uint32_t, uint32_t, hipStream_t);                                  \
+    template hipError_t launch_wall_force<S>(const S*, const uint8_t*, const WallForceArgs&, double*, DevState*, \
+                                             int, int, int, uint32_t, uint32_t, hipStream_t);                             \
     template hipError_t launch_gather<S>(const S*, const uint8_t*, double*, int, int, int, uint32_t, uint32_t,   \
                                          uint32_t, int, const DevState*, int, hipStream_t);                                \
     template hipError_t launch_scatter<S>(S*, const uint8_t*, const double*, int, int, int, uint32_t, uint32_t,  \

@@ -30,8 +30,8 @@ template <class S>
 hipError_t launch_links(const LinkArgs<S>& a, int blocks, hipStream_t s);
 
 template <class S>
-hipError_t launch_wall_force(const S* f, const uint8_t* flag, double* part, DevState* st, int NX, int NY, int NZ,
-                             uint32_t N, uint32_t SS, uint32_t PX, hipStream_t s);
+hipError_t launch_wall_force(const S* f, const uint8_t* flag, const WallForceArgs& w, double* part, DevState* st,
+                             int NX, int NY, int NZ, uint32_t SS, uint32_t PX, hipStream_t s);
 
 template <class S>
 hipError_t launch_gather(const S* f, const uint8_t* flag, double* out, int NX, int NY, int NZ, uint32_t N,
@@ -78,8 +78,8 @@ hipError_t launch_copy_probe(const void* a, void* b, size_t n16, hipStream_t s);
                                                   int, hipStream_t);                                             \
     extern template hipError_t launch_squirmer<S>(const RemapArgs<S>&, hipStream_t);                            \
     extern template hipError_t launch_links<S>(const LinkArgs<S>&, int, hipStream_t);                           \
-    extern template hipError_t launch_wall_force<S>(const S*, const uint8_t*, double*, DevState*, int, int, int, \
-                                                    uint32_t, uint32_t, uint32_t, hipStream_t);                           \
+    extern template hipError_t launch_wall_force<S>(const S*, const uint8_t*, const WallForceArgs&, double*,      \
+                                                    DevState*, int, int, int, uint32_t, uint32_t, hipStream_t);           \
     extern template hipError_t launch_gather<S>(const S*, const uint8_t*, double*, int, int, int, uint32_t,      \
                                                 uint32_t, uint32_t, int, const DevState*, int, hipStream_t);              \
     extern template hipError_t launch_scatter<S>(S*, const uint8_t*, const double*, int, int, int, uint32_t,     \

@@ -277,6 +277,16 @@ uint32_t slot_stride(uint32_t N, size_t ssize) {
     return (uint32_t)((bytes + pad) / ssize);
 }
 
+WallForceArgs wall_force_args(const lbm_hip_ctx* c) {
+    WallForceArgs w;
+    w.wall_rows = c->bargs.wall_rows;
+    w.n_wall_rows = c->bargs.n_wall_rows;
+    w.nchx = c->bargs.nchx;
+    w.walls_y = c->p.walls_y;
+    w.walls_z = c->p.walls_z;
+    return w;
+}
+
 bool layout_shifted(const lbm_hip_ctx* c) { return c->t > 0 && ((c->t - 1) & 1) == 0; }
 
 // Tiles of a block of BLOCK threads covering 2^ts x (BLOCK >> ts) nodes, or
@@ -530,8 +540,8 @@ int lbm_hip_step(lbm_hip_ctx* c, int flags) {
         // moving-boundary bounce-back and force/torque on the squirmer
         if (e == hipSuccess && c->has_squirmer) e = launch_links<S>(link_args<S>(c), c->link_blocks, c->stream);
         if (e == hipSuccess && (flags & LBM_STEP_WALL_FORCE))
-            e = launch_wall_force<S>((const S*)c->f, c->flag, c->red_part, c->st, c->NX, c->NY, c->NZ, c->N, c->SS,
-                                     c->PX, c->stream);
+            e = launch_wall_force<S>((const S*)c->f, c->flag, wall_force_args(c), c->red_part, c->st, c->NX, c->NY,
+                                     c->NZ, c->SS, c->PX, c->stream);
         return 0;
     });
     if (e != hipSuccess) return fail(LBM_HIP_EDEVICE, std::string("step launch: ") + hipGetErrorString(e));
@@ -1149,8 +1159,8 @@ int batch_step(lbm_hip_batch* b, int mass_every, int extra_flags) {
         if (e == hipSuccess && (extra_flags & LBM_STEP_WALL_FORCE))
             for (lbm_hip_ctx* c : b->mem)
                 if (e == hipSuccess)
-                    e = launch_wall_force<S>((const S*)c->f, c->flag, c->red_part, c->st, c->NX, c->NY, c->NZ, c->N,
-                                             c->SS, c->PX, b->stream);
+                    e = launch_wall_force<S>((const S*)c->f, c->flag, wall_force_args(c), c->red_part, c->st, c->NX,
+                                             c->NY, c->NZ, c->SS, c->PX, b->stream);
         if (e == hipSuccess && b->any_squirmer) e = launch_batch_squirmer<S>(d, b->count, tnext, b->stream);
         return 0;
     });

@@ -198,29 +198,42 @@ __global__ __launch_bounds__(LINK_BLOCK) void particle_links_kernel(const LinkAr
 }
 
 // Wall momentum exchange (lbm.cpp:209, 219): for every WALL link the bounced
-// population equals f*_i, so dp = 2 f*_i c_i.  Full-grid sweep, on request.
+// population equals f*_i, so dp = 2 f*_i c_i.  Only the rows next to a wall
+// have WALL links, and their wall directions follow from the row's
+// coordinates (the walls are the planes y = 0, NY - 1 and z = 0, NZ - 1), so
+// one wavefront takes one 64-node chunk of such a row: a flag load per node
+// (the squirmer may cover part of the row) and one load per wall direction
+// from the wall node's slot.  On request.
+struct WallForceArgs {
+    const uint32_t* wall_rows;  // y + NY * z of each row next to a wall
+    int n_wall_rows, nchx;      // rows and 64-node chunks per row
+    int walls_y, walls_z;
+};
+
 template <class S>
 __global__ __launch_bounds__(256) void wall_force_kernel(const S* __restrict__ f, const uint8_t* __restrict__ flag,
-                                                         double* part, DevState* st, int NX, int NY, int NZ,
-                                                         uint32_t N, uint32_t SS, uint32_t PX) {
+                                                         const WallForceArgs W, double* part, DevState* st, int NX,
+                                                         int NY, int NZ, uint32_t SS, uint32_t PX) {
     __shared__ double lds[4 * 3];
-    const uint32_t n = blockIdx.x * 256u + threadIdx.x;
+    const int wave = (int)(blockIdx.x * 4u + (threadIdx.x >> 6));
     double acc[3] = {0.0, 0.0, 0.0};
-    if (n < N) {
-        const uint8_t fl = flag[n];
-        if ((fl & TYPE_MASK) == FLUID && (fl & BOUNDARY)) {
-            const int x = (int)(n % (uint32_t)NX), y = (int)((n / (uint32_t)NX) % (uint32_t)NY),
-                      z = (int)(n / ((uint32_t)NX * (uint32_t)NY));
-            for (int i = 1; i < Q; ++i) {
-                const int xn = x + CX[i] < 0 ? NX - 1 : (x + CX[i] >= NX ? 0 : x + CX[i]);
-                const int yn = y + CY[i] < 0 ? NY - 1 : (y + CY[i] >= NY ? 0 : y + CY[i]);
-                const int zn = z + CZ[i] < 0 ? NZ - 1 : (z + CZ[i] >= NZ ? 0 : z + CZ[i]);
-                const uint32_t m = (uint32_t)xn + (uint32_t)NX * ((uint32_t)yn + (uint32_t)NY * (uint32_t)zn);
-                if ((flag[m] & TYPE_MASK) != WALL) continue;
-                const double fpost = to_real<S>(f[(size_t)i * SS + pop_index(xn, yn, zn, PX, NY)], i);
-                const double s = fpost + fpost;
-                acc[0] += s * CX[i]; acc[1] += s * CY[i]; acc[2] += s * CZ[i];
-            }
+    if (wave < W.n_wall_rows * W.nchx) {
+        const uint32_t r = W.wall_rows[wave / W.nchx];
+        const int y = (int)(r % (uint32_t)NY), z = (int)(r / (uint32_t)NY);
+        const int x = (wave % W.nchx) * 64 + (int)(threadIdx.x & 63);
+        if (x < NX && (flag[(size_t)x + (size_t)NX * r] & TYPE_MASK) == FLUID) {
+            const int xs[3] = {x == 0 ? NX - 1 : x - 1, x, x + 1 == NX ? 0 : x + 1};
+            static_for<1, Q>([&](auto I) {
+                constexpr int i = decltype(I)::value;
+                const int yn = y + CY[i], zn = z + CZ[i];
+                const bool wall = (W.walls_y && (yn == 0 || yn == NY - 1)) || (W.walls_z && (zn == 0 || zn == NZ - 1));
+                if (wall) {  // wavefront-uniform
+                    const int yw = yn < 0 ? NY - 1 : (yn >= NY ? 0 : yn), zw = zn < 0 ? NZ - 1 : (zn >= NZ ? 0 : zn);
+                    const double fpost = to_real<S>(f[(size_t)i * SS + pop_index(xs[1 + CX[i]], yw, zw, PX, NY)], i);
+                    const double s = fpost + fpost;
+                    acc[0] += s * CX[i]; acc[1] += s * CY[i]; acc[2] += s * CZ[i];
+                }
+            });
         }
     }
     block_reduce<3>(acc, lds);

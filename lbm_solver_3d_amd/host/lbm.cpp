@@ -69,15 +69,19 @@ void LBM::step() {
     check(lbm_hip_step(ctx_, flags));
     ++stepsDone_;
     if (!lazy_) {
-        check(lbm_hip_get_wall_force(ctx_, wallForce));
-        if (hasSquirmer) syncState();
+        if (hasSquirmer) syncState();  // Fh, Th and the wall force of this step
+        else check(lbm_hip_get_wall_force(ctx_, wallForce));
     }
 }
 
 void LBM::advanceSquirmer() {
     if (!hasSquirmer) return;
     check(lbm_hip_advance_squirmer(ctx_));
-    if (!lazy_) syncState();
+    if (!lazy_) {  // the squirmer moved; the wall force is the step's, unchanged
+        lbm_squirmer_state s;
+        check(lbm_hip_get_squirmer(ctx_, &s));
+        to_squirmer(s, sq);
+    }
 }
 
 void LBM::correctMass() { check(lbm_hip_correct_mass(ctx_)); }
