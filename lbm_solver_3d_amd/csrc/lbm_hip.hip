@@ -84,9 +84,11 @@ struct lbm_hip_ctx {
     int last_bparts = 0;  // boundary-kernel mass partials written by the last step
     size_t device_bytes = 0;
     std::vector<hipEvent_t> events;
-    // one captured cycle of the driver loop (step, advance, mass correction)
-    hipGraphExec_t graph = nullptr;
-    int graph_cycle = 0, graph_mass_every = -1;
+    // one captured cycle of the driver loop (step, advance, mass correction),
+    // and graph_reps cycles in one graph (graph_long): each graph launch
+    // costs a gap of several microseconds on the device
+    hipGraphExec_t graph = nullptr, graph_long = nullptr;
+    int graph_cycle = 0, graph_mass_every = -1, graph_reps = 1;
     bool owned_by_batch = false;  // member of an lbm_hip_batch
     double radius_max = 0.0;      // radius the boundary work list is sized for
     int remaps_since_step = 0;    // lbm_hip_advance_squirmer calls since the last lbm_hip_step
@@ -289,6 +291,11 @@ WallForceArgs wall_force_args(const lbm_hip_ctx* c) {
 
 bool layout_shifted(const lbm_hip_ctx* c) { return c->t > 0 && ((c->t - 1) & 1) == 0; }
 
+void drop_graphs(lbm_hip_ctx* c) {
+    for (hipGraphExec_t* g : {&c->graph, &c->graph_long})
+        if (*g) { hipGraphExecDestroy(*g); *g = nullptr; }
+}
+
 // Tiles of a block of BLOCK threads covering 2^ts x (BLOCK >> ts) nodes, or
 // (ts = 0) BLOCK / 64 consecutive 64-node row chunks.
 int tiles_for(int nx, int ny, int nz, int ts) {
@@ -317,7 +324,7 @@ void set_tile_shape(lbm_hip_ctx* c, int ts) {
     }
     c->G = c->gx * c->gy * c->gz;
     c->bargs.mass_part = c->mass_part ? c->mass_part + c->G : nullptr;
-    if (c->graph) { hipGraphExecDestroy(c->graph); c->graph = nullptr; }
+    drop_graphs(c);
 }
 
 void free_ctx(lbm_hip_ctx* c) {
@@ -328,7 +335,7 @@ void free_ctx(lbm_hip_ctx* c) {
                     (void*)c->diag_part, (void*)c->diag_out, (void*)c->red_part, (void*)c->wall_rows})
         if (q) hipFree(q);
     for (hipEvent_t e : c->events) hipEventDestroy(e);
-    if (c->graph) hipGraphExecDestroy(c->graph);
+    drop_graphs(c);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }
@@ -604,31 +611,41 @@ static int loop_step(lbm_hip_ctx* c, int mass_every, int extra_flags) {
     return LBM_HIP_OK;
 }
 
-// Capture one cycle of the loop (parity period 2 x mass period) into a graph.
-// The kernels see the step index only through its parity and the mass
-// cadence, so a cycle starting at any multiple of the cycle length replays
-// the same work.
-static int ensure_graph(lbm_hip_ctx* c, int mass_every) {
-    int cycle = 2;
-    if (c->has_squirmer && mass_every > 0) cycle = (mass_every % 2 == 0) ? mass_every : 2 * mass_every;
-    if (c->graph && c->graph_mass_every == mass_every) return LBM_HIP_OK;
-    if (c->graph) { hipGraphExecDestroy(c->graph); c->graph = nullptr; }
+// Capture `cycles` cycles of the loop (a cycle: parity period 2 x mass
+// period) into one graph.  The kernels see the step index only through its
+// parity and the mass cadence, so a capture starting at any multiple of the
+// cycle length replays the same work.
+static int capture_cycles(lbm_hip_ctx* c, int mass_every, int cycles, hipGraphExec_t* out) {
     const long long t_save = c->t;
     const bool mv = c->macro_valid, wv = c->wallf_valid, ms = c->mass_valid;
     c->t = 0;  // capture with a cycle-aligned step index
     HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     int rc = LBM_HIP_OK;
-    for (int s = 0; s < cycle && rc == LBM_HIP_OK; ++s) rc = loop_step(c, mass_every, 0);
+    for (int s = 0; s < cycles * c->graph_cycle && rc == LBM_HIP_OK; ++s) rc = loop_step(c, mass_every, 0);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(c->stream, &g);
     c->t = t_save; c->macro_valid = mv; c->wallf_valid = wv; c->mass_valid = ms;
     if (rc) { if (g) hipGraphDestroy(g); return rc; }
     if (e != hipSuccess) return fail(LBM_HIP_EDEVICE, std::string("graph capture: ") + hipGetErrorString(e));
-    e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+    e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
     hipGraphDestroy(g);
-    if (e != hipSuccess) { c->graph = nullptr; return fail(LBM_HIP_EDEVICE, std::string("graph instantiate: ") + hipGetErrorString(e)); }
-    c->graph_cycle = cycle;
-    c->graph_mass_every = mass_every;
+    if (e != hipSuccess) { *out = nullptr; return fail(LBM_HIP_EDEVICE, std::string("graph instantiate: ") + hipGetErrorString(e)); }
+    return LBM_HIP_OK;
+}
+
+// The one-cycle graph for `mass_every`, and the multi-cycle one when a run
+// of `steps` can use it (about 64 steps per launch).
+static int ensure_graph(lbm_hip_ctx* c, int mass_every, int steps) {
+    if (!(c->graph && c->graph_mass_every == mass_every)) {
+        drop_graphs(c);
+        c->graph_cycle = (c->has_squirmer && mass_every > 0) ? ((mass_every % 2 == 0) ? mass_every : 2 * mass_every) : 2;
+        c->graph_reps = std::max(1, 64 / c->graph_cycle);
+        const int rc = capture_cycles(c, mass_every, 1, &c->graph);
+        if (rc) return rc;
+        c->graph_mass_every = mass_every;
+    }
+    if (!c->graph_long && c->graph_reps > 1 && steps > c->graph_reps * c->graph_cycle)
+        return capture_cycles(c, mass_every, c->graph_reps, &c->graph_long);
     return LBM_HIP_OK;
 }
 
@@ -638,17 +655,20 @@ static int run_loop(lbm_hip_ctx* c, int steps, int mass_every, int last_flags, b
     int left = steps;
     // eager until the step index is cycle-aligned, keep the last step eager
     if (use_graph && steps > 1) {
-        if ((rc = ensure_graph(c, mass_every))) return rc;
+        if ((rc = ensure_graph(c, mass_every, steps))) return rc;
         const int C = c->graph_cycle;
         while (left > 1 && (c->t % C) != 0) {
             if ((rc = loop_step(c, mass_every, 0))) return rc;
             --left;
         }
-        const int reps = (left - 1) / C;
-        for (int r = 0; r < reps; ++r) HIP_TRY(hipGraphLaunch(c->graph, c->stream));
-        if (reps > 0) {
-            c->t += (long long)reps * C;
-            left -= reps * C;
+        int done = 0;
+        if (c->graph_long) {
+            const int CL = C * c->graph_reps;
+            for (; left - 1 >= CL; left -= CL, done += CL) HIP_TRY(hipGraphLaunch(c->graph_long, c->stream));
+        }
+        for (; left - 1 >= C; left -= C, done += C) HIP_TRY(hipGraphLaunch(c->graph, c->stream));
+        if (done > 0) {
+            c->t += done;
             c->macro_valid = c->wallf_valid = false;
             c->mass_valid = false;
             c->remaps_since_step = c->has_squirmer ? 1 : 0;
@@ -860,8 +880,8 @@ int lbm_hip_time_run(lbm_hip_ctx* c, int steps, int mass_every, double* total_ms
     }
     hipEvent_t* ev = c->events.data();
     int rc;
-    // the graph is built outside the timed region
-    if ((rc = ensure_graph(c, mass_every))) return rc;
+    // the graphs are built outside the timed region
+    if ((rc = ensure_graph(c, mass_every, steps))) return rc;
     HIP_TRY(hipStreamSynchronize(c->stream));
     // pass 1: `steps` full driver-loop steps (graph replay)
     HIP_TRY(hipEventRecord(ev[0], c->stream));
