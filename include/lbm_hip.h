@@ -204,7 +204,9 @@ typedef struct lbm_hip_tuning {
 int lbm_hip_autotune(lbm_hip_ctx* ctx, int trial_steps, int mass_every, lbm_hip_tuning* out);
 
 /* Select a tile shape directly (tile_width 64 / 128 / 256 with xcd_remap
- * 0 / 1 / 2, or tile_width 0 with xcd_remap 0 / 1). */
+ * 0 / 1 / 2, or tile_width 0 with xcd_remap 0 / 1).  A new context uses
+ * tiles of 64, 128 or 256 nodes (the smallest at least NX, else 256) in
+ * order 2. */
 int lbm_hip_set_tiling(lbm_hip_ctx* ctx, int tile_width, int xcd_remap);
 
 /* ---- ensemble of lattices ----

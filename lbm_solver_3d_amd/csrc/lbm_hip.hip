@@ -72,7 +72,7 @@ struct lbm_hip_ctx {
     BoundaryArgs bargs{};
     int bblocks = 0;                // blocks of aa_boundary_kernel
     int gx = 0, gy = 0, gz = 0, tx_shift = 8, ty = 1, G = 0, nb256 = 0;
-    int xcd_remap = 1;
+    int xcd_remap = 2;  // whole tile rows per XCD until set_tiling / autotune chooses
     int G_max = 0;  // largest tile count over the tile shapes (mass_part holds G_max + bblocks)
     double omPlus = 0, omMinus = 0, nu = 0;
     FastCoef K{};
