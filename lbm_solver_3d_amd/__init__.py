@@ -401,12 +401,12 @@ class Batch:
         _check(self.lib, self.lib.lbm_hip_batch_synchronize(self.h))
 
     def member(self, i: int) -> Lattice:
+        """Member i as a Lattice, after the batch's queued work (one stream
+        synchronisation; the member's own getters report its errors)."""
+        hm = C.c_void_p()
+        _check(self.lib, self.lib.lbm_hip_batch_member(self.h, i, C.byref(hm)))
         if self._members[i] is None:
-            hm = C.c_void_p()
-            _check(self.lib, self.lib.lbm_hip_batch_member(self.h, i, C.byref(hm)))
             self._members[i] = Lattice(_handle=hm)
-        else:
-            self.synchronize()
         return self._members[i]
 
     def info(self) -> dict:
