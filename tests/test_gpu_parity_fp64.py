@@ -114,6 +114,10 @@ def test_channel_squirmer_near_wall_fp64(oracle_mod, lbm):
     with lbm.Lattice(**kv, precision="fp64") as L:
         _run_pair(o, L, lbm, 333)
         _compare_squirmer(o, L)
+        # the wall momentum exchange with the sphere close to the wall (its
+        # flow on the rows next to the wall), summed in another order
+        wf, wo = L.wall_force(), np.array(o.L.wallForce[:])
+        assert np.abs(wf - wo).max() <= 1e-11 * np.abs(wo).max()
 
 
 @pytest.mark.parametrize("kv,steps", [
