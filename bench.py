@@ -315,6 +315,14 @@ def main():
     # algorithmic bytes of one collide-stream launch
     copy_gbs = lbm.copy_bandwidth(int(head["bytes_per_lup"] * head["N"]))
     print(f"[bench] streaming copy of the same byte count: {copy_gbs:.0f} GB/s", file=sys.stderr, flush=True)
+    # the copy ceiling falls as the bytes per launch grow, so every precision
+    # is held against a copy of its own launch's bytes
+    copy_of = {}
+    for k, v in res.items():
+        nb = int(v["bytes_per_lup"] * v["N"])
+        copy_of[k] = copy_gbs if v is head else lbm.copy_bandwidth(nb)
+        print(f"[bench] {k}: copy of {nb / 1e9:.2f} GB at {copy_of[k]:.0f} GB/s, kernel at "
+              f"{v['achieved_gbs'] / copy_of[k] * 100:.1f}% of it", file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
@@ -361,11 +369,14 @@ def main():
         out["fp32"] = {"value": round(res["fp32"]["mlups"], 1), "ms_per_step": round(res["fp32"]["ms_per_step"], 5),
                        "kernel_ms": round(res["fp32"]["kernel_ms"], 5),
                        "roofline_frac": round(res["fp32"]["frac"], 4),
-                       "achieved_gbs": round(res["fp32"]["achieved_gbs"], 1)}
+                       "achieved_gbs": round(res["fp32"]["achieved_gbs"], 1),
+                       "copy_gbs": round(copy_of["fp32"], 1),
+                       "frac_of_copy": round(res["fp32"]["achieved_gbs"] / copy_of["fp32"], 4)}
         out["fp16_over_fp32"] = round(res["fp16"]["mlups"] / res["fp32"]["mlups"], 3)
     for k, v in res.items():
         out.setdefault("per_precision", {})[k] = {kk: (round(vv, 6) if isinstance(vv, float) else vv)
                                                   for kk, vv in v.items() if kk not in ("U", "X", "tuning")}
+        out["per_precision"][k]["copy_gbs"] = round(copy_of[k], 1)
         if v.get("tuning"):
             out["per_precision"][k]["tuning_ms_per_step"] = {
                 f"{c['tile_width'] or 'run'}/order{c['xcd_remap']}": round(c["ms_per_step"], 5)
