@@ -131,6 +131,11 @@ def measure_ensemble(lbm, steps: int, warmup: int, count: int = 64, sequential: 
         wall = time.perf_counter() - t0
         info = B.info()
     nodes = int(info["total_nodes"])
+    # the collide-stream traffic of one step (19 populations read and written,
+    # 2 bytes each) against a streaming copy of as many bytes: the copy's rate
+    # falls as the bytes per launch grow
+    step_bytes = 2 * 19 * 2 * nodes
+    copy_gbs = lbm.copy_bandwidth(step_bytes)
     seq_ms = 0.0
     for kv in (members if sequential else []):
         with lbm.Lattice(**kv, precision="fp16") as L:
@@ -141,7 +146,8 @@ def measure_ensemble(lbm, steps: int, warmup: int, count: int = 64, sequential: 
     return dict(runs=count, total_nodes=nodes, steps=steps, total_ms=ms, wall_s=wall, ms_per_step=ms / steps,
                 mlups=nodes * steps / (ms * 1e3), sequential_ms_per_step=seq_ms / steps if sequential else None,
                 sequential_mlups=nodes * steps / (seq_ms * 1e3) if sequential else None, tuning=tuning,
-                device_gb=info["device_bytes"] / 1e9)
+                device_gb=info["device_bytes"] / 1e9, step_bytes=step_bytes, copy_gbs=copy_gbs,
+                step_gbs=step_bytes / (ms / steps) / 1e6)
 
 
 def measure_mlp(lbm, samples: int = 4096, epochs: int = 20, batch: int = 256) -> dict:
@@ -275,7 +281,13 @@ def main():
             "sequential": ({"mlups": round(e["sequential_mlups"], 1),
                             "ms_per_step": round(e["sequential_ms_per_step"], 5)} if e["sequential_mlups"] else None),
             "batch_over_sequential": (round(e["mlups"] / e["sequential_mlups"], 3) if e["sequential_mlups"] else None),
-            "tuning": e["tuning"], "device_gb": round(e["device_gb"], 2)}))
+            "tuning": e["tuning"], "device_gb": round(e["device_gb"], 2),
+            # whole step (every phase) at the collide-stream bytes: a lower bound of the kernel's rate
+            "roofline": {"bound": "hbm", "achieved": round(e["step_gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(e["step_gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
+                         "timing": "whole step", "bytes_per_step": e["step_bytes"],
+                         "copy_gbs": round(e["copy_gbs"], 1),
+                         "frac_of_copy": round(e["step_gbs"] / e["copy_gbs"], 4)}}))
         return
     if args.workload == "trajectory":
         log = lambda m: print(m, file=sys.stderr, flush=True)
