@@ -1,0 +1,26 @@
+"""bench.py's contract pieces that need no GPU: the refusal of several GPUs
+and the traffic figures it reads from the committed counter summary."""
+import json
+import subprocess
+import sys
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parents[1]
+sys.path.insert(0, str(ROOT))
+
+
+def test_more_than_one_gpu_is_refused():
+    r = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120, cwd=ROOT)
+    assert r.returncode == 2
+    msg = json.loads(r.stdout.strip().splitlines()[-1])
+    assert "one GPU" in msg["error"]
+
+
+def test_traffic_per_launch_from_counter_summary():
+    import bench
+    for prec, bytes_per_lup in (("fp16", 2 * 19 * 2), ("fp32", 2 * 19 * 4)):
+        t = bench.measured_traffic(prec)
+        assert t is not None, f"no {prec} traffic line in the counter summary"
+        algorithmic = bytes_per_lup * 512 * 256 * 128
+        assert 0.9 < t / algorithmic < 1.1
