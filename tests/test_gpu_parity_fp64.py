@@ -127,6 +127,15 @@ def test_channel_squirmer_near_wall_fp64(oracle_mod, lbm):
     # fully periodic, the sphere over the corner of the box: every box wraps in x, y and z
     (dict(nx=30, ny=26, nz=28, walls_y=0, walls_z=0, squirmer=1, radius=4.0, b1=0.02, beta=2.0, angle=30,
           planar=0, x0=0.4, y0=25.7, z0=0.6), 180),
+    # rows of two 64-node chunks: the boundary box wraps around x = 0 across both chunks
+    (dict(nx=128, ny=22, nz=20, walls_y=0, walls_z=1, squirmer=1, radius=4.3, b1=0.03, beta=-2.0, angle=20,
+          planar=1, x0=126.6), 200),
+    # the same rows, the box straddling the chunk edge at x = 64
+    (dict(nx=128, ny=22, nz=20, walls_y=0, walls_z=1, squirmer=1, radius=4.3, b1=0.03, beta=1.0, angle=-15,
+          planar=1, x0=64.3), 200),
+    # a short last chunk (100 = 64 + 36): a wrapping box touches it and both neighbours
+    (dict(nx=100, ny=22, nz=20, walls_y=0, walls_z=1, squirmer=1, radius=4.3, b1=0.03, beta=-1.0, angle=10,
+          planar=1, x0=98.8), 200),
 ])
 def test_squirmer_across_periodic_boundaries_fp64(oracle_mod, lbm, kv, steps):
     o = oracle_mod.Oracle(**kv)
