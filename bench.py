@@ -229,6 +229,27 @@ def measured_traffic(precision: str):
     return None
 
 
+ENSEMBLE_TRAFFIC_FILE = ROOT / "profiles" / "hbm_traffic_ensemble_64runs_fp16.txt"
+
+
+def measured_ensemble_traffic(tile_width: int, xcd_remap: int):
+    """HBM bytes per launch of the batched collide-stream kernel for one tile
+    shape (mean of the even and odd launches, which alternate), from the
+    counter runs recorded in profiles/; None for a shape not recorded there."""
+    shape = f"{tile_width},{xcd_remap}"
+    mb = {}
+    try:
+        for line in ENSEMBLE_TRAFFIC_FILE.read_text().splitlines():
+            p = line.split()
+            if len(p) == 8 and p[0] == shape and p[1] in ("even", "odd"):
+                mb[p[1]] = float(p[5])
+    except OSError:
+        return None
+    if len(mb) != 2:
+        return None
+    return (mb["even"] + mb["odd"]) / 2 * 1e6
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -284,7 +305,8 @@ def main():
             "tuning": e["tuning"], "device_gb": round(e["device_gb"], 2),
             # whole step (every phase) at the collide-stream bytes: a lower bound of the kernel's rate
             "roofline": {"bound": "hbm", "achieved": round(e["step_gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": round(e["step_gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
+                         "frac": round(e["step_gbs"] / HBM_PEAK_GBS, 4),
+                         "traffic": measured_ensemble_traffic(e["tuning"]["tile_width"], e["tuning"]["xcd_remap"]),
                          "timing": "whole step", "bytes_per_step": e["step_bytes"],
                          "copy_gbs": round(e["copy_gbs"], 1),
                          "frac_of_copy": round(e["step_gbs"] / e["copy_gbs"], 4)}}))

@@ -24,3 +24,13 @@ def test_traffic_per_launch_from_counter_summary():
         assert t is not None, f"no {prec} traffic line in the counter summary"
         algorithmic = bytes_per_lup * 512 * 256 * 128
         assert 0.9 < t / algorithmic < 1.1
+
+
+def test_ensemble_traffic_per_shape_from_counter_summary():
+    import bench
+    nodes = 200219976  # the 64 runs of seed 0
+    for shape in ((64, 1), (0, 0), (256, 2)):
+        t = bench.measured_ensemble_traffic(*shape)
+        assert t is not None, shape
+        assert 0.95 < t / (76 * nodes) < 1.2
+    assert bench.measured_ensemble_traffic(128, 2) is None  # no counter run for that shape
