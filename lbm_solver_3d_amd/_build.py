@@ -57,7 +57,7 @@ def _compile(name: str, extra: list[str]) -> Path:
 HOST = PKG / "host"
 FACADE = PKG / "liblbm_facade.so"
 DRIVER = PKG / "bin" / "lbm_squirmer"
-HOST_SRC = ["params.cpp", "squirmer.cpp", "lbm.cpp", "io.cpp", "analytic.cpp"]
+HOST_SRC = ["params.cpp", "squirmer.cpp", "lbm.cpp", "io.cpp", "analytic.cpp", "lbm_io.cpp"]
 
 
 def build_host(verbose: bool = True) -> None:

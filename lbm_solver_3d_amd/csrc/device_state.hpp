@@ -39,12 +39,12 @@ struct DevSquirmer {
     int planar, plane_xz;
 };
 
-// Excess-mass partial sum in 128-bit fixed point: value hi + lo * 2^-64.
-// Integer addition is associative, so the total of the partials does not
-// depend on how the nodes were grouped into tiles, blocks and kernels (tile
-// shape, XCD mapping, bulk / boundary split, ensemble or single lattice);
-// each node's excess is truncated to a multiple of 2^-64 (an error below
-// 6e-20 per node).
+// Excess-mass partial sum in 128-bit fixed point (fixed_point.hpp): value
+// hi + lo * 2^-64.  Integer addition is associative, so the total of the
+// partials does not depend on how the nodes were grouped into tiles, blocks
+// and kernels (tile shape, XCD mapping, bulk / boundary split, ensemble or
+// single lattice); each node's excess is truncated toward zero to a
+// multiple of 2^-64 (an error below 5.5e-20 per node).
 struct MassPart {
     long long hi;
     unsigned long long lo;

@@ -123,48 +123,47 @@ template hipError_t launch_batch_boundary<double>(const BatchMember<double>*, co
 // ---------------------------------------------------------------------------
 // Block sum of fixed-point values (blockDim.x a multiple of 64, at most
 // 1024); thread 0 gets the total.
-__device__ unsigned __int128 block_sum_fixed(unsigned __int128 v) {
+__device__ u128 block_sum_fixed(u128 v) {
     __shared__ unsigned long long lds[2 * 16];
     for (int off = 32; off > 0; off >>= 1) {
-        const unsigned long long h = __shfl_xor((unsigned long long)(v >> 64), off, 64);
-        const unsigned long long l = __shfl_xor((unsigned long long)v, off, 64);
-        v += mass_join(h, l);
+        const unsigned long long h = __shfl_xor((unsigned long long)fixed_hi(v), off, 64);
+        const unsigned long long l = __shfl_xor((unsigned long long)fixed_lo(v), off, 64);
+        v += fixed_join(h, l);
     }
     if ((threadIdx.x & 63) == 0) {
-        lds[2 * (threadIdx.x >> 6)] = (unsigned long long)(v >> 64);
-        lds[2 * (threadIdx.x >> 6) + 1] = (unsigned long long)v;
+        lds[2 * (threadIdx.x >> 6)] = (unsigned long long)fixed_hi(v);
+        lds[2 * (threadIdx.x >> 6) + 1] = fixed_lo(v);
     }
     __syncthreads();
-    unsigned __int128 s = 0;
+    u128 s = 0;
     if (threadIdx.x == 0)
-        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += mass_join(lds[2 * w], lds[2 * w + 1]);
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += fixed_join(lds[2 * w], lds[2 * w + 1]);
     return s;
 }
 
 // The shift d = -excess / n_fluid from the fixed-point excess total.
-__device__ void apply_mass_total(unsigned __int128 s, DevState* st, int terms, int shift_slot) {
-    // hi + lo 2^-64 as a double (|hi| < 2^53)
-    double e = (double)(long long)(unsigned long long)(s >> 64) + (double)(unsigned long long)s * 0x1p-64;
+__device__ void apply_mass_total(u128 s, DevState* st, int terms, int shift_slot) {
+    double e = fixed_to_double(s);
     if (terms & MASS_TERM_REMAP) e += st->remap_dmass;
     if (terms & MASS_TERM_LINKS) e += st->link_dmass;
     const long long nf = st->nfluid;
     if (nf > 0) st->shift[shift_slot] += -e / (double)nf;
 }
 
-__device__ __forceinline__ unsigned __int128 load_part(const MassPart* p) {
-    return mass_join((unsigned long long)p->hi, p->lo);
+__device__ __forceinline__ u128 load_part(const MassPart* p) {
+    return fixed_join((unsigned long long)p->hi, p->lo);
 }
 
 // One block of 1024 threads over all partials (an ensemble member's).
 __device__ void mass_finalize_block(const MassPart* part, int nparts, DevState* st, int terms, int shift_slot) {
-    unsigned __int128 a[4] = {0, 0, 0, 0};  // four loads in flight per thread
+    u128 a[4] = {0, 0, 0, 0};  // four loads in flight per thread
     int k = threadIdx.x;
     for (; k + 3 * 1024 < nparts; k += 4 * 1024) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) a[j] += load_part(&part[k + j * 1024]);
     }
     for (; k < nparts; k += 1024) a[0] += load_part(&part[k]);
-    const unsigned __int128 s = block_sum_fixed((a[0] + a[1]) + (a[2] + a[3]));
+    const u128 s = block_sum_fixed((a[0] + a[1]) + (a[2] + a[3]));
     if (threadIdx.x == 0) apply_mass_total(s, st, terms, shift_slot);
 }
 
@@ -174,21 +173,21 @@ constexpr int FIN_BLOCK = 256;
 __global__ __launch_bounds__(FIN_BLOCK) void mass_finalize_kernel(const MassPart* part, int nparts, DevState* st,
                                                                   int terms, int shift_slot) {
     __shared__ bool is_last;
-    unsigned __int128 v = 0;
+    u128 v = 0;
     for (int k = blockIdx.x * FIN_BLOCK + threadIdx.x; k < nparts; k += gridDim.x * FIN_BLOCK) v += load_part(&part[k]);
     v = block_sum_fixed(v);
     if (threadIdx.x == 0) {
-        st->fin_part[blockIdx.x].hi = (long long)(unsigned long long)(v >> 64);
-        st->fin_part[blockIdx.x].lo = (unsigned long long)v;
+        st->fin_part[blockIdx.x].hi = fixed_hi(v);
+        st->fin_part[blockIdx.x].lo = fixed_lo(v);
         __threadfence();
         is_last = atomicAdd(&st->fin_ticket, 1u) == gridDim.x - 1;
     }
     __syncthreads();
     if (!is_last) return;
     __threadfence();
-    unsigned __int128 w = 0;
+    u128 w = 0;
     for (unsigned b = threadIdx.x; b < gridDim.x; b += FIN_BLOCK)
-        w += mass_join((unsigned long long)__builtin_nontemporal_load(&st->fin_part[b].hi),
+        w += fixed_join((unsigned long long)__builtin_nontemporal_load(&st->fin_part[b].hi),
                        __builtin_nontemporal_load(&st->fin_part[b].lo));
     w = block_sum_fixed(w);
     if (threadIdx.x == 0) {

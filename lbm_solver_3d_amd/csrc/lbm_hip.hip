@@ -930,6 +930,7 @@ struct Snapshot {
     long long t = 0;
     bool macro_valid = false, wallf_valid = false, mass_valid = false;
     int last_bparts = 0;
+    int remaps_since_step = 0;
 };
 
 void snapshot_parts(lbm_hip_ctx* c, void* (&src)[7], size_t (&bytes)[7]) {
@@ -959,6 +960,7 @@ int snapshot_take(lbm_hip_ctx* c, Snapshot& s, hipStream_t stream) {
     s.t = c->t;
     s.macro_valid = c->macro_valid; s.wallf_valid = c->wallf_valid; s.mass_valid = c->mass_valid;
     s.last_bparts = c->last_bparts;
+    s.remaps_since_step = c->remaps_since_step;
     return LBM_HIP_OK;
 }
 
@@ -968,8 +970,13 @@ int snapshot_restore(lbm_hip_ctx* c, const Snapshot& s, hipStream_t stream) {
     snapshot_parts(c, dst, bytes);
     for (int k = 0; k < 7; ++k) HIP_TRY(hipMemcpyAsync(dst[k], s.buf[k], bytes[k], hipMemcpyDeviceToDevice, stream));
     c->t = s.t;
-    c->macro_valid = s.macro_valid; c->wallf_valid = s.wallf_valid; c->mass_valid = s.mass_valid;
+    c->macro_valid = s.macro_valid; c->wallf_valid = s.wallf_valid;
+    // The per-block mass partials of a pending mass step are not part of the
+    // copy (the trials overwrite them, possibly with another tile shape):
+    // a following correct_mass sums the populations instead.
+    c->mass_valid = false;
     c->last_bparts = s.last_bparts;
+    c->remaps_since_step = s.remaps_since_step;
     return LBM_HIP_OK;
 }
 

@@ -102,7 +102,8 @@ __device__ inline void store_macro(void* p, uint32_t n, double v) {
 
 template <class S>
 __device__ void squirmer_block(const RemapArgs<S>& A) {
-    __shared__ uint32_t list[REMAP_CAP];  // node | cover bit, in sweep order
+    __shared__ uint32_t list[REMAP_CAP];  // changed nodes in sweep order
+    __shared__ uint8_t lcov[REMAP_CAP];   // 1: the node is covered, 0: uncovered
     __shared__ int lpos[REMAP_CAP];       // box position of each entry (ascending)
     __shared__ double lrr[REMAP_CAP], lub[REMAP_CAP][3];
     __shared__ double gval[GATHER][Q];    // cover: f_i(n); uncover: rho of neighbour i
@@ -318,7 +319,8 @@ __device__ void squirmer_block(const RemapArgs<S>& A) {
         int pos = scount[wave] + incl - cnt;
         visit([&](int ix, bool inside) {
             lpos[pos] = ix + bwx * (iy + bwy * iz);
-            list[pos++] = node_at(ix, iy, iz) | (inside ? 0x80000000u : 0u);
+            lcov[pos] = inside ? 1 : 0;
+            list[pos++] = node_at(ix, iy, iz);
         });
         K = sK;
         __syncthreads();  // scount / sK reused by the next round
@@ -335,8 +337,7 @@ __device__ void squirmer_block(const RemapArgs<S>& A) {
         // which gives back the type before this remap.
         for (int w = tid; w < nc * Q; w += SQ_THREADS) {
             const int kk = w / Q, i = w - kk * Q, k = c0 + kk;
-            const uint32_t e = list[k];
-            const bool cover = (e >> 31) != 0;
+            const bool cover = lcov[k] != 0;
             const int p = lpos[k];
             const int ix = p % bwx, iy = (p / bwx) % bwy, iz = p / (bwx * bwy);
             const int x = boxc[0][ix], y = boxc[1][iy], z = boxc[2][iz];
@@ -391,9 +392,8 @@ __device__ void squirmer_block(const RemapArgs<S>& A) {
             const int i = lane;  // direction handled by this lane (lanes >= Q idle)
             for (int kk = 0; kk < nc; ++kk) {
                 const int k = c0 + kk;
-                const uint32_t e = list[k];
-                const uint32_t n = e & 0x7FFFFFFFu;
-                const bool cover = (e >> 31) != 0;
+                const uint32_t n = list[k];
+                const bool cover = lcov[k] != 0;
                 const int p = lpos[k];
                 const int ix = p % bwx, iy = (p / bwx) % bwy, iz = p / (bwx * bwy);
                 const double r[3] = {boxr[0][ix], boxr[1][iy], boxr[2][iz]};  // lbm.cpp:100-104
@@ -418,7 +418,7 @@ __device__ void squirmer_block(const RemapArgs<S>& A) {
                     if (i >= 1 && i < Q) {
                         const int j = gidx[kk][i];
                         if (j >= 0 && j < k) {  // changed earlier in the sweep
-                            isf = (list[j] >> 31) == 0;
+                            isf = lcov[j] == 0;
                             if (isf) rv = macro_round<S>(lrr[j]);
                         } else {
                             isf = gfl[kk][i] == FLUID;
@@ -481,9 +481,8 @@ __device__ void squirmer_block(const RemapArgs<S>& A) {
     S* __restrict__ f = A.f;
     for (int w = tid; w < K * Q; w += SQ_THREADS) {
         const int k = w / Q, i = w - k * Q;
-        const uint32_t e = list[k];
-        const uint32_t n = e & 0x7FFFFFFFu;
-        const bool cover = (e >> 31) != 0;
+        const uint32_t n = list[k];
+        const bool cover = lcov[k] != 0;
         const int x = (int)(n % (uint32_t)NX), y = (int)((n / (uint32_t)NX) % (uint32_t)NY),
                   z = (int)(n / ((uint32_t)NX * (uint32_t)NY));
         const uint32_t pn = pop_index(x, y, z, A.PX, NY);
@@ -511,14 +510,14 @@ __device__ void squirmer_block(const RemapArgs<S>& A) {
     }
     __syncthreads();
     for (int k = tid; k < K; k += SQ_THREADS) {
-        const uint32_t n = list[k] & 0x7FFFFFFFu;
+        const uint32_t n = list[k];
         A.flag[n] = A.flag[n] & TYPE_MASK;
     }
     __syncthreads();
     // BOUNDARY bits of the changed nodes and their neighbours
     for (int w = tid; w < K * Q; w += SQ_THREADS) {
         const int k = w / Q, i = w - k * Q;
-        const uint32_t n0 = list[k] & 0x7FFFFFFFu;
+        const uint32_t n0 = list[k];
         const int x0 = (int)(n0 % (uint32_t)NX), y0 = (int)((n0 / (uint32_t)NX) % (uint32_t)NY),
                   z0 = (int)(n0 / ((uint32_t)NX * (uint32_t)NY));
         const int x = step1(x0, CX[i], NX), y = step1(y0, CY[i], NY), z = step1(z0, CZ[i], NZ);

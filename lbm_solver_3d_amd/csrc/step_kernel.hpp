@@ -34,6 +34,7 @@
 #include <type_traits>
 
 #include "device_state.hpp"
+#include "fixed_point.hpp"
 #include "storage.hpp"
 
 namespace lbmhip {
@@ -290,38 +291,27 @@ __device__ inline bool in_link_box(const LinkBoxDims& b, int c, int y, int z, in
     return dy < b.wy && dz < b.wz && dc < b.nck;
 }
 
-// Fixed-point value of one node's excess mass (see MassPart).
-__device__ __forceinline__ unsigned __int128 mass_fixed(double e) {
-    const double fl = floor(e);
-    const unsigned long long lo = (unsigned long long)((e - fl) * 0x1p64);  // e - fl in [0, 1), exact
-    return ((unsigned __int128)(unsigned long long)(long long)fl << 64) | lo;
-}
-
-__device__ __forceinline__ unsigned __int128 mass_join(unsigned long long hi, unsigned long long lo) {
-    return ((unsigned __int128)hi << 64) | lo;
-}
-
-// Block sum (blocks of BLOCK threads) of the fixed-point excess; thread 0
-// stores it.
+// Block sum (blocks of BLOCK threads) of the nodes' excess mass in 128-bit
+// fixed point (fixed_point.hpp); thread 0 stores it.
 __device__ __forceinline__ void block_mass_sum(double excess, MassPart* out) {
     __shared__ unsigned long long lds[2 * (BLOCK / 64)];
-    unsigned __int128 v = mass_fixed(excess);
+    u128 v = fixed_from_double(excess);
     for (int off = 32; off > 0; off >>= 1) {
-        const unsigned long long h = __shfl_xor((unsigned long long)(v >> 64), off, 64);
-        const unsigned long long l = __shfl_xor((unsigned long long)v, off, 64);
-        v += mass_join(h, l);
+        const unsigned long long h = __shfl_xor((unsigned long long)fixed_hi(v), off, 64);
+        const unsigned long long l = __shfl_xor((unsigned long long)fixed_lo(v), off, 64);
+        v += fixed_join(h, l);
     }
     const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
-        lds[2 * w] = (unsigned long long)(v >> 64);
-        lds[2 * w + 1] = (unsigned long long)v;
+        lds[2 * w] = (unsigned long long)fixed_hi(v);
+        lds[2 * w + 1] = fixed_lo(v);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        unsigned __int128 s = 0;
-        for (int k = 0; k < BLOCK / 64; ++k) s += mass_join(lds[2 * k], lds[2 * k + 1]);
-        out->hi = (long long)(unsigned long long)(s >> 64);
-        out->lo = (unsigned long long)s;
+        u128 s = 0;
+        for (int k = 0; k < BLOCK / 64; ++k) s += fixed_join(lds[2 * k], lds[2 * k + 1]);
+        out->hi = fixed_hi(s);
+        out->lo = fixed_lo(s);
     }
 }
 

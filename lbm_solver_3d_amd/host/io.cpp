@@ -1,4 +1,6 @@
-// io.cpp -- output writers; formats follow the reference byte for byte.
+// io.cpp -- output writers on host arrays; formats follow the reference byte
+// for byte (io.cpp:44-97).  The overloads taking a class LBM (lbm_io.cpp)
+// refresh its host mirrors and call these.
 #include "io.hpp"
 
 #include <cstdint>
@@ -28,26 +30,27 @@ void ensureDirectory(const std::string& dir) {
     if (ec) throw std::runtime_error("cannot create output directory '" + dir + "': " + ec.message());
 }
 
-void writeVTK(const LBM& L, const std::string& path) {
-    L.syncHost(false);
+void writeVTKFields(const std::string& path, int NX, int NY, int NZ, const uint8_t* flag, const double* rho,
+                    const double* ux, const double* uy, const double* uz) {
+    const size_t N = (size_t)NX * NY * NZ;
     std::FILE* fp = std::fopen(path.c_str(), "wb");
     if (!fp) throw std::runtime_error("cannot write '" + path + "'");
     std::fprintf(fp, "# vtk DataFile Version 3.0\nlbm-squirmer\nBINARY\nDATASET STRUCTURED_POINTS\n");
-    std::fprintf(fp, "DIMENSIONS %d %d %d\nORIGIN 0 0 0\nSPACING 1 1 1\nPOINT_DATA %zu\n", L.NX, L.NY, L.NZ, L.N);
+    std::fprintf(fp, "DIMENSIONS %d %d %d\nORIGIN 0 0 0\nSPACING 1 1 1\nPOINT_DATA %zu\n", NX, NY, NZ, N);
     std::fprintf(fp, "SCALARS flag unsigned_char 1\nLOOKUP_TABLE default\n");
-    std::fwrite(L.flag.data(), 1, L.N, fp);
+    std::fwrite(flag, 1, N, fp);
     std::fprintf(fp, "\n");
-    std::vector<double> buf(L.N);
-    for (size_t n = 0; n < L.N; ++n) buf[n] = L.flag[n] == FLUID ? L.rho[n] : 1.0;
+    std::vector<double> buf(N);
+    for (size_t n = 0; n < N; ++n) buf[n] = flag[n] == FLUID ? rho[n] : 1.0;
     std::fprintf(fp, "SCALARS density double 1\nLOOKUP_TABLE default\n");
     put_be_doubles(fp, buf);
     std::fprintf(fp, "\n");
-    buf.assign(3 * L.N, 0.0);
-    for (size_t n = 0; n < L.N; ++n)
-        if (L.flag[n] == FLUID) {
-            buf[3 * n] = L.ux[n];
-            buf[3 * n + 1] = L.uy[n];
-            buf[3 * n + 2] = L.uz[n];
+    buf.assign(3 * N, 0.0);
+    for (size_t n = 0; n < N; ++n)
+        if (flag[n] == FLUID) {
+            buf[3 * n] = ux[n];
+            buf[3 * n + 1] = uy[n];
+            buf[3 * n + 2] = uz[n];
         }
     std::fprintf(fp, "VECTORS velocity double\n");
     put_be_doubles(fp, buf);
@@ -71,12 +74,11 @@ void TrajectoryWriter::write(int t, const Squirmer& s) {
                  s.X[2], s.e[0], s.e[1], s.e[2], s.U[0], s.U[1], s.U[2], s.W[0], s.W[1], s.W[2], s.Fw[1], s.Fw[2]);
 }
 
-void writeSquirmerState(const LBM& L, int t, const std::string& path) {
+void writeSquirmerStateOf(const Squirmer& s, double nu, int t, const std::string& path) {
     std::FILE* fp = std::fopen(path.c_str(), "w");
     if (!fp) throw std::runtime_error("cannot write '" + path + "'");
-    const Squirmer& s = L.sq;
     std::fprintf(fp, "t %d\nR %.10g\nB1 %.10g\nbeta %.10g\nU0 %.10g\nnu %.10g\nFS %.10g\n", t, s.R, s.B1, s.beta,
-                 s.U0(), L.nu, s.FS);
+                 s.U0(), nu, s.FS);
     std::fprintf(fp, "X %.10g %.10g %.10g\ne %.10g %.10g %.10g\n", s.X[0], s.X[1], s.X[2], s.e[0], s.e[1], s.e[2]);
     std::fprintf(fp, "U %.10g %.10g %.10g\nW %.10g %.10g %.10g\ncontacts %ld\n", s.U[0], s.U[1], s.U[2], s.W[0],
                  s.W[1], s.W[2], s.contacts);
