@@ -77,10 +77,16 @@ typedef struct lbm_squirmer_state {
     int32_t planar, plane_xz;
 } lbm_squirmer_state;
 
-/* LBM diagnostics (src/utils/lbm.cpp:342-364). */
+/* LBM diagnostics (src/utils/lbm.cpp:342-364), plus a stability check the
+ * reference does not make: unstable_nodes counts the FLUID nodes whose
+ * density or velocity is not finite, whose density lies outside
+ * (0, LBM_HIP_RHO_MAX) or whose speed reaches the lattice sound speed
+ * 1/sqrt(3) (a run past those bounds has lost its accuracy or diverged). */
+#define LBM_HIP_RHO_MAX 2.0
 typedef struct lbm_hip_diag {
     int64_t count_fluid;
     double mean_density, max_speed, sum_ux;
+    int64_t unstable_nodes;
 } lbm_hip_diag;
 
 /* Sizes and derived constants of a context. */
@@ -150,9 +156,10 @@ int lbm_hip_set_squirmer(lbm_hip_ctx* ctx, const lbm_squirmer_state* in);
  * step ran without LBM_STEP_WALL_FORCE. */
 int lbm_hip_get_wall_force(lbm_hip_ctx* ctx, double out[3]);
 
-/* countFluid / meanDensity / maxSpeed / sumUx (lbm.cpp:342-364). meanDensity,
- * maxSpeed and sumUx read the moments of the last step: LBM_HIP_ESTATE if it
- * ran without LBM_STEP_MACRO. */
+/* countFluid / meanDensity / maxSpeed / sumUx (lbm.cpp:342-364) and the
+ * count of unstable nodes (lbm_hip_diag), one pass of a device reduction.
+ * They read the moments of the last step: LBM_HIP_ESTATE if it ran without
+ * LBM_STEP_MACRO. */
 int lbm_hip_diagnostics(lbm_hip_ctx* ctx, lbm_hip_diag* out);
 
 /* Copy the public arrays of class LBM (lbm.hpp:71-73) to host buffers in the

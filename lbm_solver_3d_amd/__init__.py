@@ -71,7 +71,7 @@ class SquirmerState(C.Structure):
 
 class Diag(C.Structure):
     _fields_ = [("count_fluid", C.c_int64), ("mean_density", C.c_double), ("max_speed", C.c_double),
-                ("sum_ux", C.c_double)]
+                ("sum_ux", C.c_double), ("unstable_nodes", C.c_int64)]
 
 
 class Info(C.Structure):
@@ -315,7 +315,7 @@ class Lattice:
         d = Diag()
         _check(self.lib, self.lib.lbm_hip_diagnostics(self.h, C.byref(d)))
         return dict(count_fluid=int(d.count_fluid), mean_density=d.mean_density, max_speed=d.max_speed,
-                    sum_ux=d.sum_ux)
+                    sum_ux=d.sum_ux, unstable_nodes=int(d.unstable_nodes))
 
     def populations(self) -> np.ndarray:
         f = np.empty(Q * self.N)

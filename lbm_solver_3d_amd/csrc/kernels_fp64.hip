@@ -5,6 +5,7 @@
 #include <algorithm>
 
 #include "launch_step_impl.hpp"
+#include "lbm_hip.h"
 
 namespace lbmhip {
 
@@ -54,19 +55,23 @@ hipError_t launch_mass(const S* f, const uint8_t* flag, MassPart* part, int NX, 
     return hipGetLastError();
 }
 
-// f_i = w_i everywhere (lbm.cpp:82-84), in storage format S (slot padding too).
+// f_i = w_i everywhere (lbm.cpp:82-84), in storage format S (slot padding
+// too).  19 * SS elements can exceed the 2^32 work-items of one dispatch, so
+// a bounded grid strides over them.
 template <class S>
 __global__ __launch_bounds__(256) void fill_populations_kernel(S* f, uint32_t SS) {
-    const size_t e = (size_t)blockIdx.x * 256u + threadIdx.x;
-    if (e >= (size_t)Q * SS) return;
-    const int i = (int)(e / SS);
-    f[e] = from_real<S>(W[i], i);
+    const size_t tot = (size_t)Q * SS, stride = (size_t)gridDim.x * 256u;
+    for (size_t e = (size_t)blockIdx.x * 256u + threadIdx.x; e < tot; e += stride) {
+        const int i = (int)(e / SS);
+        f[e] = from_real<S>(W[i], i);
+    }
 }
 
 template <class S>
 hipError_t launch_fill_populations(S* f, uint32_t SS, hipStream_t s) {
-    const size_t tot = (size_t)Q * SS;
-    hipLaunchKernelGGL((fill_populations_kernel<S>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, f, SS);
+    const size_t want = ((size_t)Q * SS + 255) / 256;
+    const unsigned grid = (unsigned)std::min<size_t>(want, 1u << 20);
+    hipLaunchKernelGGL((fill_populations_kernel<S>), dim3(grid), dim3(256), 0, s, f, SS);
     return hipGetLastError();
 }
 
@@ -223,14 +228,37 @@ hipError_t launch_mass_finalize(const MassPart* part, int nparts, DevState* st, 
 
 // ---------------------------------------------------------------------------
 // Diagnostics (lbm.cpp:342-364): count of FLUID nodes, sum of rho, maximum
-// speed and sum of ux over FLUID nodes; per-block partials, then one block.
+// speed and sum of ux over FLUID nodes, and the count of unstable FLUID
+// nodes (non-finite moments, rho outside (0, LBM_HIP_RHO_MAX), speed at or
+// above 1/sqrt(3)); per-block partials, then one block.  The maximum
+// propagates a NaN speed (a comparison with NaN would drop it).
 // ---------------------------------------------------------------------------
+constexpr int NDIAG = 5;
+
+__device__ __forceinline__ double diag_max(double a, double b) { return (a != a || a > b) ? a : b; }
+
+__device__ __forceinline__ void diag_combine(double (&v)[NDIAG], const double* o) {
+    v[0] += o[0];
+    v[1] += o[1];
+    v[2] = diag_max(v[2], o[2]);
+    v[3] += o[3];
+    v[4] += o[4];
+}
+
+__device__ __forceinline__ void diag_wave(double (&v)[NDIAG]) {
+    for (int off = 32; off > 0; off >>= 1) {
+        double o[NDIAG];
+        for (int k = 0; k < NDIAG; ++k) o[k] = __shfl_xor(v[k], off, 64);
+        diag_combine(v, o);
+    }
+}
+
 __global__ __launch_bounds__(256) void diag_kernel(const uint8_t* flag, const void* rho, const void* ux,
                                                    const void* uy, const void* uz, int is_double, uint32_t N,
                                                    double* part) {
-    __shared__ double lds[4 * 4];
+    __shared__ double lds[4 * NDIAG];
     const uint32_t n = blockIdx.x * 256u + threadIdx.x;
-    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    double v[NDIAG] = {0.0, 0.0, 0.0, 0.0, 0.0};
     if (n < N && (flag[n] & TYPE_MASK) == FLUID) {
         double r, a, b, c;
         if (is_double) {
@@ -240,70 +268,107 @@ __global__ __launch_bounds__(256) void diag_kernel(const uint8_t* flag, const vo
             r = ((const float*)rho)[n]; a = ((const float*)ux)[n];
             b = ((const float*)uy)[n]; c = ((const float*)uz)[n];
         }
-        v[0] = 1.0; v[1] = r; v[2] = sqrt(a * a + b * b + c * c); v[3] = a;
+        const double u2 = a * a + b * b + c * c;
+        v[0] = 1.0; v[1] = r; v[2] = sqrt(u2); v[3] = a;
+        // the negated test also catches NaN: every comparison with NaN is false
+        const bool ok = r > 0.0 && r < LBM_HIP_RHO_MAX && u2 < 1.0 / 3.0;
+        v[4] = ok ? 0.0 : 1.0;
     }
-    for (int off = 32; off > 0; off >>= 1) {
-        v[0] += __shfl_xor(v[0], off, 64);
-        v[1] += __shfl_xor(v[1], off, 64);
-        const double m = __shfl_xor(v[2], off, 64);
-        v[2] = v[2] < m ? m : v[2];
-        v[3] += __shfl_xor(v[3], off, 64);
-    }
+    diag_wave(v);
     const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0)
-        for (int k = 0; k < 4; ++k) lds[w * 4 + k] = v[k];
+        for (int k = 0; k < NDIAG; ++k) lds[w * NDIAG + k] = v[k];
     __syncthreads();
     if (threadIdx.x == 0) {
-        double o[4] = {lds[0], lds[1], lds[2], lds[3]};
-        for (int ww = 1; ww < 4; ++ww) {
-            o[0] += lds[ww * 4 + 0];
-            o[1] += lds[ww * 4 + 1];
-            o[2] = o[2] < lds[ww * 4 + 2] ? lds[ww * 4 + 2] : o[2];
-            o[3] += lds[ww * 4 + 3];
-        }
-        for (int k = 0; k < 4; ++k) part[(size_t)blockIdx.x * 4 + k] = o[k];
+        double o[NDIAG];
+        for (int k = 0; k < NDIAG; ++k) o[k] = lds[k];
+        for (int ww = 1; ww < 4; ++ww) diag_combine(o, &lds[ww * NDIAG]);
+        for (int k = 0; k < NDIAG; ++k) part[(size_t)blockIdx.x * NDIAG + k] = o[k];
     }
 }
 
-__global__ __launch_bounds__(1024) void diag_finalize_kernel(const double* part, int nparts, double* out4) {
-    __shared__ double lds[16 * 4];
-    double v[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int k = threadIdx.x; k < nparts; k += 1024) {
-        v[0] += part[(size_t)k * 4 + 0];
-        v[1] += part[(size_t)k * 4 + 1];
-        v[2] = v[2] < part[(size_t)k * 4 + 2] ? part[(size_t)k * 4 + 2] : v[2];
-        v[3] += part[(size_t)k * 4 + 3];
-    }
-    for (int off = 32; off > 0; off >>= 1) {
-        v[0] += __shfl_xor(v[0], off, 64);
-        v[1] += __shfl_xor(v[1], off, 64);
-        const double m = __shfl_xor(v[2], off, 64);
-        v[2] = v[2] < m ? m : v[2];
-        v[3] += __shfl_xor(v[3], off, 64);
-    }
+__global__ __launch_bounds__(1024) void diag_finalize_kernel(const double* part, int nparts, double* out) {
+    __shared__ double lds[16 * NDIAG];
+    double v[NDIAG] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int k = threadIdx.x; k < nparts; k += 1024) diag_combine(v, &part[(size_t)k * NDIAG]);
+    diag_wave(v);
     const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0)
-        for (int k = 0; k < 4; ++k) lds[w * 4 + k] = v[k];
+        for (int k = 0; k < NDIAG; ++k) lds[w * NDIAG + k] = v[k];
     __syncthreads();
     if (threadIdx.x == 0) {
-        double o[4] = {lds[0], lds[1], lds[2], lds[3]};
-        for (int ww = 1; ww < 16; ++ww) {
-            o[0] += lds[ww * 4 + 0];
-            o[1] += lds[ww * 4 + 1];
-            o[2] = o[2] < lds[ww * 4 + 2] ? lds[ww * 4 + 2] : o[2];
-            o[3] += lds[ww * 4 + 3];
-        }
-        for (int k = 0; k < 4; ++k) out4[k] = o[k];
+        double o[NDIAG];
+        for (int k = 0; k < NDIAG; ++k) o[k] = lds[k];
+        for (int ww = 1; ww < 16; ++ww) diag_combine(o, &lds[ww * NDIAG]);
+        for (int k = 0; k < NDIAG; ++k) out[k] = o[k];
     }
 }
 
 hipError_t launch_diag(const uint8_t* flag, const void* rho, const void* ux, const void* uy, const void* uz,
-                       int is_double, uint32_t N, double* part, int nparts, double* out4, hipStream_t s) {
+                       int is_double, uint32_t N, double* part, int nparts, double* out5, hipStream_t s) {
     hipLaunchKernelGGL(diag_kernel, dim3(nparts), dim3(256), 0, s, flag, rho, ux, uy, uz, is_double, N, part);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(diag_finalize_kernel, dim3(1), dim3(1024), 0, s, part, nparts, out4);
+    hipLaunchKernelGGL(diag_finalize_kernel, dim3(1), dim3(1024), 0, s, part, nparts, out5);
     return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Initial node flags (lbm.cpp:72-93) on the device: WALL layers, the
+// PARTICLE nodes of the initial sphere (classified on the host over the
+// sphere's box, uploaded as a list), then the BOUNDARY bit of every FLUID
+// node with a non-FLUID neighbour.  One thread per node.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void init_flags_kernel(uint8_t* flag, int NX, int NY, int NZ, uint32_t N,
+                                                         int walls_y, int walls_z) {
+    const uint32_t n = blockIdx.x * 256u + threadIdx.x;
+    if (n >= N) return;
+    const uint32_t r = n / (uint32_t)NX;
+    const int y = (int)(r % (uint32_t)NY), z = (int)(r / (uint32_t)NY);
+    const bool wall = (walls_y && (y == 0 || y == NY - 1)) || (walls_z && (z == 0 || z == NZ - 1));
+    flag[n] = wall ? (uint8_t)WALL : (uint8_t)FLUID;
+}
+
+__global__ __launch_bounds__(256) void set_flags_kernel(uint8_t* flag, const uint32_t* nodes, int count,
+                                                        uint8_t value) {
+    const int k = (int)(blockIdx.x * 256u + threadIdx.x);
+    if (k < count) flag[nodes[k]] = value;
+}
+
+__global__ __launch_bounds__(256) void boundary_bits_kernel(uint8_t* flag, int NX, int NY, int NZ, uint32_t N) {
+    const uint32_t n = blockIdx.x * 256u + threadIdx.x;
+    if (n >= N || flag[n] != FLUID) return;
+    const uint32_t r = n / (uint32_t)NX;
+    const int x = (int)(n - r * (uint32_t)NX), y = (int)(r % (uint32_t)NY), z = (int)(r / (uint32_t)NY);
+    const int xs[3] = {x == 0 ? NX - 1 : x - 1, x, x + 1 == NX ? 0 : x + 1};
+    const int ys[3] = {y == 0 ? NY - 1 : y - 1, y, y + 1 == NY ? 0 : y + 1};
+    const int zs[3] = {z == 0 ? NZ - 1 : z - 1, z, z + 1 == NZ ? 0 : z + 1};
+    bool b = false;
+#pragma unroll
+    for (int i = 1; i < Q; ++i) {
+        const uint32_t m = (uint32_t)xs[1 + CX[i]] + (uint32_t)NX * ((uint32_t)ys[1 + CY[i]] + (uint32_t)NY * (uint32_t)zs[1 + CZ[i]]);
+        b |= (flag[m] & TYPE_MASK) != FLUID;
+    }
+    // only this node's own byte is written; neighbours are read-only here
+    // (the BOUNDARY bit lives in bits the type test masks off)
+    if (b) flag[n] = (uint8_t)(FLUID | BOUNDARY);
+}
+
+hipError_t launch_init_flags(uint8_t* flag, int NX, int NY, int NZ, uint32_t N, int walls_y, int walls_z,
+                             const uint32_t* particle_nodes, int nparticle, hipStream_t s) {
+    const unsigned blocks = (unsigned)(((size_t)N + 255) / 256);
+    hipLaunchKernelGGL(init_flags_kernel, dim3(blocks), dim3(256), 0, s, flag, NX, NY, NZ, N, walls_y, walls_z);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess && nparticle > 0) {
+        hipLaunchKernelGGL(set_flags_kernel, dim3((unsigned)((nparticle + 255) / 256)), dim3(256), 0, s, flag,
+                           particle_nodes, nparticle, (uint8_t)PARTICLE);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(boundary_bits_kernel, dim3(blocks), dim3(256), 0, s, flag, NX, NY, NZ, N);
+        e = hipGetLastError();
+    }
+    return e;
 }
 
 __global__ __launch_bounds__(256) void fill_macro_kernel(void* rho, void* ux, void* uy, void* uz, int is_double,

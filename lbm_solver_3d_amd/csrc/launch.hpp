@@ -66,9 +66,14 @@ constexpr int MASS_TERM_LINKS = 1;  // moving-boundary corrections of the last s
 constexpr int MASS_TERM_REMAP = 2;  // remaps since the last step (remap_dmass)
 hipError_t launch_mass_finalize(const MassPart* part, int nparts, DevState* st, int terms, int shift_slot,
                                 hipStream_t s);
+// Diagnostics: out5 = {fluid nodes, sum rho, max speed, sum ux, unstable nodes};
+// `part` holds 5 doubles per block of 256 nodes.
 hipError_t launch_diag(const uint8_t* flag, const void* rho, const void* ux, const void* uy, const void* uz,
-                       int is_double, uint32_t N, double* part, int nparts, double* out4, hipStream_t s);
+                       int is_double, uint32_t N, double* part, int nparts, double* out5, hipStream_t s);
 hipError_t launch_fill_macro(void* rho, void* ux, void* uy, void* uz, int is_double, uint32_t N, hipStream_t s);
+// Initial flags: WALL layers, the listed PARTICLE nodes, BOUNDARY bits.
+hipError_t launch_init_flags(uint8_t* flag, int NX, int NY, int NZ, uint32_t N, int walls_y, int walls_z,
+                             const uint32_t* particle_nodes, int nparticle, hipStream_t s);
 hipError_t launch_macro_to_double(const void* src, int is_double, double* dst, uint32_t N, hipStream_t s);
 hipError_t launch_copy_probe(const void* a, void* b, size_t n16, hipStream_t s);
 

@@ -64,8 +64,8 @@ struct lbm_hip_ctx {
     void* uz = nullptr;
     DevState* st = nullptr;
     MassPart* mass_part = nullptr;  // >= max(G, ceil(N/256)) fixed-point partials
-    double* diag_part = nullptr;  // 4 * ceil(N/256) doubles
-    double* diag_out = nullptr;   // 4 doubles
+    double* diag_part = nullptr;  // 5 * ceil(N/256) doubles
+    double* diag_out = nullptr;   // 5 doubles
     double* red_part = nullptr;   // 8 * max(LINK_MAX_BLOCKS, ceil(N/256)) doubles (link / wall kernels)
     int link_blocks = 0;
     uint32_t* wall_rows = nullptr;  // rows next to a wall (even-step boundary chunks)
@@ -383,6 +383,11 @@ int lbm_hip_create(const lbm_hip_params* pin, lbm_hip_ctx** out) {
 
     // tiles: one wavefront row-chunk along x, block of 256 threads
     for (int ts : {0, 6, 7, 8}) c->G_max = std::max(c->G_max, tiles_for(pin->nx, pin->ny, pin->nz, ts));
+    // one dispatch holds fewer than 2^32 work-items
+    if ((double)c->G_max * BLOCK >= 4294967296.0) {
+        delete c;
+        return fail(LBM_HIP_EINVAL, "grid too large: its collide-stream tiles exceed 2^32 work-items per launch");
+    }
     set_tile_shape(c, pin->nx <= 64 ? 6 : (pin->nx <= 128 ? 7 : 8));
     c->nb256 = (int)(((size_t)c->N + 255) / 256);
 
@@ -434,48 +439,51 @@ int lbm_hip_create(const lbm_hip_params* pin, lbm_hip_ctx** out) {
         !dmalloc(&c->rho, N * c->msize) || !dmalloc(&c->ux, N * c->msize) || !dmalloc(&c->uy, N * c->msize) ||
         !dmalloc(&c->uz, N * c->msize) || !dmalloc((void**)&c->st, sizeof(DevState)) ||
         !dmalloc((void**)&c->mass_part, sizeof(MassPart) * nparts) ||
-        !dmalloc((void**)&c->diag_part, sizeof(double) * 4 * c->nb256) ||
-        !dmalloc((void**)&c->diag_out, sizeof(double) * 4) ||
+        !dmalloc((void**)&c->diag_part, sizeof(double) * 5 * c->nb256) ||
+        !dmalloc((void**)&c->diag_out, sizeof(double) * 5) ||
         !dmalloc((void**)&c->red_part, sizeof(double) * 8 * std::max(LINK_MAX_BLOCKS, c->nb256)) ||
         !dmalloc((void**)&c->wall_rows, sizeof(uint32_t) * std::max<size_t>(1, wall_rows.size())))
         return cleanup(fail(LBM_HIP_ENOMEM, "device allocation failed"));
 
-    // flags: wall layers (lbm.cpp:72-78)
-    std::vector<uint8_t> flag(N);
+    // flags (lbm.cpp:72-93): WALL layers and BOUNDARY bits are set by a
+    // device kernel; the host classifies the initial sphere's box and
+    // uploads its PARTICLE nodes.  FLUID count: the nodes of the rows that
+    // are not wall layers, less the PARTICLE nodes.
+    auto wall_at = [&](int y, int z) {
+        return (pin->walls_y && (y == 0 || y == c->NY - 1)) || (pin->walls_z && (z == 0 || z == c->NZ - 1));
+    };
+    long long fluid_rows = 0;
     for (int z = 0; z < c->NZ; ++z)
-        for (int y = 0; y < c->NY; ++y)
-            for (int x = 0; x < c->NX; ++x) {
-                const bool wall = (pin->walls_y && (y == 0 || y == c->NY - 1)) ||
-                                  (pin->walls_z && (z == 0 || z == c->NZ - 1));
-                flag[(size_t)x + (size_t)c->NX * ((size_t)y + (size_t)c->NY * z)] = wall ? WALL : FLUID;
-            }
+        for (int y = 0; y < c->NY; ++y) fluid_rows += wall_at(y, z) ? 0 : 1;
+    std::vector<uint32_t> particle_nodes;
     DevState hs;
     std::memset(&hs, 0, sizeof hs);
     c->has_squirmer = pin->squirmer != 0;
     if (c->has_squirmer) {
         init_squirmer(hs.sq, *pin, c->nu, pin->x0 >= 0 ? pin->x0 : 0.5 * (c->NX - 1),
                       pin->y0 >= 0 ? pin->y0 : 0.5 * (c->NY - 1), pin->z0 >= 0 ? pin->z0 : 0.5 * (c->NZ - 1));
-        // updateParticleFlags(true) (lbm.cpp:271-291)
+        // updateParticleFlags(true) (lbm.cpp:271-291): the box is cut to one
+        // period per axis, so no node is listed twice
         const DevSquirmer& s = hs.sq;
         const double R = s.R;
         const int x0 = (int)std::floor(s.X[0] - R) - 2, x1 = (int)std::ceil(s.X[0] + R) + 2;
         int y0 = (int)std::floor(s.X[1] - R) - 2, y1 = (int)std::ceil(s.X[1] + R) + 2;
         const int z0 = (int)std::floor(s.X[2] - R) - 2, z1 = (int)std::ceil(s.X[2] + R) + 2;
         if (pin->walls_y) { y0 = std::max(0, y0); y1 = std::min(c->NY - 1, y1); }
-        for (int zz = z0; zz <= z1; ++zz)
-            for (int yy = y0; yy <= y1; ++yy)
-                for (int xx = x0; xx <= x1; ++xx) {
-                    const int x = ((xx % c->NX) + c->NX) % c->NX;
-                    const int y = ((yy % c->NY) + c->NY) % c->NY;
-                    const int z = ((zz % c->NZ) + c->NZ) % c->NZ;
-                    const size_t n = (size_t)x + (size_t)c->NX * ((size_t)y + (size_t)c->NY * z);
-                    if (flag[n] == WALL) continue;
+        const int wx = std::min(x1 - x0 + 1, c->NX), wy = std::min(y1 - y0 + 1, c->NY), wz = std::min(z1 - z0 + 1, c->NZ);
+        for (int kz = 0; kz < wz; ++kz)
+            for (int ky = 0; ky < wy; ++ky)
+                for (int kx = 0; kx < wx; ++kx) {
+                    const int x = (((x0 + kx) % c->NX) + c->NX) % c->NX;
+                    const int y = (((y0 + ky) % c->NY) + c->NY) % c->NY;
+                    const int z = (((z0 + kz) % c->NZ) + c->NZ) % c->NZ;
+                    if (wall_at(y, z)) continue;
                     double r[3];
                     r[0] = x - s.X[0]; r[0] -= c->NX * std::round(r[0] / c->NX);
                     r[1] = y - s.X[1]; if (!pin->walls_y) r[1] -= c->NY * std::round(r[1] / c->NY);
                     r[2] = z - s.X[2]; if (!pin->walls_z) r[2] -= c->NZ * std::round(r[2] / c->NZ);
-                    const bool inside = r[0] * r[0] + r[1] * r[1] + r[2] * r[2] < R * R;
-                    flag[n] = inside ? PARTICLE : FLUID;
+                    if (r[0] * r[0] + r[1] * r[1] + r[2] * r[2] < R * R)
+                        particle_nodes.push_back((uint32_t)x + (uint32_t)c->NX * ((uint32_t)y + (uint32_t)c->NY * (uint32_t)z));
                 }
         for (int d = 0; d < 3; ++d) hs.xf[d] = s.X[d];
         hs.rf = R;
@@ -486,24 +494,20 @@ int lbm_hip_create(const lbm_hip_params* pin, lbm_hip_ctx** out) {
         hs.sq.R = 4.0; hs.sq.B1 = 0.015; hs.sq.planar = 1; hs.sq.e[0] = 1.0;
         hs.sq.eps = 0.2; hs.sq.hc = 2.0; hs.sq.hmin = 0.5;
     }
-    // BOUNDARY bits and fluid count
-    long long nf = 0;
-    std::vector<uint8_t> dflag(flag);
-    for (int z = 0; z < c->NZ; ++z)
-        for (int y = 0; y < c->NY; ++y)
-            for (int x = 0; x < c->NX; ++x) {
-                const size_t n = (size_t)x + (size_t)c->NX * ((size_t)y + (size_t)c->NY * z);
-                if (flag[n] != FLUID) continue;
-                ++nf;
-                bool b = false;
-                for (int i = 1; i < Q && !b; ++i) {
-                    const int xn = (x + CX[i] + c->NX) % c->NX, yn = (y + CY[i] + c->NY) % c->NY,
-                              zn = (z + CZ[i] + c->NZ) % c->NZ;
-                    b = flag[(size_t)xn + (size_t)c->NX * ((size_t)yn + (size_t)c->NY * zn)] != FLUID;
-                }
-                if (b) dflag[n] |= BOUNDARY;
-            }
-    hs.nfluid = nf;
+    hs.nfluid = fluid_rows * c->NX - (long long)particle_nodes.size();
+    uint32_t* dparticles = nullptr;
+    if (!particle_nodes.empty() &&
+        (hipMalloc((void**)&dparticles, sizeof(uint32_t) * particle_nodes.size()) != hipSuccess ||
+         hipMemcpyAsync(dparticles, particle_nodes.data(), sizeof(uint32_t) * particle_nodes.size(),
+                        hipMemcpyHostToDevice, c->stream) != hipSuccess)) {
+        if (dparticles) hipFree(dparticles);
+        return cleanup(fail(LBM_HIP_ENOMEM, "context initialisation: particle list"));
+    }
+    const hipError_t fe = launch_init_flags(c->flag, c->NX, c->NY, c->NZ, c->N, pin->walls_y, pin->walls_z, dparticles,
+                                            (int)particle_nodes.size(), c->stream);
+    const hipError_t se = hipStreamSynchronize(c->stream);  // the list is freed next
+    if (dparticles) hipFree(dparticles);
+    if (fe != hipSuccess || se != hipSuccess) return cleanup(fail(LBM_HIP_EDEVICE, "context initialisation: flags"));
 
     c->bargs.wall_rows = c->wall_rows;
     c->bargs.mass_part = c->mass_part + c->G;
@@ -511,8 +515,7 @@ int lbm_hip_create(const lbm_hip_params* pin, lbm_hip_ctx** out) {
         hipMemcpyAsync(c->wall_rows, wall_rows.data(), sizeof(uint32_t) * wall_rows.size(), hipMemcpyHostToDevice,
                        c->stream) != hipSuccess)
         return cleanup(fail(LBM_HIP_EDEVICE, "context initialisation failed"));
-    if (hipMemcpyAsync(c->flag, dflag.data(), N, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
-        hipMemcpyAsync(c->st, &hs, sizeof hs, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+    if (hipMemcpyAsync(c->st, &hs, sizeof hs, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
         launch_fill_macro(c->rho, c->ux, c->uy, c->uz, c->prec == LBM_HIP_FP64, c->N, c->stream) != hipSuccess ||
         dispatch(c->prec, [&](auto* tag) -> int {
             using S = std::remove_pointer_t<decltype(tag)>;
@@ -768,13 +771,14 @@ int lbm_hip_diagnostics(lbm_hip_ctx* c, lbm_hip_diag* o) {
     if (!c->macro_valid) return fail(LBM_HIP_ESTATE, "the last step did not write the macroscopic fields");
     HIP_TRY(launch_diag(c->flag, c->rho, c->ux, c->uy, c->uz, c->prec == LBM_HIP_FP64, c->N, c->diag_part, c->nb256,
                         c->diag_out, c->stream));
-    double h[4];
+    double h[5];
     HIP_TRY(hipMemcpyAsync(h, c->diag_out, sizeof h, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     o->count_fluid = (int64_t)h[0];
     o->mean_density = h[0] > 0 ? h[1] / h[0] : 0.0;
     o->max_speed = h[2];
     o->sum_ux = h[3];
+    o->unstable_nodes = (int64_t)h[4];
     return LBM_HIP_OK;
 }
 

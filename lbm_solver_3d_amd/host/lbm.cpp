@@ -146,5 +146,10 @@ double LBM::sumUx() const {
     check(lbm_hip_diagnostics(ctx_, &d));
     return d.sum_ux;
 }
+long LBM::unstableNodes() const {
+    lbm_hip_diag d;
+    check(lbm_hip_diagnostics(ctx_, &d));
+    return (long)d.unstable_nodes;
+}
 
 }  // namespace lbm

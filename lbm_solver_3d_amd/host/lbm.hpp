@@ -69,6 +69,9 @@ public:
     Squirmer sq;
 
     // ---- this build ----
+    // FLUID nodes of the last step's moments with a non-finite or
+    // out-of-range density or velocity (lbm_hip_diag::unstable_nodes).
+    long unstableNodes() const;
     // Refresh the host mirrors f, rho, ux, uy, uz, flag (fnew mirrors f).
     void syncHost(bool populations = true) const;
     // Upload host-side changes of sq to the device.

@@ -73,6 +73,11 @@ int main(int argc, char** argv) {
             if (p.vtkEvery > 0 && t % p.vtkEvery == 0) writeVTK(L, base + "_" + std::to_string(t) + ".vtk");
             if (t % p.printEvery == 0) {
                 L.syncState();
+                // a diverged run stops here instead of printing NaN to the end
+                if (const long bad = L.unstableNodes())
+                    throw std::runtime_error("numerical instability at step " + std::to_string(t) + ": " +
+                                             std::to_string(bad) +
+                                             " fluid nodes with a non-finite or out-of-range density or velocity");
                 if (L.hasSquirmer) {
                     const Squirmer& s = L.sq;
                     std::printf("t=%8d  X=(%9.2f,%7.2f,%7.2f)  angle=%7.2f deg  |U|/U0=%.3f  <rho>=%.6f\n", t, s.X[0],

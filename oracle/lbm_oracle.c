@@ -380,18 +380,58 @@ void orc_advance_squirmer(orc_lbm* L) {
     update_particle_flags(L, 0);
 }
 
+/* Summation of the mass correction: 0 = the reference's (lbm.cpp:325-331,
+ * m += f into a double, per OpenMP thread); 1 = exact.  The reference's
+ * partial sums grow to ~N / threads, whose ulp exceeds the per-node
+ * deviations from rho = 1, and the far-field populations are all w_i, so
+ * every addition drops the same low bits: at 16.7M nodes the shift d is off
+ * by ~1e-10 per correction, by an amount that depends on the thread count.
+ * The exact mode sums each node's excess (sum_i f_i) - 1 as a 128-bit
+ * fixed-point integer (trunc(e * 2^64), as fixed_point.hpp of the product
+ * does) and sets d = -excess / n_fluid: the same algorithm without that
+ * rounding. */
+static int g_mass_sum_exact = 0;
+void orc_set_mass_sum_exact(int exact) { g_mass_sum_exact = exact; }
+
+static __int128 fixed_of(double v) {
+    if (v == 0.0 || !isfinite(v)) return 0;
+    int ex;
+    const double m = frexp(fabs(v), &ex); /* |v| = m 2^ex, m in [0.5, 1) */
+    const uint64_t mant = (uint64_t)ldexp(m, 53); /* exact: 53-bit significand */
+    const int sh = ex - 53 + 64;                  /* |v| 2^64 = mant 2^sh */
+    unsigned __int128 a;
+    if (sh >= 0) a = sh > 74 ? (((unsigned __int128)1 << 127) - 1) : ((unsigned __int128)mant << sh);
+    else a = sh <= -64 ? 0 : (unsigned __int128)(mant >> (-sh));
+    return v < 0 ? -(__int128)a : (__int128)a;
+}
+
 /* lbm.cpp:324-339 */
 void orc_correct_mass(orc_lbm* L) {
     const size_t N = L->N;
     double m = 0; long nf = 0;
+    double d;
+    if (g_mass_sum_exact) {
+        __int128 ex = 0;
+        for (long n = 0; n < (long)N; ++n) {
+            if (L->flag[n] != FLUID) continue;
+            double s = 0.0;
+            for (int i = 0; i < Q; ++i) s += L->f[(size_t)i * N + n];
+            ex += fixed_of(s - 1.0);
+            ++nf;
+        }
+        if (nf == 0) return;
+        const double hi = (double)(int64_t)(ex >> 64), lo = (double)(uint64_t)ex;
+        d = -(hi + lo * 0x1p-64) / (double)nf;
+    } else {
 #pragma omp parallel for reduction(+ : m, nf)
-    for (long n = 0; n < (long)N; ++n) {
-        if (L->flag[n] != FLUID) continue;
-        for (int i = 0; i < Q; ++i) m += L->f[(size_t)i * N + n];
-        ++nf;
+        for (long n = 0; n < (long)N; ++n) {
+            if (L->flag[n] != FLUID) continue;
+            for (int i = 0; i < Q; ++i) m += L->f[(size_t)i * N + n];
+            ++nf;
+        }
+        if (nf == 0) return;
+        d = 1.0 - m / nf;
     }
-    if (nf == 0) return;
-    const double d = 1.0 - m / nf;
 #pragma omp parallel for
     for (long n = 0; n < (long)N; ++n) {
         if (L->flag[n] != FLUID) continue;

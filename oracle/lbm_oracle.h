@@ -71,6 +71,9 @@ void orc_destroy(orc_lbm* L);
 void orc_step(orc_lbm* L);
 void orc_advance_squirmer(orc_lbm* L);
 void orc_correct_mass(orc_lbm* L);
+/* 0 (default): the mass correction sums as the reference does; 1: exactly
+ * (see lbm_oracle.c).  Process-wide. */
+void orc_set_mass_sum_exact(int exact);
 
 long orc_count_fluid(const orc_lbm* L);
 double orc_mean_density(const orc_lbm* L);

@@ -118,6 +118,12 @@ def params(**kv) -> OrcParams:
     return p
 
 
+def set_mass_sum_exact(exact: bool) -> None:
+    """Mass-correction summation of the oracle: the reference's double sum
+    (default) or the exact fixed-point sum (orc_set_mass_sum_exact)."""
+    _load().orc_set_mass_sum_exact(C.c_int(1 if exact else 0))
+
+
 def set_threads(n: int) -> None:
     """OpenMP threads for the oracle (and the reference build)."""
     os.environ["OMP_NUM_THREADS"] = str(n)

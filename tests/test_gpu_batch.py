@@ -126,3 +126,80 @@ def test_single_autotune_keeps_state(lbm):
         t = L.autotune(trial_steps=4, mass_every=10)
         assert t["tile_width"] in (0, 64, 128, 256)
         _same(a, _state(L))
+
+
+# Members for the oracle comparison: walls in y, walls in z (x-z plane),
+# fully periodic, and one without a squirmer.
+ORACLE_MEMBERS = [
+    dict(nx=40, ny=22, nz=16, walls_y=1, walls_z=0, squirmer=1, radius=4.0, b1=0.03, beta=-1.0, angle=-30.0, y0=8.0,
+         planar=1, hmin=0.5),
+    dict(nx=36, ny=20, nz=22, walls_y=0, walls_z=1, squirmer=1, radius=4.5, b1=0.02, beta=-3.0, angle=-25.0,
+         planar=1, plane_xz=1, z0=9.0),
+    dict(nx=24, ny=24, nz=24, walls_y=0, walls_z=0, squirmer=1, radius=4.0, b1=0.015, beta=-3.0, angle=20.0, planar=1),
+    dict(nx=4, ny=10, nz=12, tau=0.8, fx=5e-5, walls_y=1, walls_z=1),
+]
+
+
+def test_batch_members_match_oracle_fp64(oracle_mod, lbm):
+    """Every member of an FP64 batch against the oracle run alone (the
+    reference's driver loop, main.cpp:52-87 over lbm.cpp:128-339): 160
+    steps with a mass correction every 10, graph-replayed.  Populations to
+    1e-12 relative (the force sums run in another order) and flags exactly;
+    the member without a squirmer bit for bit."""
+    steps = 160
+    with lbm.Batch(ORACLE_MEMBERS, precision="fp64") as B:
+        B.run(steps, 10, lbm.STEP_ALL)
+        got = []
+        for i in range(len(ORACLE_MEMBERS)):
+            L = B.member(i)
+            fl = L.flags()
+            got.append(dict(flag=fl, f=L.populations(), sq=L.squirmer() if L.has_squirmer else None,
+                            rho=L.fields(("rho",))["rho"]))
+    for kv, g in zip(ORACLE_MEMBERS, got):
+        o = oracle_mod.Oracle(**kv)
+        o.run(steps, 10)
+        assert np.array_equal(g["flag"], o.flags())
+        m = g["flag"] == 0
+        fo, fg = o.f()[:, m], g["f"][:, m]
+        if not kv.get("squirmer"):
+            assert np.array_equal(fg, fo)
+            assert np.array_equal(g["rho"][m], o.field("rho")[m])
+            continue
+        err = np.abs(fg - fo).max() / np.abs(fo).max()
+        assert err < 1e-12, f"population rel err {err:.3e}"
+        so = o.state()
+        for k in ("X", "e"):
+            assert np.allclose(g["sq"][k], so[k], rtol=0, atol=1e-12), k
+        for k in ("U", "W"):
+            assert np.abs(g["sq"][k] - so[k]).max() <= 1e-9 * max(np.abs(so[k]).max(), 1e-30) + 1e-18, k
+        assert g["sq"]["contacts"] == so["contacts"]
+        # the remaps moved the sphere off its initial nodes
+        assert not np.array_equal(g["flag"], oracle_mod.Oracle(**kv).flags())
+
+
+def test_c4_ensemble_full_size(lbm):
+    """The 64 slit-channel runs of seed 0 (200.2M nodes, FP16 store) in one
+    batch, 40 graph-replayed steps and one with moments: every member stays
+    finite at unit mean density, and the smallest, the largest and one R = 8
+    member equal the same lattice run alone, bit for bit."""
+    members = lbm.ensemble_members(64, 0)
+    H = [m["nz"] for m in members]
+    pick = sorted({int(np.argmin(H)), int(np.argmax(H)),
+                   next(i for i, m in enumerate(members) if m["radius"] == 8.0)})
+    steps = 41
+    with lbm.Batch(members, precision="fp16") as B:
+        info = B.info()
+        assert info["count"] == 64 and info["total_nodes"] == sum(m["nx"] * m["ny"] * m["nz"] for m in members)
+        B.run(steps, 10, lbm.STEP_MACRO)
+        for i in range(64):
+            d = B.member(i).diagnostics()
+            assert d["unstable_nodes"] == 0, i
+            assert np.isfinite(d["max_speed"]) and d["max_speed"] > 0, i
+            assert abs(d["mean_density"] - 1.0) < 1e-5, (i, d["mean_density"])
+            s = B.member(i).squirmer()
+            assert np.all(np.isfinite(s["X"])) and np.all(np.isfinite(s["U"])), i
+        got = {i: _state(B.member(i)) for i in pick}
+    for i in pick:
+        with lbm.Lattice(**members[i], precision="fp16") as L:
+            L.run(steps, 10, lbm.STEP_MACRO)
+            _same(got[i], _state(L))

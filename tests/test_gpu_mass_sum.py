@@ -76,3 +76,45 @@ def test_set_squirmer_rejects_larger_radius(lbm):
         L.set_squirmer(R=3.5)
         L.run(4, 0, 0)
         L.synchronize()
+
+
+def test_autotune_between_mass_step_and_correction(lbm):
+    """Autotuning after a step with mass partials and before its
+    correct_mass: the trials overwrite the partials, so the correction must
+    sum the populations instead; the result equals the same calls without
+    the autotune to the rounding of the two sums."""
+    with lbm.Lattice(**SLIT, precision="fp64") as A, lbm.Lattice(**SLIT, precision="fp64") as B:
+        for L in (A, B):
+            L.run(29, 10, 0)
+            L.step(lbm.STEP_MASS)
+            L.advance_squirmer()
+        A.autotune(trial_steps=3, mass_every=10)
+        for L in (A, B):
+            L.correct_mass()
+            L.step(lbm.STEP_ALL)
+        m = A.flags() == 0
+        assert np.array_equal(A.flags(), B.flags())
+        fa, fb = A.populations()[:, m], B.populations()[:, m]
+        assert np.abs(fa - fb).max() <= 1e-13
+        assert A.diagnostics()["mean_density"] == pytest.approx(1.0, abs=1e-12)
+
+
+def test_unstable_nodes_counted(lbm):
+    """A population set to NaN or to a large value makes its node (and,
+    after a step, its neighbours) count as unstable; a healthy lattice
+    counts none."""
+    kv = dict(nx=16, ny=12, nz=10, walls_y=1, walls_z=0, precision="fp64")
+    with lbm.Lattice(**kv) as L:
+        L.run(3, 0, lbm.STEP_ALL)
+        assert L.diagnostics()["unstable_nodes"] == 0
+        f = L.populations()
+        fl = L.flags()
+        n_nan = int(np.flatnonzero(fl == 0)[5])
+        n_big = int(np.flatnonzero(fl == 0)[-7])
+        f[0, n_nan] = np.nan
+        f[0, n_big] = 5.0
+        L.upload_populations(f)
+        L.step(lbm.STEP_ALL)
+        d = L.diagnostics()
+        assert d["unstable_nodes"] >= 2
+        assert not np.isfinite(d["max_speed"]) or not np.isfinite(d["mean_density"])

@@ -396,3 +396,27 @@ def test_plane_xz_extension_is_relabelled_xy(oracle_mod):
     assert sa["W"][2] == pytest.approx(-sb["W"][1], rel=1e-10)   # rotation about z <-> about -y
     assert abs(sa["W"][2]) > 1e-6
     assert sa["contacts"] == sb["contacts"]
+
+
+def test_exact_mass_sum_mode_agrees_on_small_lattices(oracle_mod):
+    """The oracle's exact-sum mass correction (used where the reference's
+    double sum rounds visibly, at millions of nodes) equals the reference's
+    summation to round-off on a small lattice with remaps and corrections."""
+    kv = dict(nx=40, ny=22, nz=16, walls_y=1, walls_z=0, squirmer=1, radius=4.0, b1=0.03, beta=-1.0,
+              angle=-40.0, y0=6.2, hmin=0.5)
+    out = []
+    for exact in (False, True):
+        oracle_mod.set_mass_sum_exact(exact)
+        try:
+            o = oracle_mod.Oracle(**kv)
+            o.run(120, 10)
+            out.append((o.f().copy(), o.flags().copy(), np.array(o.state()["X"])))
+        finally:
+            oracle_mod.set_mass_sum_exact(False)
+    (fa, la, xa), (fb, lb, xb) = out
+    assert np.array_equal(la, lb)
+    m = la == 0
+    assert np.abs(fa[:, m] - fb[:, m]).max() <= 1e-14
+    assert np.abs(xa - xb).max() <= 1e-12
+    # and the total mass: the exact mode restores it to round-off
+    assert abs(fb[:, m].sum() / m.sum() - 1.0) < 1e-13
