@@ -53,6 +53,21 @@ int lbm_mlp_forward(lbm_mlp* m, const float* x, int n, float* y);
  * largest |mean| among them (relative disagreement of the ensemble). */
 int lbm_mlp_predict(lbm_mlp* m, const float* x, int n, float* mean, float* spread);
 
+/* No counterpart in the reference.  For callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc): lbm_mlp_predict for at most
+ * LBM_MLP_NOALLOC_MAX_SAMPLES inputs per call, with the same kernels and
+ * results bit for bit, keeping its inputs, the nets' outputs, mean and
+ * spread in one static device array of LBM_MLP_NOALLOC_SCRATCH_BYTES per GPU
+ * (held until the process ends) instead of buffers it allocates.  It calls
+ * no allocation or free; n above the maximum is rejected (LBM_HIP_EINVAL).
+ * Every call on one GPU uses that GPU's array: calls that overlap on one GPU
+ * (other handles, other threads or streams) race and must be serialised by
+ * the caller. */
+#define LBM_MLP_NOALLOC_MAX_SAMPLES 4096
+#define LBM_MLP_NOALLOC_SCRATCH_BYTES 6488064 /* 4 * 4096 * (5 + 64 * 6 + 6 + 1) */
+int lbm_mlp_predict_noalloc(lbm_mlp* m, const float* x, int n, float* mean, float* spread);
+
 /* Loss L = mean over n samples and 6 outputs of (y_net - t)^2 for net k,
  * and its gradient dL/dparams (LBM_MLP_PARAMS floats); no update. */
 int lbm_mlp_loss_grad(lbm_mlp* m, int k, const float* x, const float* t, int n, float* loss, float* grad);

@@ -184,6 +184,7 @@ def load_library(path: Path | None = None):
     lib.lbm_mlp_set_params.argtypes = [mh, C.c_int, fp]
     lib.lbm_mlp_forward.argtypes = [mh, fp, C.c_int, fp]
     lib.lbm_mlp_predict.argtypes = [mh, fp, C.c_int, fp, fp]
+    lib.lbm_mlp_predict_noalloc.argtypes = [mh, fp, C.c_int, fp, fp]
     lib.lbm_mlp_loss_grad.argtypes = [mh, C.c_int, fp, fp, C.c_int, fp, fp]
     lib.lbm_mlp_train.argtypes = [mh, fp, fp, C.c_int, C.c_int, C.c_int, C.c_float, fp]
     lib.lbm_hip_last_error.restype = C.c_char_p

@@ -20,6 +20,7 @@
 #include <cstdint>
 #include <cstring>
 #include <algorithm>
+#include <mutex>
 #include <random>
 #include <string>
 #include <vector>
@@ -230,7 +231,47 @@ __global__ __launch_bounds__(256) void mlp_cast_kernel(const float* w, uint16_t*
     if (e < total) wb[e] = to_bf16(w[e]);
 }
 
+// Ensemble statistics of y[nets][n][6]: mean over the nets -> mean[n][6];
+// spread[s] = the largest standard deviation over the nets among the six
+// outputs / the largest |mean| among them.  In FP64 without contraction,
+// one thread per sample.
+__global__ __launch_bounds__(256) void mlp_stats_kernel(const float* y, int nets, int n, float* mean, float* spread) {
+#pragma clang fp contract(off)
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    if (s >= n) return;
+    double big_mean = 0.0, big_sd = 0.0;
+    for (int c = 0; c < OUT; ++c) {
+        double mu = 0.0;
+        for (int k = 0; k < nets; ++k) mu += (double)y[((size_t)k * n + s) * OUT + c];
+        mu /= nets;
+        double var = 0.0;
+        for (int k = 0; k < nets; ++k) {
+            const double d = (double)y[((size_t)k * n + s) * OUT + c] - mu;
+            var += d * d;
+        }
+        const double sd = nets > 1 ? sqrt(var / (nets - 1)) : 0.0;
+        mean[(size_t)s * OUT + c] = (float)mu;
+        big_mean = fmax(big_mean, fabs(mu));
+        big_sd = fmax(big_sd, sd);
+    }
+    spread[s] = (float)(big_mean > 0.0 ? big_sd / big_mean : (big_sd > 0.0 ? INFINITY : 0.0));
+}
+
+// Layout of the device memory of lbm_mlp_predict_noalloc (`scratch` below):
+// inputs, per-net outputs, mean and spread of up to LBM_MLP_NOALLOC_MAX_SAMPLES
+// samples for up to 64 nets (6.5 MB).
+constexpr int NA_MAX = LBM_MLP_NOALLOC_MAX_SAMPLES;
+constexpr int MAX_NETS = 64;
+constexpr size_t NA_X = 0, NA_Y = NA_X + (size_t)NA_MAX * IN, NA_MEAN = NA_Y + (size_t)MAX_NETS * NA_MAX * OUT,
+                 NA_SPREAD = NA_MEAN + (size_t)NA_MAX * OUT, NA_FLOATS = NA_SPREAD + NA_MAX;
+static_assert(NA_FLOATS * sizeof(float) == LBM_MLP_NOALLOC_SCRATCH_BYTES, "scratch size in lbm_mlp.h");
+
 }  // namespace
+
+// One copy per GPU, for the life of the process.  Host code uses the address
+// hipGetSymbolAddress returns for the current device (noalloc_scratch()),
+// never this name.
+static __device__ __align__(256) float scratch[NA_FLOATS];
 
 // The error text is the one lbm_hip_last_error returns (lbm_hip.hip).
 void lbmhip_set_error_text(const char* msg);
@@ -253,7 +294,10 @@ struct lbm_mlp {
     float* x = nullptr;
     float* t = nullptr;
     float* y = nullptr;
-    size_t xcap = 0, tcap = 0, ycap = 0;
+    float* mean = nullptr;    // [n][6] and [n] of lbm_mlp_predict
+    float* spread = nullptr;
+    size_t xcap = 0, tcap = 0, ycap = 0, mcap = 0, scap = 0;
+    float* noalloc = nullptr;  // this device's copy of `scratch` (device address)
 };
 
 namespace {
@@ -277,6 +321,32 @@ int ensure(T*& p, size_t& cap, size_t need) {
     cap = 0;
     if (hipMalloc((void**)&p, need * sizeof(T)) != hipSuccess) return mfail(LBM_HIP_ENOMEM, "MLP scratch allocation failed");
     cap = need;
+    return LBM_HIP_OK;
+}
+
+// Device address of `scratch` on `device` (current), resolved once per device.
+float* noalloc_scratch(int device, int* rc) {
+    static std::mutex mu;
+    static float* addr[256] = {};
+    std::lock_guard<std::mutex> lock(mu);
+    if (device < 0 || device >= 256) { *rc = mfail(LBM_HIP_EINVAL, "device ordinal out of range"); return nullptr; }
+    if (!addr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            *rc = mfail(LBM_HIP_EDEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+            return nullptr;
+        }
+        addr[device] = static_cast<float*>(p);
+    }
+    *rc = LBM_HIP_OK;
+    return addr[device];
+}
+
+int launch_stats(lbm_mlp* M, const float* y, int n, float* mean, float* spread) {
+    hipLaunchKernelGGL(mlp_stats_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, M->stream, y, M->nets, n,
+                       mean, spread);
+    MLP_TRY(hipGetLastError());
     return LBM_HIP_OK;
 }
 
@@ -326,6 +396,15 @@ int lbm_mlp_create(int nets, int mode, unsigned long long seed, int device, lbm_
     if (hipSetDevice(device) != hipSuccess) return bail(LBM_HIP_EDEVICE, "hipSetDevice failed");
     if (hipStreamCreateWithFlags(&M->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(LBM_HIP_EDEVICE, "hipStreamCreate failed");
+    {
+        int src = LBM_HIP_OK;
+        M->noalloc = noalloc_scratch(device, &src);
+        if (src) {
+            const std::string msg = lbm_hip_last_error();
+            lbm_mlp_destroy(M);
+            return mfail(src, msg);
+        }
+    }
     const size_t tot = (size_t)nets * P;
     if (hipMalloc((void**)&M->w, tot * 4) != hipSuccess || hipMalloc((void**)&M->wb, tot * 2) != hipSuccess ||
         hipMalloc((void**)&M->m, tot * 4) != hipSuccess || hipMalloc((void**)&M->v, tot * 4) != hipSuccess ||
@@ -359,7 +438,7 @@ void lbm_mlp_destroy(lbm_mlp* M) {
     hipSetDevice(M->device);
     if (M->stream) hipStreamSynchronize(M->stream);
     for (void* q : {(void*)M->w, (void*)M->wb, (void*)M->m, (void*)M->v, (void*)M->grad, (void*)M->part,
-                    (void*)M->loss_part, (void*)M->x, (void*)M->t, (void*)M->y})
+                    (void*)M->loss_part, (void*)M->x, (void*)M->t, (void*)M->y, (void*)M->mean, (void*)M->spread})
         if (q) hipFree(q);
     if (M->stream) hipStreamDestroy(M->stream);
     delete M;
@@ -401,27 +480,44 @@ int lbm_mlp_forward(lbm_mlp* M, const float* x, int n, float* y) {
 
 int lbm_mlp_predict(lbm_mlp* M, const float* x, int n, float* mean, float* spread) {
     if (!M || !x || !mean) return mfail(LBM_HIP_EINVAL, "null argument");
-    std::vector<float> y((size_t)M->nets * n * OUT);
-    int rc = lbm_mlp_forward(M, x, n, y.data());
-    if (rc) return rc;
-    for (int s = 0; s < n; ++s) {
-        double big_mean = 0.0, big_sd = 0.0;
-        for (int c = 0; c < OUT; ++c) {
-            double mu = 0.0;
-            for (int k = 0; k < M->nets; ++k) mu += y[((size_t)k * n + s) * OUT + c];
-            mu /= M->nets;
-            double var = 0.0;
-            for (int k = 0; k < M->nets; ++k) {
-                const double d = y[((size_t)k * n + s) * OUT + c] - mu;
-                var += d * d;
-            }
-            const double sd = M->nets > 1 ? std::sqrt(var / (M->nets - 1)) : 0.0;
-            mean[(size_t)s * OUT + c] = (float)mu;
-            big_mean = std::fmax(big_mean, std::fabs(mu));
-            big_sd = std::fmax(big_sd, sd);
-        }
-        if (spread) spread[s] = (float)(big_mean > 0.0 ? big_sd / big_mean : (big_sd > 0.0 ? INFINITY : 0.0));
-    }
+    if (n < 0) return mfail(LBM_HIP_EINVAL, "n must be >= 0");
+    if (n == 0) return LBM_HIP_OK;
+    int rc;
+    if ((rc = upload_xt(M, x, nullptr, n))) return rc;
+    if ((rc = ensure(M->y, M->ycap, (size_t)M->nets * n * OUT))) return rc;
+    if ((rc = ensure(M->mean, M->mcap, (size_t)n * OUT))) return rc;
+    if ((rc = ensure(M->spread, M->scap, (size_t)n))) return rc;
+    TileArgs a{};
+    a.x = M->x; a.n = n; a.y = M->y;
+    if ((rc = launch_tiles(M, a, M->nets))) return rc;
+    if ((rc = launch_stats(M, M->y, n, M->mean, M->spread))) return rc;
+    MLP_TRY(hipMemcpyAsync(mean, M->mean, sizeof(float) * n * OUT, hipMemcpyDeviceToHost, M->stream));
+    if (spread) MLP_TRY(hipMemcpyAsync(spread, M->spread, sizeof(float) * n, hipMemcpyDeviceToHost, M->stream));
+    MLP_TRY(hipStreamSynchronize(M->stream));
+    return LBM_HIP_OK;
+}
+
+int lbm_mlp_predict_noalloc(lbm_mlp* M, const float* x, int n, float* mean, float* spread) {
+    if (!M || !x || !mean) return mfail(LBM_HIP_EINVAL, "null argument");
+    if (n < 0) return mfail(LBM_HIP_EINVAL, "n must be >= 0");
+    if (n > NA_MAX)
+        return mfail(LBM_HIP_EINVAL, "lbm_mlp_predict_noalloc takes at most " + std::to_string(NA_MAX) +
+                                         " samples per call (LBM_MLP_NOALLOC_MAX_SAMPLES); got " + std::to_string(n));
+    if (n == 0) return LBM_HIP_OK;
+    MLP_TRY(hipSetDevice(M->device));
+    float* sx = M->noalloc + NA_X;
+    float* sy = M->noalloc + NA_Y;
+    float* sm = M->noalloc + NA_MEAN;
+    float* ss = M->noalloc + NA_SPREAD;
+    MLP_TRY(hipMemcpyAsync(sx, x, sizeof(float) * n * IN, hipMemcpyHostToDevice, M->stream));
+    TileArgs a{};
+    a.x = sx; a.n = n; a.y = sy;
+    int rc;
+    if ((rc = launch_tiles(M, a, M->nets))) return rc;
+    if ((rc = launch_stats(M, sy, n, sm, ss))) return rc;
+    MLP_TRY(hipMemcpyAsync(mean, sm, sizeof(float) * n * OUT, hipMemcpyDeviceToHost, M->stream));
+    if (spread) MLP_TRY(hipMemcpyAsync(spread, ss, sizeof(float) * n, hipMemcpyDeviceToHost, M->stream));
+    MLP_TRY(hipStreamSynchronize(M->stream));
     return LBM_HIP_OK;
 }
 
