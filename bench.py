@@ -170,32 +170,56 @@ def measure_mlp(lbm, samples: int = 4096, epochs: int = 20, batch: int = 256) ->
             "final_loss": [round(float(v), 6) for v in loss]}
 
 
-def cpu_baseline(budget_s: float = 12.0) -> dict | None:
-    """Reference step on this host's cores, bounded sample of the workload."""
+# The same slit channel as the reference can express it (its orientation
+# angle and planar motion are in x-y only, params.hpp:48-50): y and z
+# relabelled, walls in y, so the reference runs this very workload.
+REFERENCE_DECK = dict(nx=512, ny=128, nz=256, tau=1.0, walls_y=1, walls_z=0, squirmer=1, radius=12.0, b1=1e-3,
+                      beta=-3.0, y0=64.0, angle=15.0, planar=1, eps=0.2, hc=2.0, hmin=1.0)
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_threads() -> tuple[int, int]:
+    """(threads used, cores this process may run on).  The threads are
+    OMP_NUM_THREADS when the environment sets it (a GPU box allots each GPU
+    a share of its host cores and sets it to that share), else every core
+    of the affinity mask."""
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return (max(1, min(int(env), aff)) if env.isdigit() else aff), aff
+
+
+def cpu_baseline(budget_s: float = 15.0) -> dict | None:
+    """The reference's own step (compiled from its sources into oracle/_ref,
+    with its Makefile's optimisation) on this host's cores, on a bounded
+    sample of the workload: the y<->z-relabelled deck it can express; the
+    oracle restatement of the same workload where that build is absent."""
     try:
         from oracle import oracle as O
     except Exception as e:  # pragma: no cover
         print(f"cpu baseline unavailable: {e}", file=sys.stderr)
         return None
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        cores = os.cpu_count() or 1
-    cores = max(1, min(cores, 16))
-    O.set_threads(cores)
-    kv = dict(WORKLOAD)
+    threads, aff = cpu_threads()
+    O.set_threads(threads)
     use_ref = (O.HERE / "_ref" / "libref_lbm_fast.so").exists()
     if use_ref:
-        # the reference has no x-z orientation key; the timing does not depend on it
-        kv.pop("plane_xz")
-        kv["angle"] = 0.0
-        kv["y0"] = -1
-        obj = O.Reference(variant="_fast", **{k: v for k, v in kv.items()})
-        obj.set_threads(cores)
-        kind = "reference"
+        obj = O.Reference(variant="_fast", **REFERENCE_DECK)
+        obj.set_threads(threads)
+        kind, deck = "reference", "512x128x256 walls in y (the workload with y and z relabelled)"
     else:
-        obj = O.Oracle(**kv)
-        kind = "port"
+        obj = O.Oracle(**WORKLOAD)
+        kind, deck = "port", "512x256x128 walls in z"
 
     def one(n):
         t = time.perf_counter()
@@ -204,27 +228,31 @@ def cpu_baseline(budget_s: float = 12.0) -> dict | None:
 
     t1 = one(2)
     per = t1 / 2
-    n = max(2, min(200, int(budget_s / max(per, 1e-6))))
+    n = max(2, min(400, int(budget_s / max(per, 1e-6))))
     dt = one(n)
     N = 512 * 256 * 128
-    return dict(value=N * n / dt / 1e6, unit="MLUPS", cores=cores, kind=kind,
-                sample=f"{n} full steps (step + squirmer update + mass correction every {MASS_EVERY}) of the "
-                       f"512x256x128 workload after 2 untimed steps, FP64, {cores} OpenMP threads")
+    return dict(value=N * n / dt / 1e6, unit="MLUPS", cores=threads, kind=kind, cpu_model=cpu_model(),
+                affinity_cores=aff,
+                sample=f"{n} full steps (step + squirmer update + mass correction every {MASS_EVERY}) of the pusher "
+                       f"channel, {deck}, after 2 untimed steps, FP64, {threads} OpenMP threads")
 
 
 TRAFFIC_FILE = ROOT / "profiles" / "hbm_traffic_aa_step_512x256x128.txt"
 
 
-def measured_traffic(precision: str):
-    """HBM bytes per launch of the collide-stream kernel (mean of the two AA
+def measured_traffic(precision: str, tile_width: int, xcd_remap: int):
+    """HBM bytes of the collide-stream update of one step (bulk kernel of
+    both parities and the even-step boundary kernel, averaged over the two
     parities) from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs recorded in
-    profiles/, scaled by the counters' reading of a 1 GiB copy."""
+    profiles/ for one tile shape, scaled by the counters' reading of a copy
+    of known size; None when the shape in use was not profiled."""
     try:
         for line in TRAFFIC_FILE.read_text().splitlines():
             p = line.split()
-            if len(p) == 3 and p[0] == "traffic_bytes_per_launch" and p[1] == precision:
-                return float(p[2])
-    except OSError:
+            if len(p) == 5 and p[0] == "traffic_bytes_per_step" and p[1] == precision and \
+                    (int(p[2]), int(p[3])) == (tile_width, xcd_remap):
+                return float(p[4])
+    except (OSError, ValueError):
         pass
     return None
 
@@ -256,7 +284,7 @@ def main():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=500)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--precisions", default="fp16,fp32")
+    ap.add_argument("--precisions", default="fp16,fp32,fp64")
     ap.add_argument("--workload", default="channel", choices=["channel", "ensemble", "surrogate", "trajectory", "mlp"],
                     help="channel: the 512x256x128 slit channel (headline); ensemble: 64 slit-channel runs batched; "
                          "surrogate: the learned velocity surrogate with LBM fallback, 1000 trajectories; "
@@ -388,8 +416,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(head["frac"], 4),
-            "traffic": measured_traffic(head["precision"]),
-            "kernel": "aa_step_kernel",
+            "traffic": measured_traffic(head["precision"], head["tile_width"], head["xcd_remap"]),
+            "traffic_source": f"{TRAFFIC_FILE.relative_to(ROOT)} for tiles {head['tile_width']},{head['xcd_remap']}",
+            "kernel": "aa_step_kernel + aa_boundary_kernel (even steps): the collide-stream update of a step",
             "copy_gbs": round(copy_gbs, 1),
             "frac_of_copy": round(head["achieved_gbs"] / copy_gbs, 4),
             "bytes_per_lup": head["bytes_per_lup"],
@@ -399,13 +428,14 @@ def main():
         },
         "cpu_baseline": cpu,
     }
+    for k in ("fp32", "fp64"):
+        if k in res and res[k] is not head:
+            out[k] = {"value": round(res[k]["mlups"], 1), "ms_per_step": round(res[k]["ms_per_step"], 5),
+                      "kernel_ms": round(res[k]["kernel_ms"], 5), "bytes_per_lup": res[k]["bytes_per_lup"],
+                      "roofline_frac": round(res[k]["frac"], 4), "achieved_gbs": round(res[k]["achieved_gbs"], 1),
+                      "copy_gbs": round(copy_of[k], 1), "frac_of_copy": round(res[k]["achieved_gbs"] / copy_of[k], 4),
+                      "traffic": measured_traffic(k, res[k]["tile_width"], res[k]["xcd_remap"])}
     if "fp32" in res and "fp16" in res:
-        out["fp32"] = {"value": round(res["fp32"]["mlups"], 1), "ms_per_step": round(res["fp32"]["ms_per_step"], 5),
-                       "kernel_ms": round(res["fp32"]["kernel_ms"], 5),
-                       "roofline_frac": round(res["fp32"]["frac"], 4),
-                       "achieved_gbs": round(res["fp32"]["achieved_gbs"], 1),
-                       "copy_gbs": round(copy_of["fp32"], 1),
-                       "frac_of_copy": round(res["fp32"]["achieved_gbs"] / copy_of["fp32"], 4)}
         out["fp16_over_fp32"] = round(res["fp16"]["mlups"] / res["fp32"]["mlups"], 3)
     for k, v in res.items():
         out.setdefault("per_precision", {})[k] = {kk: (round(vv, 6) if isinstance(vv, float) else vv)

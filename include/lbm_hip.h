@@ -180,8 +180,10 @@ int lbm_hip_get_info(lbm_hip_ctx* ctx, lbm_hip_info* out);
 
 /* Bench support: time `steps` driver-loop steps (as lbm_hip_run, graph
  * replayed) with HIP events on the context's stream -> *total_ms; then run up
- * to 64 further steps eagerly with an event pair around each fused step
- * kernel and return their mean duration times `steps` -> *step_kernel_ms. */
+ * to 64 further steps eagerly with an event pair around the collide-stream
+ * update of each step (the bulk kernel, and on even steps the boundary
+ * kernel after it) and return their mean duration times `steps`
+ * -> *step_kernel_ms. */
 int lbm_hip_time_run(lbm_hip_ctx* ctx, int steps, int mass_every, double* total_ms,
                      double* step_kernel_ms);
 

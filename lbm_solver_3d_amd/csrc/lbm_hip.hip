@@ -45,8 +45,9 @@ size_t storage_bytes(int prec) {
 
 struct lbm_hip_ctx {
     lbm_hip_params p{};
-    // when set, lbm_hip_step records these around the collide-stream kernel
-    // launch alone (timing of that kernel by lbm_hip_time_run)
+    // when set, lbm_hip_step records these around the collide-stream update
+    // (bulk kernel, plus the boundary kernel on even steps), for
+    // lbm_hip_time_run
     hipEvent_t kev[2] = {nullptr, nullptr};
     int device = 0;
     hipStream_t stream = nullptr;
@@ -545,8 +546,8 @@ int lbm_hip_step(lbm_hip_ctx* c, int flags) {
         const StepArgs<S> sa = step_args<S>(c);
         if (c->kev[0]) e = hipEventRecord(c->kev[0], c->stream);
         if (e == hipSuccess) e = launch_step<S>(sa, c->K, odd, mode, c->stream);
-        if (e == hipSuccess && c->kev[1]) e = hipEventRecord(c->kev[1], c->stream);
         if (e == hipSuccess && !odd) e = launch_boundary<S>(sa, c->K, c->bargs, c->bblocks, mode, c->stream);
+        if (e == hipSuccess && c->kev[1]) e = hipEventRecord(c->kev[1], c->stream);
         // moving-boundary bounce-back and force/torque on the squirmer
         if (e == hipSuccess && c->has_squirmer) e = launch_links<S>(link_args<S>(c), c->link_blocks, c->stream);
         if (e == hipSuccess && (flags & LBM_STEP_WALL_FORCE))
@@ -894,8 +895,9 @@ int lbm_hip_time_run(lbm_hip_ctx* c, int steps, int mass_every, double* total_ms
     HIP_TRY(hipEventSynchronize(ev[1]));
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[1]));
-    // pass 2: the collide-stream kernel alone, its launches bracketed by
-    // events on its stream (the driver loop around it runs as usual)
+    // pass 2: the collide-stream update alone (bulk + even-step boundary
+    // kernel), bracketed by events on its stream (the driver loop around it
+    // runs as usual)
     double ksum = 0.0;
     for (int s = 0; s < ksteps; ++s) {
         const long long tnext = c->t + 1;

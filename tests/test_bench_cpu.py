@@ -17,13 +17,28 @@ def test_more_than_one_gpu_is_refused():
     assert "one GPU" in msg["error"]
 
 
-def test_traffic_per_launch_from_counter_summary():
+def test_traffic_per_step_from_counter_summary():
+    """The counter summary gives the collide-stream traffic per step for the
+    profiled tile shape; another shape gets none (no stale figure)."""
     import bench
     for prec, bytes_per_lup in (("fp16", 2 * 19 * 2), ("fp32", 2 * 19 * 4)):
-        t = bench.measured_traffic(prec)
+        t = bench.measured_traffic(prec, 256, 2)
         assert t is not None, f"no {prec} traffic line in the counter summary"
         algorithmic = bytes_per_lup * 512 * 256 * 128
         assert 0.9 < t / algorithmic < 1.1
+        assert bench.measured_traffic(prec, 64, 1) is None
+
+
+def test_reference_deck_is_the_workload_relabelled():
+    """The CPU baseline's deck is the benchmark channel with y and z swapped:
+    same sizes, walls, sphere and distance from the wall."""
+    import bench
+    w, r = bench.WORKLOAD, bench.REFERENCE_DECK
+    assert (r["nx"], r["ny"], r["nz"]) == (w["nx"], w["nz"], w["ny"])
+    assert (r["walls_y"], r["walls_z"]) == (w["walls_z"], w["walls_y"])
+    assert r["y0"] == w["z0"] and r["angle"] == w["angle"] and "plane_xz" not in r
+    for k in ("radius", "b1", "beta", "tau", "eps", "hc", "hmin", "planar"):
+        assert r[k] == w[k], k
 
 
 def test_ensemble_traffic_per_shape_from_counter_summary():
