@@ -77,9 +77,14 @@ def test_puller_oscillation_is_damped(tmp_path):
     assert amp[1] < amp[0] - 0.05, f"amplitudes {amp} should decrease"
 
 
+# The bounds on the late position difference are set by how far the
+# reference itself strays from its own files: its sources built here
+# (oracle/_ref/lbm_squirmer, GCC, 8 OpenMP threads) and run on the same decks
+# end 4.75e-2 (neutral, 40000 steps), 4.7e-3 (pusher, 10000), 7.3e-3 (puller
+# 64 x 34 x 32, 20000) and 1.5e-2 (puller deck, 20000) away from them.
 @pytest.mark.parametrize("deck,over,fixture,digits_until,steps,pos_tol", [
     # observed: printed digits to t = 10100 / 8200 (U, W); position error 2.9e-3 at 40000
-    ("channel_neutral", {}, "example_channel_neutral_traj.dat", 6000, 40000, 1e-2),
+    ("channel_neutral", {}, "example_channel_neutral_traj.dat", 6000, 40000, 6e-2),
     # observed: printed digits to 4900 / 4100; 4.8e-3 at 10000
     ("channel_pusher", {"beta": -3}, "example_channel_pusher_traj.dat", 4000, 10000, 2e-2),
     # observed: printed digits to 8700 / 7100; 6.3e-3 at 20000
