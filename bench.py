@@ -360,8 +360,11 @@ def main():
             "metric": "trajectories/s, learned velocity surrogate with LBM fallback (1000 slit-channel swimmers)",
             "value": round(traj_s, 2), "unit": "trajectories/s", "n_gpus": 1, "steps": t["steps"], "warmup": 0,
             "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16",
-            "data": "synthetic", "config": {"workload": "MLP 5-128-128-128-6 x4, trained on batched LBM runs; "
-                                                        "slit channels H in [16,32], R in [3,6], B1=1e-2",
+            "data": "synthetic", "config": {"workload": "MLP 5-128-128-128-6 x4, trained on batched LBM runs of "
+                                                        "slit channels 4Hx2HxH, walls in z, H in [24,64], R in [4,8], "
+                                                        "beta in [-5,5], B1=1e-3; 1000 trajectories of 8000 steps in "
+                                                        "windows of 1000, a window with ensemble spread > 2% run by "
+                                                        "the LBM",
                                             "parallelism": "single GPU"},
             "report": rep}))
         return

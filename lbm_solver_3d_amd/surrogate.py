@@ -5,10 +5,16 @@ the Stokes regime, with a velocity fixed by its configuration.  The
 surrogate predicts that velocity from five numbers -- the distance from the
 centre to the nearest wall over R, sin and cos of the pitch, beta and R/H --
 with an ensemble of four MLPs (include/lbm_mlp.h, trained and evaluated by
-the library's HIP kernels) and integrates trajectories with it.  Where the
-ensemble disagrees by more than 2 % the trajectory is run by the full
-lattice-Boltzmann solver instead (lbm_hip_batch_*).  Training data come from
-batched lattice-Boltzmann runs of the same channels.
+the library's HIP kernels) and integrates trajectories with it window by
+window.  A window that starts where the ensemble disagrees by more than 2 %
+is run by the full lattice-Boltzmann solver instead (lbm_hip_batch_*), from
+the trajectory's state at that point.  Training data come from batched
+lattice-Boltzmann runs of the same channels.
+
+The channels are the slit channels of the 512 x 256 x 128 pusher benchmark
+at smaller sizes: 4H x 2H x H, walls in z, x and y periodic, B1 = 1e-3
+(U0 = 6.7e-4), tau = 1, the hard gap 1 and the soft wall potential of that
+benchmark.
 
 Wall frame: the configuration is mirrored in z so that the nearest wall lies
 below (z' = z if the lower wall z = 0.5 is nearer, z' = -z otherwise;
@@ -73,15 +79,13 @@ def free_space(x: np.ndarray) -> np.ndarray:
 # ---------------------------------------------------------------------------
 # lattice-Boltzmann runs
 # ---------------------------------------------------------------------------
-def channel_configs(count: int, seed: int, H=(16, 32), R=(3, 6), beta=(-3.0, 3.0), pitch=(-40.0, 40.0),
-                    B1=1e-2) -> list[dict]:
+def channel_configs(count: int, seed: int, H=(24, 64), R=(4, 8), beta=(-5.0, 5.0), pitch=(-44.0, 44.0),
+                    B1=1e-3) -> list[dict]:
     """Slit channels 4H x 2H x H (walls in z, x and y periodic) with one
     squirmer each, drawn with numpy's default_rng(seed): H, R integers;
     beta, pitch (x-z plane, toward +z when positive) and the centre height
-    uniform, the height keeping the gap to either wall at least 1.5 (R at
-    most (H - 5) / 2).  B1 = 1e-2 (U0 = 6.7e-3, Reynolds number 2 R U0 / nu
-    0.24 - 0.48): the swimmer crosses a lattice spacing in ~150 steps, so
-    window means average the staircase of its lattice surface out."""
+    uniform, the height keeping the gap to either wall at least 1.5.
+    B1 = 1e-3 (U0 = 6.7e-4, Reynolds number 2 R U0 / nu 0.03 - 0.06)."""
     rng = np.random.default_rng(seed)
     out = []
     for _ in range(count):
@@ -146,6 +150,53 @@ def lbm_trajectories(configs: list[dict], steps: int, every: int, precision: str
     return np.array(Xs).transpose(1, 0, 2), np.array(es).transpose(1, 0, 2)
 
 
+def lbm_windows(configs: list[dict], X: np.ndarray, e: np.ndarray, steps: int, warmup: int, hold_every: int = 100,
+                precision: str = "fp16", batch: int = 128):
+    """The LBM fallback for one window of each listed trajectory: a lattice
+    of the trajectory's channel with the squirmer at its current position X
+    and orientation e, a warm-up of `warmup` steps in which the body is put
+    back at (X, e) every `hold_every` steps (its velocities keep relaxing, so
+    the flow settles around the body where it is), then `steps` free steps.
+    Returns the positions and orientations after the window."""
+    Xo, eo = np.array(X, float).copy(), np.array(e, float).copy()
+    for a in range(0, len(configs), batch):
+        sl = slice(a, a + batch)
+        cfg, Xs, es = configs[sl], Xo[sl].copy(), eo[sl].copy()
+        kv, start = [], []
+        for c, x in zip(cfg, Xs):
+            k = dict(c)
+            x0 = np.array([x[0] % c["nx"], x[1] % c["ny"], x[2]])
+            k.update(x0=float(x0[0]), y0=float(x0[1]), z0=float(x0[2]), angle=0.0)
+            kv.append(k)
+            start.append(x0)
+        with Batch(kv, precision=precision) as B:
+            held = [dict(X=list(x0), e=list(ee)) for x0, ee in zip(start, es)]
+            for i, h in enumerate(held):
+                B.member(i).set_squirmer(**h)
+            done = 0
+            while done < warmup:
+                n = min(hold_every, warmup - done)
+                B.run(n, MASS_EVERY, 0)
+                done += n
+                for i, h in enumerate(held):
+                    B.member(i).set_squirmer(**h)
+            B.run(steps, MASS_EVERY, 0)
+            for i in range(len(cfg)):
+                q = B.member(i).squirmer()
+                Xo[a + i] = Xs[i] + (q["X"] - start[i])   # the displacement, in the trajectory's unwrapped frame
+                eo[a + i] = q["e"]
+    return Xo, eo
+
+
+def _features_v(X, e, R, beta, nz):
+    """features() for arrays of configurations."""
+    lo, hi = X[:, 2] - 0.5, (nz - 1.5) - X[:, 2]
+    sgn = np.where(lo <= hi, 1.0, -1.0)
+    d = np.minimum(lo, hi)
+    th = np.arctan2(sgn * e[:, 2], e[:, 0])
+    return np.stack([d / R, np.sin(th), np.cos(th), beta, R / (nz - 2.0)], 1), sgn
+
+
 # ---------------------------------------------------------------------------
 # the surrogate
 # ---------------------------------------------------------------------------
@@ -173,50 +224,92 @@ class Surrogate:
         spread = sd.max(1) / np.maximum(np.abs(mean).max(1), 1e-12)
         return mean, spread
 
-    def trajectories(self, configs: list[dict], X0, e0, steps: int, every: int, spread_limit: float = 0.02):
-        """Integrate every configuration from (X0, e0) for `steps` steps, the
-        velocity re-evaluated every `every` steps (explicit Euler on X and on
-        e, e renormalised, the hard gap applied as in lbm.cpp:245-253).
-        Returns positions, orientations (runs, samples, 3) and, per run,
-        whether the ensemble spread ever exceeded `spread_limit`."""
+    def advance(self, configs: list[dict], X, e, steps: int, every: int):
+        """Integrate from (X, e) for `steps` steps with the velocity
+        re-evaluated every `every` steps (explicit Euler on X and e, e
+        renormalised, planar motion in x-z, the hard gap of lbm.cpp:245-253).
+        Returns the new (X, e)."""
+        R = np.array([c["radius"] for c in configs])
+        beta = np.array([c["beta"] for c in configs])
+        nz = np.array([c["nz"] for c in configs], float)
+        U0 = np.array([2.0 * c["b1"] / 3.0 for c in configs])
+        hmin = np.array([c["hmin"] for c in configs])
+        X, e = np.array(X, float).copy(), np.array(e, float).copy()
+        for _ in range(steps // every):
+            x, sgn = _features_v(X, e, R, beta, nz)
+            y, _ = self.predict(x)
+            U = np.stack([y[:, 0], np.zeros_like(sgn), sgn * y[:, 2]], 1) * U0[:, None]
+            Wy = sgn * y[:, 4] * U0 / R                  # only the rotation about y acts in the x-z plane
+            X = X + U * every
+            e = e + np.stack([Wy * e[:, 2], np.zeros_like(Wy), -Wy * e[:, 0]], 1) * every
+            e[:, 1] = 0.0
+            e /= np.linalg.norm(e, axis=1)[:, None]
+            X[:, 2] = np.clip(X[:, 2], 0.5 + R + hmin, (nz - 1.5) - R - hmin)
+        return X, e
+
+    def trajectories(self, configs: list[dict], X0, e0, steps: int, every: int, window: int,
+                     spread_limit: float = 0.02, fallback_warmup: int = 2000, fallback_precision: str = "fp16"):
+        """Integrate every configuration from (X0, e0) for `steps` steps in
+        windows of `window` steps.  A window whose starting configuration has
+        an ensemble spread above `spread_limit` is run by the LBM
+        (lbm_windows) instead of the surrogate.  Returns positions and
+        orientations at the window ends (runs, windows + 1, 3), the number of
+        windows run by the LBM per run, and the seconds spent in the
+        surrogate and in the fallback."""
         n = len(configs)
         X, e = np.array(X0, float).copy(), np.array(e0, float).copy()
+        R = np.array([c["radius"] for c in configs])
+        beta = np.array([c["beta"] for c in configs])
+        nz = np.array([c["nz"] for c in configs], float)
         Xs, es = [X.copy()], [e.copy()]
-        uncertain = np.zeros(n, bool)
-        for _ in range(steps // every):
-            x = np.array([features(X[i], e[i], c["radius"], c["beta"], c["nz"]) for i, c in enumerate(configs)])
-            y, sp = self.predict(x)
-            uncertain |= sp > spread_limit
-            for i, c in enumerate(configs):
-                U, W = to_lab(y[i], X[i], c["radius"], c["b1"], c["nz"])
-                U[1] = 0.0; W[0] = W[2] = 0.0        # planar motion in x-z (squirmer.cpp:58, 65)
-                X[i] = X[i] + U * every
-                e[i] = e[i] + np.cross(W, e[i]) * every
-                e[i][1] = 0.0
-                e[i] /= np.linalg.norm(e[i])
-                lo = 0.5 + c["radius"] + c["hmin"]
-                hi = (c["nz"] - 1.5) - c["radius"] - c["hmin"]
-                X[i][2] = min(max(X[i][2], lo), hi)
+        fallback = np.zeros(n, int)
+        t_sur = t_fb = 0.0
+        for _ in range(steps // window):
+            t0 = time.perf_counter()
+            _, sp = self.predict(_features_v(X, e, R, beta, nz)[0])
+            unc = sp > spread_limit
+            ok = np.flatnonzero(~unc)
+            if len(ok):
+                X[ok], e[ok] = self.advance([configs[i] for i in ok], X[ok], e[ok], window, every)
+            t1 = time.perf_counter()
+            bad = np.flatnonzero(unc)
+            if len(bad):
+                X[bad], e[bad] = lbm_windows([configs[i] for i in bad], X[bad], e[bad], window, fallback_warmup,
+                                             precision=fallback_precision)
+                fallback[bad] += 1
+            t_sur += t1 - t0
+            t_fb += time.perf_counter() - t1
             Xs.append(X.copy())
             es.append(e.copy())
-        return np.array(Xs).transpose(1, 0, 2), np.array(es).transpose(1, 0, 2), uncertain
+        return (np.array(Xs).transpose(1, 0, 2), np.array(es).transpose(1, 0, 2), fallback, t_sur, t_fb)
 
 
-def study(n_train_runs=256, n_test_runs=64, n_traj=1000, n_ref=16, seed=0,
-          sample_at=tuple(range(3000, 5001, 250)), epochs=400, traj_warmup=3000, traj_steps=4000, every=100,
-          fallback_precision="fp16", log=print):
+def study(n_train_runs=256, n_test_runs=64, n_traj=1000, n_ref=16, seed=0, H=(24, 64), warmup=6000, window=3000,
+          windows=5, epochs=600, traj_warmup=3000, traj_steps=8000, every=100, traj_window=1000,
+          fallback_warmup=2000, fallback_precision="fp16", log=print):
     """The surrogate end to end: training data from batched LBM runs, 4-net
     training on the GPU, held-out error against the free-space model, 1000
-    trajectories driven by the surrogate with LBM fallback where the
-    ensemble disagrees, and the trajectory error against full-LBM FP64 runs
-    of a subset.  Returns a report (dict)."""
+    trajectories driven by the surrogate with LBM fallback over the windows
+    where the ensemble disagrees, and the trajectory error against full-LBM
+    FP64 runs of a subset.  Returns a report (dict)."""
     rep = {}
     t0 = time.perf_counter()
-    train_cfg = channel_configs(n_train_runs, seed)
-    test_cfg = channel_configs(n_test_runs, seed + 1)
-    xtr, ytr = lbm_samples(train_cfg, list(sample_at))
-    xte, yte = lbm_samples(test_cfg, list(sample_at))
-    rep["data"] = dict(train_samples=len(xtr), test_samples=len(xte), seconds=time.perf_counter() - t0)
+    sample_at = [warmup + k * window for k in range(windows + 1)]
+
+    def samples(cfg):
+        xs, ys = [], []
+        for a in range(0, len(cfg), 64):
+            x, y = lbm_samples(cfg[a:a + 64], sample_at)
+            xs.append(x)
+            ys.append(y)
+        return np.concatenate(xs), np.concatenate(ys)
+
+    train_cfg = channel_configs(n_train_runs, seed, H=H)
+    test_cfg = channel_configs(n_test_runs, seed + 1, H=H)
+    xtr, ytr = samples(train_cfg)
+    xte, yte = samples(test_cfg)
+    rep["data"] = dict(train_runs=n_train_runs, train_samples=len(xtr), test_samples=len(xte), H=list(H),
+                       warmup=warmup, window=window, windows=windows, seconds=time.perf_counter() - t0)
     log(f"[surrogate] data: {len(xtr)} train / {len(xte)} held-out samples in {rep['data']['seconds']:.1f} s")
 
     S = Surrogate(nets=4, mode="bf16", seed=seed)
@@ -224,7 +317,7 @@ def study(n_train_runs=256, n_test_runs=64, n_traj=1000, n_ref=16, seed=0,
     losses = S.fit(xtr, ytr, epochs=epochs)
     log(f"[surrogate] trained 4 nets: normalised loss {[round(float(v), 5) for v in losses]} "
         f"in {time.perf_counter() - t1:.1f} s")
-    rep["train"] = dict(loss=[float(v) for v in losses], seconds=time.perf_counter() - t1)
+    rep["train"] = dict(loss=[float(v) for v in losses], seconds=time.perf_counter() - t1, epochs=epochs)
     pred, spread = S.predict(xte)
     rms = lambda a: float(np.sqrt(np.mean(a ** 2)))
     rep["held_out"] = dict(rms_mlp=rms(pred - yte), rms_free_space=rms(free_space(xte) - yte),
@@ -234,50 +327,43 @@ def study(n_train_runs=256, n_test_runs=64, n_traj=1000, n_ref=16, seed=0,
     log(f"[surrogate] held-out RMS error (targets in U0, U0/R): MLP {rep['held_out']['rms_mlp']:.4f}, "
         f"free-space model {rep['held_out']['rms_free_space']:.4f}")
 
-    # trajectories: warm the flow up with the LBM, then continue with the surrogate
-    traj_cfg = channel_configs(n_traj, seed + 2)
+    # trajectories: the flow develops with the LBM, then the surrogate takes over
+    traj_cfg = channel_configs(n_traj, seed + 2, H=H)
     t2 = time.perf_counter()
     X0, e0 = [], []
-    for a in range(0, n_traj, 256):
-        Xw, ew = lbm_trajectories(traj_cfg[a:a + 256], 0, 1, fallback_precision, warmup=traj_warmup)
-        X0 += list(Xw[:, 0]); e0 += list(ew[:, 0])
+    for a in range(0, n_traj, 128):
+        Xw, ew = lbm_trajectories(traj_cfg[a:a + 128], 0, 1, fallback_precision, warmup=traj_warmup)
+        X0 += list(Xw[:, 0])
+        e0 += list(ew[:, 0])
+    X0, e0 = np.array(X0), np.array(e0)
     t_warm = time.perf_counter() - t2
-    t3 = time.perf_counter()
-    Xs, es, unc = S.trajectories(traj_cfg, X0, e0, traj_steps, every)
-    t_sur = time.perf_counter() - t3
-    fb = np.flatnonzero(unc)
-    t4 = time.perf_counter()
-    for a in range(0, len(fb), 256):
-        idx = fb[a:a + 256]
-        Xl, el = lbm_trajectories([traj_cfg[i] for i in idx], traj_steps, every, fallback_precision, warmup=traj_warmup)
-        Xs[idx], es[idx] = Xl, el
-    t_fb = time.perf_counter() - t4
-    rep["trajectories"] = dict(count=n_traj, steps=traj_steps, fallback=int(len(fb)), surrogate_seconds=t_sur,
-                               fallback_seconds=t_fb, warmup_seconds=t_warm)
+    Xs, es, fb, t_sur, t_fb = S.trajectories(traj_cfg, X0, e0, traj_steps, every, traj_window,
+                                             fallback_warmup=fallback_warmup, fallback_precision=fallback_precision)
+    nwin = traj_steps // traj_window
+    rep["trajectories"] = dict(count=n_traj, steps=traj_steps, window=traj_window, windows=nwin,
+                               fallback_windows=int(fb.sum()), runs_with_fallback=int(np.sum(fb > 0)),
+                               surrogate_seconds=t_sur, fallback_seconds=t_fb, warmup_seconds=t_warm)
     log(f"[surrogate] {n_traj} trajectories x {traj_steps} steps: surrogate {t_sur:.2f} s, "
-        f"{len(fb)} fell back to the LBM ({t_fb:.1f} s)")
+        f"{int(fb.sum())} of {n_traj * nwin} windows by the LBM ({t_fb:.1f} s)")
 
     # error against full-LBM FP64 runs of the first n_ref trajectories
     ref = list(range(n_ref))
-    Xr, er = lbm_trajectories([traj_cfg[i] for i in ref], traj_steps, every, "fp64", warmup=traj_warmup)
-    Xf = np.array([X0[i] for i in ref])
-    ef = np.array([e0[i] for i in ref])
-    # free-space model from the same starting states
-    Xb = Xf.copy()
-    eb = ef.copy()
-    for _ in range(traj_steps // every):
-        for j, i in enumerate(ref):
-            U0 = 2.0 * traj_cfg[i]["b1"] / 3.0
-            Xb[j] = Xb[j] + U0 * eb[j] * every
+    Xr, er = lbm_trajectories([traj_cfg[i] for i in ref], traj_steps, traj_window, "fp64", warmup=traj_warmup)
+    # the model without the learned part from the same starting states
+    Xb = X0[ref].copy()
+    for j, i in enumerate(ref):
+        U0 = 2.0 * traj_cfg[i]["b1"] / 3.0
+        Xb[j] = Xb[j] + U0 * e0[i] * traj_steps
     travel = np.linalg.norm(Xr[:, -1] - Xr[:, 0], axis=1)
     err = np.linalg.norm(Xs[ref][:, -1] - Xr[:, -1], axis=1)
     err_b = np.linalg.norm(Xb - Xr[:, -1], axis=1)
+    start = np.linalg.norm(X0[ref] - Xr[:, 0], axis=1)
     ang = np.degrees(np.arccos(np.clip(np.sum(es[ref][:, -1] * er[:, -1], 1), -1, 1)))
-    rep["vs_fp64"] = dict(runs=n_ref, mean_travel=float(travel.mean()),
+    rep["vs_fp64"] = dict(runs=n_ref, mean_travel=float(travel.mean()), mean_start_offset=float(start.mean()),
                           mean_position_error=float(err.mean()), max_position_error=float(err.max()),
                           mean_position_error_free_space=float(err_b.mean()),
                           mean_heading_error_deg=float(ang.mean()),
-                          fallback_among_them=int(np.sum(unc[ref])))
+                          fallback_windows_among_them=int(fb[ref].sum()))
     log(f"[surrogate] vs FP64 LBM ({n_ref} runs, mean travel {travel.mean():.3f}): position error "
         f"{err.mean():.4f} (free-space model {err_b.mean():.4f}), heading error {ang.mean():.3f} deg")
     S.close()

@@ -46,7 +46,49 @@ def test_channel_draws_are_valid_and_reproducible(lbm):
     assert a == S.channel_configs(40, 3)
     for c in a:
         H, R = c["nz"], c["radius"]
-        assert 16 <= H <= 32 and 3 <= R <= 6 and (c["nx"], c["ny"]) == (4 * H, 2 * H)
+        assert 24 <= H <= 64 and 4 <= R <= 8 and (c["nx"], c["ny"]) == (4 * H, 2 * H)
+        assert c["b1"] == 1e-3 and c["walls_z"] == 1 and c["plane_xz"] == 1 and c["hmin"] == 1.0
+        assert -5.0 <= c["beta"] <= 5.0 and abs(c["angle"]) < 45.0
         d, _ = S.wall_frame(np.array([0, 0, c["z0"]]), H)
         assert d - R >= 1.5 - 1e-9
         lbm.validate(**c)
+
+
+class _Fixed(S.Surrogate):
+    """The integrator with a given prediction in place of the networks."""
+
+    def __init__(self, y):
+        self.y = np.asarray(y, float)
+
+    def predict(self, x):
+        return np.tile(self.y, (len(x), 1)), np.zeros(len(x))
+
+
+def test_advance_free_swimming_is_a_straight_line():
+    """Free-space outputs (U along e, no rotation): every configuration moves
+    U0 e per step, whatever side its nearest wall is on."""
+    cfg = [dict(radius=5.0, beta=0.0, nz=40, b1=1e-3, hmin=1.0)] * 2
+    th = 0.3
+    X0 = np.array([[10.0, 3.0, 12.0], [10.0, 3.0, 27.0]])      # nearer the lower / the upper wall
+    e0 = np.array([[math.cos(th), 0.0, math.sin(th)]] * 2)
+    # wall-frame outputs of free swimming: U = U0 (cos, 0, s sin), s the side
+    X, e = _Fixed([math.cos(th), 0, math.sin(th), 0, 0, 0]).advance(cfg[:1], X0[:1], e0[:1], 1000, 100)
+    assert np.allclose(X[0], X0[0] + 1000 * (2e-3 / 3) * e0[0]) and np.allclose(e, e0)
+    X, e = _Fixed([math.cos(th), 0, -math.sin(th), 0, 0, 0]).advance(cfg[1:], X0[1:], e0[1:], 1000, 100)
+    assert np.allclose(X[0], X0[1] + 1000 * (2e-3 / 3) * e0[1]) and np.allclose(e, e0[1:])
+
+
+def test_advance_rotation_and_hard_gap():
+    """A wall-frame rotation rate w (Omega_y R / U0) turns e at w U0 / R per
+    step about y (toward +x for the lower wall's frame); the centre stays at
+    least R + hmin + 0.5 from the wall it swims into."""
+    cfg = [dict(radius=4.0, beta=0.0, nz=30, b1=1e-3, hmin=1.0)]
+    U0, R = 2e-3 / 3, 4.0
+    X0 = np.array([[0.0, 0.0, 7.0]])
+    e0 = np.array([[1.0, 0.0, 0.0]])
+    w = 0.5
+    X, e = _Fixed([0, 0, 0, 0, w, 0]).advance(cfg, X0, e0, 100, 1)
+    ang = math.atan2(-e[0, 2], e[0, 0])   # Omega_y > 0 turns +x toward -z
+    assert ang == pytest.approx(100 * w * U0 / R, rel=1e-4)
+    X, e = _Fixed([0, 0, -5.0, 0, 0, 0]).advance(cfg, X0, e0, 1000, 10)   # straight into the lower wall
+    assert X[0, 2] == pytest.approx(0.5 + R + 1.0)
