@@ -32,18 +32,19 @@ hipError_t launch_wall_force(const S* f, const uint8_t* flag, const WallForceArg
 }
 
 template <class S>
-hipError_t launch_gather(const S* f, const uint8_t* flag, double* out, int NX, int NY, int NZ, uint32_t N,
-                         uint32_t SS, uint32_t PX, int shifted, const DevState* st, int shift_slot, hipStream_t s) {
-    hipLaunchKernelGGL((gather_kernel<S>), dim3((N + 255) / 256), dim3(256), 0, s, f, flag, out, NX, NY, NZ, N, SS,
-                       PX, shifted, st, shift_slot);
+hipError_t launch_gather(const S* f, const uint8_t* flag, double* out, int NX, int NY, int NZ, uint32_t n0,
+                         uint32_t cnt, uint32_t SS, uint32_t PX, int shifted, const DevState* st, int shift_slot,
+                         hipStream_t s) {
+    hipLaunchKernelGGL((gather_kernel<S>), dim3((cnt + 255) / 256), dim3(256), 0, s, f, flag, out, NX, NY, NZ, n0, cnt,
+                       SS, PX, shifted, st, shift_slot);
     return hipGetLastError();
 }
 
 template <class S>
-hipError_t launch_scatter(S* f, const uint8_t* flag, const double* in, int NX, int NY, int NZ, uint32_t N,
-                          uint32_t SS, uint32_t PX, int shifted, hipStream_t s) {
-    hipLaunchKernelGGL((scatter_kernel<S>), dim3((N + 255) / 256), dim3(256), 0, s, f, flag, in, NX, NY, NZ, N, SS,
-                       PX, shifted);
+hipError_t launch_scatter(S* f, const uint8_t* flag, const double* in, int NX, int NY, int NZ, uint32_t n0,
+                          uint32_t cnt, uint32_t SS, uint32_t PX, int shifted, hipStream_t s) {
+    hipLaunchKernelGGL((scatter_kernel<S>), dim3((cnt + 255) / 256), dim3(256), 0, s, f, flag, in, NX, NY, NZ, n0, cnt,
+                       SS, PX, shifted);
     return hipGetLastError();
 }
 
@@ -96,9 +97,9 @@ hipError_t launch_batch_squirmer(const BatchMember<S>* m, int count, long long t
     template hipError_t launch_links<S>(const LinkArgs<S>&, int, hipStream_t);                                   \
     template hipError_t launch_wall_force<S>(const S*, const uint8_t*, const WallForceArgs&, double*, DevState*, \
                                              int, int, int, uint32_t, uint32_t, hipStream_t);                             \
-    template hipError_t launch_gather<S>(const S*, const uint8_t*, double*, int, int, int, uint32_t, uint32_t,   \
+    template hipError_t launch_gather<S>(const S*, const uint8_t*, double*, int, int, int, uint32_t, uint32_t, uint32_t, \
                                          uint32_t, int, const DevState*, int, hipStream_t);                                \
-    template hipError_t launch_scatter<S>(S*, const uint8_t*, const double*, int, int, int, uint32_t, uint32_t,  \
+    template hipError_t launch_scatter<S>(S*, const uint8_t*, const double*, int, int, int, uint32_t, uint32_t, uint32_t, \
                                           uint32_t, int, hipStream_t);                                                     \
     template hipError_t launch_mass<S>(const S*, const uint8_t*, MassPart*, int, int, int, uint32_t, uint32_t,   \
                                        uint32_t, int, const DevState*, int, hipStream_t);                                  \

@@ -33,13 +33,16 @@ template <class S>
 hipError_t launch_wall_force(const S* f, const uint8_t* flag, const WallForceArgs& w, double* part, DevState* st,
                              int NX, int NY, int NZ, uint32_t SS, uint32_t PX, hipStream_t s);
 
+// Logical populations of the nodes n0 .. n0 + cnt - 1 to / from a staging
+// array [19][cnt] of doubles.
 template <class S>
-hipError_t launch_gather(const S* f, const uint8_t* flag, double* out, int NX, int NY, int NZ, uint32_t N,
-                         uint32_t SS, uint32_t PX, int shifted, const DevState* st, int shift_slot, hipStream_t s);
+hipError_t launch_gather(const S* f, const uint8_t* flag, double* out, int NX, int NY, int NZ, uint32_t n0,
+                         uint32_t cnt, uint32_t SS, uint32_t PX, int shifted, const DevState* st, int shift_slot,
+                         hipStream_t s);
 
 template <class S>
-hipError_t launch_scatter(S* f, const uint8_t* flag, const double* in, int NX, int NY, int NZ, uint32_t N,
-                          uint32_t SS, uint32_t PX, int shifted, hipStream_t s);
+hipError_t launch_scatter(S* f, const uint8_t* flag, const double* in, int NX, int NY, int NZ, uint32_t n0,
+                          uint32_t cnt, uint32_t SS, uint32_t PX, int shifted, hipStream_t s);
 
 template <class S>
 hipError_t launch_mass(const S* f, const uint8_t* flag, MassPart* part, int NX, int NY, int NZ, uint32_t N,
@@ -86,9 +89,9 @@ hipError_t launch_copy_probe(const void* a, void* b, size_t n16, hipStream_t s);
     extern template hipError_t launch_wall_force<S>(const S*, const uint8_t*, const WallForceArgs&, double*,      \
                                                     DevState*, int, int, int, uint32_t, uint32_t, hipStream_t);           \
     extern template hipError_t launch_gather<S>(const S*, const uint8_t*, double*, int, int, int, uint32_t,      \
-                                                uint32_t, uint32_t, int, const DevState*, int, hipStream_t);              \
+                                                uint32_t, uint32_t, uint32_t, int, const DevState*, int, hipStream_t);    \
     extern template hipError_t launch_scatter<S>(S*, const uint8_t*, const double*, int, int, int, uint32_t,     \
-                                                 uint32_t, uint32_t, int, hipStream_t);                                   \
+                                                 uint32_t, uint32_t, uint32_t, int, hipStream_t);                         \
     extern template hipError_t launch_mass<S>(const S*, const uint8_t*, MassPart*, int, int, int, uint32_t,      \
                                               uint32_t, uint32_t, int, const DevState*, int, hipStream_t);                \
     extern template hipError_t launch_fill_populations<S>(S*, uint32_t, hipStream_t);                         \

@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -94,6 +95,33 @@ struct lbm_hip_ctx {
     double radius_max = 0.0;      // radius the boundary work list is sized for
     int remaps_since_step = 0;    // lbm_hip_advance_squirmer calls since the last lbm_hip_step
 };
+
+// Staging of lbm_hip_download / lbm_hip_upload_populations: one array of
+// STAGE_NODES x 19 doubles (40 MB) per device for the life of the process,
+// allocated by the first transfer on that device and used under
+// g_stage_mu, so a transfer of any lattice moves in chunks through it and
+// allocates nothing once it exists.
+namespace {
+
+constexpr size_t STAGE_NODES = size_t(1) << 18;
+std::mutex g_stage_mu;
+
+int staging(int device, double** out) {
+    static std::vector<double*> buf;
+    if (device < 0) return fail(LBM_HIP_EINVAL, "device ordinal out of range");
+    if ((size_t)device >= buf.size()) buf.resize((size_t)device + 1, nullptr);
+    if (!buf[(size_t)device]) {
+        if (hipSetDevice(device) != hipSuccess) return fail(LBM_HIP_EDEVICE, "hipSetDevice failed");
+        if (hipMalloc((void**)&buf[(size_t)device], sizeof(double) * Q * STAGE_NODES) != hipSuccess) {
+            buf[(size_t)device] = nullptr;
+            return fail(LBM_HIP_ENOMEM, "transfer staging allocation failed");
+        }
+    }
+    *out = buf[(size_t)device];
+    return LBM_HIP_OK;
+}
+
+}  // namespace
 
 // Error text of the other entry points of the library (the MLP surrogate).
 void lbmhip_set_error_text(const char* msg) { g_err = msg; }
@@ -788,67 +816,78 @@ int lbm_hip_download(lbm_hip_ctx* c, double* f, double* rho, double* ux, double*
     const size_t N = c->N;
     if ((rho || ux || uy || uz) && !c->macro_valid)
         return fail(LBM_HIP_ESTATE, "the last step did not write the macroscopic fields");
-    if (f) {
-        double* tmp = nullptr;
-        if (hipMalloc((void**)&tmp, sizeof(double) * Q * N) != hipSuccess)
-            return fail(LBM_HIP_ENOMEM, "download staging allocation failed");
-        hipError_t e = hipSuccess;
-        const int shifted = layout_shifted(c) ? 1 : 0;
-        dispatch(c->prec, [&](auto* tag) -> int {
-            using S = std::remove_pointer_t<decltype(tag)>;
-            e = launch_gather<S>((const S*)c->f, c->flag, tmp, c->NX, c->NY, c->NZ, c->N, c->SS, c->PX, shifted, c->st,
-                                 (int)(c->t & 1), c->stream);
-            return 0;
-        });
-        if (e == hipSuccess) e = hipMemcpyAsync(f, tmp, sizeof(double) * Q * N, hipMemcpyDeviceToHost, c->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-        hipFree(tmp);
-        if (e != hipSuccess) return fail(LBM_HIP_EDEVICE, std::string("download f: ") + hipGetErrorString(e));
-    }
     double* outs[4] = {rho, ux, uy, uz};
     void* srcs[4] = {c->rho, c->ux, c->uy, c->uz};
-    if (rho || ux || uy || uz) {
-        double* tmp = nullptr;
-        if (hipMalloc((void**)&tmp, sizeof(double) * N) != hipSuccess)
-            return fail(LBM_HIP_ENOMEM, "download staging allocation failed");
-        for (int k = 0; k < 4; ++k) {
-            if (!outs[k]) continue;
-            hipError_t e = launch_macro_to_double(srcs[k], c->prec == LBM_HIP_FP64, tmp, c->N, c->stream);
-            if (e == hipSuccess) e = hipMemcpyAsync(outs[k], tmp, sizeof(double) * N, hipMemcpyDeviceToHost, c->stream);
-            if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-            if (e != hipSuccess) {
-                hipFree(tmp);
-                return fail(LBM_HIP_EDEVICE, std::string("download fields: ") + hipGetErrorString(e));
+    const bool macro_double = c->prec == LBM_HIP_FP64;
+    if (f || ((rho || ux || uy || uz) && !macro_double)) {
+        std::lock_guard<std::mutex> lock(g_stage_mu);
+        double* stage = nullptr;
+        if (int rc = staging(c->device, &stage)) return rc;
+        hipError_t e = hipSuccess;
+        // chunks of STAGE_NODES nodes; stream order keeps a chunk's copies
+        // ahead of the kernel that refills the staging array
+        if (f) {
+            const int shifted = layout_shifted(c) ? 1 : 0;
+            for (size_t n0 = 0; n0 < N && e == hipSuccess; n0 += STAGE_NODES) {
+                const uint32_t cnt = (uint32_t)std::min<size_t>(STAGE_NODES, N - n0);
+                dispatch(c->prec, [&](auto* tag) -> int {
+                    using S = std::remove_pointer_t<decltype(tag)>;
+                    e = launch_gather<S>((const S*)c->f, c->flag, stage, c->NX, c->NY, c->NZ, (uint32_t)n0, cnt, c->SS,
+                                         c->PX, shifted, c->st, (int)(c->t & 1), c->stream);
+                    return 0;
+                });
+                for (int i = 0; i < Q && e == hipSuccess; ++i)
+                    e = hipMemcpyAsync(f + (size_t)i * N + n0, stage + (size_t)i * cnt, sizeof(double) * cnt,
+                                       hipMemcpyDeviceToHost, c->stream);
             }
         }
-        hipFree(tmp);
+        if (!macro_double)
+            for (int k = 0; k < 4 && e == hipSuccess; ++k) {
+                if (!outs[k]) continue;
+                for (size_t n0 = 0; n0 < N && e == hipSuccess; n0 += STAGE_NODES * Q) {
+                    const uint32_t cnt = (uint32_t)std::min<size_t>(STAGE_NODES * Q, N - n0);
+                    e = launch_macro_to_double((const float*)srcs[k] + n0, 0, stage, cnt, c->stream);
+                    if (e == hipSuccess)
+                        e = hipMemcpyAsync(outs[k] + n0, stage, sizeof(double) * cnt, hipMemcpyDeviceToHost, c->stream);
+                }
+            }
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) return fail(LBM_HIP_EDEVICE, std::string("download: ") + hipGetErrorString(e));
     }
-    if (flag) {
-        HIP_TRY(hipMemcpyAsync(flag, c->flag, N, hipMemcpyDeviceToHost, c->stream));
-        HIP_TRY(hipStreamSynchronize(c->stream));
+    if (macro_double)
+        for (int k = 0; k < 4; ++k)
+            if (outs[k]) HIP_TRY(hipMemcpyAsync(outs[k], srcs[k], sizeof(double) * N, hipMemcpyDeviceToHost, c->stream));
+    if (flag) HIP_TRY(hipMemcpyAsync(flag, c->flag, N, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (flag)
         for (size_t n = 0; n < N; ++n) flag[n] &= TYPE_MASK;
-    }
     return check_device_error(c);
 }
 
 int lbm_hip_upload_populations(lbm_hip_ctx* c, const double* f) {
     if (!c || !f) return fail(LBM_HIP_EINVAL, "null argument");
     const size_t N = c->N;
-    double* tmp = nullptr;
-    if (hipMalloc((void**)&tmp, sizeof(double) * Q * N) != hipSuccess)
-        return fail(LBM_HIP_ENOMEM, "upload staging allocation failed");
-    hipError_t e = hipMemcpyAsync(tmp, f, sizeof(double) * Q * N, hipMemcpyHostToDevice, c->stream);
+    std::lock_guard<std::mutex> lock(g_stage_mu);
+    double* stage = nullptr;
+    if (int rc = staging(c->device, &stage)) return rc;
     const int shifted = layout_shifted(c) ? 1 : 0;
-    if (e == hipSuccess)
-        dispatch(c->prec, [&](auto* tag) -> int {
-            using S = std::remove_pointer_t<decltype(tag)>;
-            e = launch_scatter<S>((S*)c->f, c->flag, tmp, c->NX, c->NY, c->NZ, c->N, c->SS, c->PX, shifted, c->stream);
-            return 0;
-        });
+    hipError_t e = hipSuccess;
+    for (size_t n0 = 0; n0 < N && e == hipSuccess; n0 += STAGE_NODES) {
+        const uint32_t cnt = (uint32_t)std::min<size_t>(STAGE_NODES, N - n0);
+        for (int i = 0; i < Q && e == hipSuccess; ++i)
+            e = hipMemcpyAsync(stage + (size_t)i * cnt, f + (size_t)i * N + n0, sizeof(double) * cnt,
+                               hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess)
+            dispatch(c->prec, [&](auto* tag) -> int {
+                using S = std::remove_pointer_t<decltype(tag)>;
+                e = launch_scatter<S>((S*)c->f, c->flag, stage, c->NX, c->NY, c->NZ, (uint32_t)n0, cnt, c->SS, c->PX,
+                                      shifted, c->stream);
+                return 0;
+            });
+    }
     const double zero = 0.0;
     if (e == hipSuccess) e = hipMemcpyAsync(&c->st->shift[c->t & 1], &zero, sizeof zero, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    hipFree(tmp);
     if (e != hipSuccess) return fail(LBM_HIP_EDEVICE, std::string("upload: ") + hipGetErrorString(e));
     c->mass_valid = false;
     return LBM_HIP_OK;

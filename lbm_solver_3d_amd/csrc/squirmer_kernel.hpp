@@ -560,14 +560,17 @@ __global__ __launch_bounds__(SQ_THREADS) void squirmer_kernel(const RemapArgs<S>
 
 // ---------------------------------------------------------------------------
 // Logical population f_i(n) for download / upload (either layout; for
-// non-fluid nodes slot i at n).
+// non-fluid nodes slot i at n), for the nodes n0 .. n0 + cnt - 1: value i of
+// node n0 + k at out[i * cnt + k] (in[i * cnt + k]).
 // ---------------------------------------------------------------------------
 template <class S>
 __global__ __launch_bounds__(256) void gather_kernel(const S* __restrict__ f, const uint8_t* __restrict__ flag,
-                                                     double* __restrict__ out, int NX, int NY, int NZ, uint32_t N,
-                                                     uint32_t SS, uint32_t PX, int shifted, const DevState* st, int shift_slot) {
-    const uint32_t n = blockIdx.x * 256u + threadIdx.x;
-    if (n >= N) return;
+                                                     double* __restrict__ out, int NX, int NY, int NZ, uint32_t n0,
+                                                     uint32_t cnt, uint32_t SS, uint32_t PX, int shifted,
+                                                     const DevState* st, int shift_slot) {
+    const uint32_t k = blockIdx.x * 256u + threadIdx.x;
+    if (k >= cnt) return;
+    const uint32_t n = n0 + k;
     const int x = (int)(n % (uint32_t)NX), y = (int)((n / (uint32_t)NX) % (uint32_t)NY),
               z = (int)(n / ((uint32_t)NX * (uint32_t)NY));
     const bool fluid = (flag[n] & TYPE_MASK) == FLUID;
@@ -581,16 +584,17 @@ __global__ __launch_bounds__(256) void gather_kernel(const S* __restrict__ f, co
         }
         double v = to_real<S>(f[slot], i);
         if (fluid && dsh != 0.0) v += W[i] * dsh;
-        out[(size_t)i * N + n] = v;
+        out[(size_t)i * cnt + k] = v;
     }
 }
 
 template <class S>
 __global__ __launch_bounds__(256) void scatter_kernel(S* __restrict__ f, const uint8_t* __restrict__ flag,
-                                                      const double* __restrict__ in, int NX, int NY, int NZ, uint32_t N,
-                                                      uint32_t SS, uint32_t PX, int shifted) {
-    const uint32_t n = blockIdx.x * 256u + threadIdx.x;
-    if (n >= N) return;
+                                                      const double* __restrict__ in, int NX, int NY, int NZ, uint32_t n0,
+                                                      uint32_t cnt, uint32_t SS, uint32_t PX, int shifted) {
+    const uint32_t k = blockIdx.x * 256u + threadIdx.x;
+    if (k >= cnt) return;
+    const uint32_t n = n0 + k;
     const int x = (int)(n % (uint32_t)NX), y = (int)((n / (uint32_t)NX) % (uint32_t)NY),
               z = (int)(n / ((uint32_t)NX * (uint32_t)NY));
     // only FLUID nodes own populations: the slots of solid nodes may hold the
@@ -604,7 +608,7 @@ __global__ __launch_bounds__(256) void scatter_kernel(S* __restrict__ f, const u
             const uint32_t m = (uint32_t)xm + (uint32_t)NX * ((uint32_t)ym + (uint32_t)NY * (uint32_t)zm);
             if (shifted || (flag[m] & TYPE_MASK) != FLUID) slot = (size_t)OPP[i] * SS + pop_index(xm, ym, zm, PX, NY);
         }
-        f[slot] = from_real<S>(in[(size_t)i * N + n], i);
+        f[slot] = from_real<S>(in[(size_t)i * cnt + k], i);
     }
 }
 
