@@ -150,6 +150,25 @@ def measure_ensemble(lbm, steps: int, warmup: int, count: int = 64, sequential: 
                 step_gbs=step_bytes / (ms / steps) / 1e6)
 
 
+def sweep_ensemble_tiling(lbm, steps: int, count: int = 64) -> dict:
+    """Every tile shape the batch tuner tries, `steps` graph-replayed steps
+    each, in the tuner's order (the candidates come from one short tuner
+    pass that runs first): per-shape step times, and under rocprofv3 per
+    dispatch counters that follow the same order."""
+    members = lbm.ensemble_members(count, 0)
+    rows = []
+    with lbm.Batch(members, precision="fp16") as B:
+        cands = B.autotune(trial_steps=2, mass_every=MASS_EVERY)["candidates"]
+        for c in cands:
+            B.set_tiling(c["tile_width"], c["xcd_remap"])
+            ms = B.time_run(steps, MASS_EVERY)
+            rows.append(dict(tile_width=c["tile_width"], xcd_remap=c["xcd_remap"], ms_per_step=round(ms / steps, 4)))
+            print(f"[sweep] tile width {c['tile_width']}, XCD order {c['xcd_remap']}: {ms / steps:.3f} ms per step",
+                  file=sys.stderr, flush=True)
+        nodes = int(B.info()["total_nodes"])
+    return dict(workload="ensemble tiling sweep", runs=count, total_nodes=nodes, steps=steps, rows=rows)
+
+
 def measure_mlp(lbm, samples: int = 4096, epochs: int = 20, batch: int = 256) -> dict:
     """Training throughput of the surrogate's networks alone: 4 nets
     5-128-128-128-6 (BF16 operands, FP32 accumulation) on synthetic data."""
@@ -168,6 +187,21 @@ def measure_mlp(lbm, samples: int = 4096, epochs: int = 20, batch: int = 256) ->
             "higher_is_better": True, "dtype": "bf16", "data": "synthetic",
             "config": {"workload": f"{samples} samples, {epochs} epochs, minibatch {batch}"},
             "final_loss": [round(float(v), 6) for v in loss]}
+
+
+def measure_copy(lbm, reps: int = 10) -> dict:
+    """Streaming-copy bandwidth (lbm_hip_copy_bandwidth_parts) against the
+    footprint: the bytes one collide-stream launch moves for the 512x256x128
+    channel in FP16 (1.27 GB) up to those of the 64-run ensemble (15.2 GB),
+    in one pair of buffers and in 64 pairs copied by one launch (the
+    ensemble's members are separate allocations)."""
+    cases = [(1.275e9, 1), (2.55e9, 1), (5.1e9, 1), (10.2e9, 1), (15.22e9, 1), (15.22e9, 64), (1.275e9, 64)]
+    rows = []
+    for nbytes, parts in cases:
+        g = lbm.copy_bandwidth(int(nbytes), reps=reps, parts=parts)
+        rows.append({"bytes": int(nbytes), "parts": parts, "GBps": round(g, 1)})
+        print(f"[copy] {nbytes / 1e9:.2f} GB in {parts} part(s): {g:.0f} GB/s", file=sys.stderr, flush=True)
+    return {"workload": "copy probe", "reps": reps, "rows": rows}
 
 
 # The same slit channel as the reference can express it (its orientation
@@ -285,15 +319,18 @@ def main():
     ap.add_argument("--warmup", type=int, default=500)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--precisions", default="fp16,fp32,fp64")
-    ap.add_argument("--workload", default="channel", choices=["channel", "ensemble", "surrogate", "trajectory", "mlp"],
+    ap.add_argument("--workload", default="channel",
+                    choices=["channel", "ensemble", "surrogate", "trajectory", "mlp", "copy"],
                     help="channel: the 512x256x128 slit channel (headline); ensemble: 64 slit-channel runs batched; "
                          "surrogate: the learned velocity surrogate with LBM fallback, 1000 trajectories; "
                          "trajectory: the 512x256x128 pusher in FP32 / FP16 / BF16 storage against FP64 "
                          "(--steps steps); mlp: training of the surrogate's 4-net ensemble alone on 4096 "
-                         "synthetic samples, 20 epochs of 256-sample minibatches")
+                         "synthetic samples, 20 epochs of 256-sample minibatches; copy: the streaming-copy "
+                         "probe over footprints from 1.27 GB to 15.2 GB, in one allocation pair and in 64")
     ap.add_argument("--tiling", default=None,
                     help="WIDTH,XCD: use this collide-stream tile shape instead of autotuning (e.g. 256,0; "
-                         "width 0: row runs); channel and ensemble workloads")
+                         "width 0: row runs); channel and ensemble workloads.  'sweep' (ensemble): time "
+                         "--steps steps of every shape the tuner tries, in its order")
     ap.add_argument("--no-sequential", action="store_true",
                     help="ensemble: skip timing the runs one after another")
     ap.add_argument("--copy-probe", action="store_true",
@@ -317,6 +354,9 @@ def main():
     lbm.load_library()
     if args.workload == "ensemble":
         print(f"[bench] ensemble: warmup {args.warmup}, timed {args.steps}", file=sys.stderr, flush=True)
+        if args.tiling == "sweep":
+            print(json.dumps(sweep_ensemble_tiling(lbm, args.steps)))
+            return
         tiling = tuple(int(v) for v in args.tiling.split(",")) if args.tiling else None
         e = measure_ensemble(lbm, args.steps, args.warmup, sequential=not args.no_sequential, tiling=tiling)
         print(json.dumps({
@@ -349,6 +389,9 @@ def main():
         return
     if args.workload == "mlp":
         print(json.dumps(measure_mlp(lbm)))
+        return
+    if args.workload == "copy":
+        print(json.dumps(measure_copy(lbm)))
         return
     if args.workload == "surrogate":
         from lbm_solver_3d_amd import surrogate as S

@@ -268,6 +268,11 @@ int lbm_hip_batch_time_run(lbm_hip_batch* b, int steps, int mass_every, double* 
  * reports the bandwidth in GB/s counting bytes read plus written.
  * Allocates the two buffers and frees them before returning. */
 int lbm_hip_copy_bandwidth(int device, size_t bytes, int reps, double* gbs);
+/* The same copy spread over `parts` separate allocations (1..4096) of
+ * bytes / parts each, all parts in one launch, as the batched ensemble
+ * covers its members: separates the cost of the total footprint from the
+ * cost of many allocations.  parts = 1 is lbm_hip_copy_bandwidth. */
+int lbm_hip_copy_bandwidth_parts(int device, size_t bytes, int parts, int reps, double* gbs);
 
 /* Message of the last failed call on this thread ("" if none). */
 const char* lbm_hip_last_error(void);

@@ -175,6 +175,7 @@ def load_library(path: Path | None = None):
     lib.lbm_hip_batch_set_tiling.argtypes = [bt, C.c_int, C.c_int]
     lib.lbm_hip_batch_time_run.argtypes = [bt, C.c_int, C.c_int, dp]
     lib.lbm_hip_copy_bandwidth.argtypes = [C.c_int, C.c_size_t, C.c_int, dp]
+    lib.lbm_hip_copy_bandwidth_parts.argtypes = [C.c_int, C.c_size_t, C.c_int, C.c_int, dp]
     fp = C.POINTER(C.c_float)
     mh = C.c_void_p
     lib.lbm_mlp_create.argtypes = [C.c_int, C.c_int, C.c_ulonglong, C.c_int, C.POINTER(C.c_void_p)]
@@ -199,12 +200,13 @@ def _check(lib, rc: int):
         raise LbmHipError(rc, lib.lbm_hip_last_error().decode())
 
 
-def copy_bandwidth(nbytes: int, reps: int = 20, device: int = 0) -> float:
-    """Device streaming-copy bandwidth in GB/s (bytes read + written) over two
-    buffers of nbytes / 2 bytes (lbm_hip_copy_bandwidth)."""
+def copy_bandwidth(nbytes: int, reps: int = 20, device: int = 0, parts: int = 1) -> float:
+    """Device streaming-copy bandwidth in GB/s (bytes read + written) moving
+    nbytes in total per copy, over `parts` pairs of separate buffers copied
+    in one launch (lbm_hip_copy_bandwidth_parts)."""
     lib = load_library()
     out = C.c_double()
-    _check(lib, lib.lbm_hip_copy_bandwidth(int(device), int(nbytes), int(reps), C.byref(out)))
+    _check(lib, lib.lbm_hip_copy_bandwidth_parts(int(device), int(nbytes), int(parts), int(reps), C.byref(out)))
     return out.value
 
 

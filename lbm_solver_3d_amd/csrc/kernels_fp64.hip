@@ -416,4 +416,26 @@ hipError_t launch_copy_probe(const void* a, void* b, size_t n16, hipStream_t s) 
     return hipGetLastError();
 }
 
+// The same copy over `parts` separate allocations in one launch, as the
+// ensemble kernels cover their members: blocks [p * bpp, (p + 1) * bpp)
+// stream part p (a[p] -> b[p], n16 16-byte words each).
+__global__ __launch_bounds__(256) void copy_parts_kernel(const word4* const* __restrict__ a, word4* const* __restrict__ b,
+                                                         size_t n16, unsigned bpp) {
+    const unsigned p = blockIdx.x / bpp, k = blockIdx.x % bpp;
+    const word4* src = a[p];
+    word4* dst = b[p];
+    const size_t stride = (size_t)bpp * 256u;
+    for (size_t i = (size_t)k * 256u + threadIdx.x; i < n16; i += stride)
+        __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
+}
+
+hipError_t launch_copy_parts(const void* const* a, void* const* b, int parts, size_t n16, hipStream_t s) {
+    const size_t want = (n16 + 255) / 256;
+    const unsigned per = 65536u / (unsigned)parts;
+    const unsigned bpp = (unsigned)(want < per ? (want ? want : 1) : per);
+    hipLaunchKernelGGL(copy_parts_kernel, dim3(bpp * (unsigned)parts), dim3(256), 0, s, (const word4* const*)a,
+                       (word4* const*)b, n16, bpp);
+    return hipGetLastError();
+}
+
 }  // namespace lbmhip

@@ -79,6 +79,9 @@ hipError_t launch_init_flags(uint8_t* flag, int NX, int NY, int NZ, uint32_t N, 
                              const uint32_t* particle_nodes, int nparticle, hipStream_t s);
 hipError_t launch_macro_to_double(const void* src, int is_double, double* dst, uint32_t N, hipStream_t s);
 hipError_t launch_copy_probe(const void* a, void* b, size_t n16, hipStream_t s);
+// The copy probe over `parts` separate allocations (a[p] -> b[p], n16
+// 16-byte words each; parts <= 65536), one launch.
+hipError_t launch_copy_parts(const void* const* a, void* const* b, int parts, size_t n16, hipStream_t s);
 
 #define LBMHIP_DECLARE(S)                                                                                          \
     extern template hipError_t launch_step<S>(const StepArgs<S>&, const FastCoef&, bool, int, hipStream_t);     \

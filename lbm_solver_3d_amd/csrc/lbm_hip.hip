@@ -1518,23 +1518,31 @@ int lbm_hip_batch_time_run(lbm_hip_batch* b, int steps, int mass_every, double* 
 }
 
 int lbm_hip_copy_bandwidth(int device, size_t bytes, int reps, double* gbs) {
-    if (!gbs || reps <= 0 || bytes < 512) return fail(LBM_HIP_EINVAL, "copy_bandwidth: bytes >= 512, reps > 0, gbs");
-    const size_t n16 = bytes / 2 / 16;
+    return lbm_hip_copy_bandwidth_parts(device, bytes, 1, reps, gbs);
+}
+
+int lbm_hip_copy_bandwidth_parts(int device, size_t bytes, int parts, int reps, double* gbs) {
+    if (!gbs || reps <= 0 || parts < 1 || parts > 4096 || bytes < 512 * (size_t)parts)
+        return fail(LBM_HIP_EINVAL, "copy_bandwidth: bytes >= 512 * parts, 1 <= parts <= 4096, reps > 0, gbs");
+    const size_t n16 = bytes / (size_t)parts / 2 / 16;  // 16-byte words per part and direction
     HIP_TRY(hipSetDevice(device));
-    void* a = nullptr;
-    void* b = nullptr;
+    std::vector<void*> ptr(2 * (size_t)parts, nullptr);  // sources, then destinations
+    void** table = nullptr;
     hipStream_t s = nullptr;
     hipEvent_t ev[2] = {nullptr, nullptr};
     float ms = 0.f;
-    hipError_t e = hipMalloc(&a, n16 * 16);
-    if (e == hipSuccess) e = hipMalloc(&b, n16 * 16);
+    hipError_t e = hipSuccess;
+    for (size_t k = 0; k < ptr.size() && e == hipSuccess; ++k) e = hipMalloc(&ptr[k], n16 * 16);
+    if (e == hipSuccess) e = hipMalloc((void**)&table, sizeof(void*) * ptr.size());
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMemcpyAsync(table, ptr.data(), sizeof(void*) * ptr.size(), hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipEventCreate(&ev[0]);
     if (e == hipSuccess) e = hipEventCreate(&ev[1]);
-    if (e == hipSuccess) e = hipMemsetAsync(a, 0, n16 * 16, s);
-    for (int r = 0; r < 2 && e == hipSuccess; ++r) e = launch_copy_probe(a, b, n16, s);
+    for (int k = 0; k < parts && e == hipSuccess; ++k) e = hipMemsetAsync(ptr[(size_t)k], 0, n16 * 16, s);
+    auto copy = [&]() { return launch_copy_parts(table, table + parts, parts, n16, s); };
+    for (int r = 0; r < 2 && e == hipSuccess; ++r) e = copy();
     if (e == hipSuccess) e = hipEventRecord(ev[0], s);
-    for (int r = 0; r < reps && e == hipSuccess; ++r) e = launch_copy_probe(a, b, n16, s);
+    for (int r = 0; r < reps && e == hipSuccess; ++r) e = copy();
     if (e == hipSuccess) e = hipEventRecord(ev[1], s);
     if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
     if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev[0], ev[1]);
@@ -1542,10 +1550,11 @@ int lbm_hip_copy_bandwidth(int device, size_t bytes, int reps, double* gbs) {
     for (hipEvent_t v : ev)
         if (v) hipEventDestroy(v);
     if (s) hipStreamDestroy(s);
-    if (a) hipFree(a);
-    if (b) hipFree(b);
+    for (void* q : ptr)
+        if (q) hipFree(q);
+    if (table) hipFree(table);
     if (e != hipSuccess) return fail(LBM_HIP_EDEVICE, std::string("copy_bandwidth: ") + hipGetErrorString(e));
-    *gbs = 2.0 * (double)(n16 * 16) * reps / (ms * 1e-3) / 1e9;
+    *gbs = 2.0 * (double)(n16 * 16) * parts * reps / (ms * 1e-3) / 1e9;
     return LBM_HIP_OK;
 }
 

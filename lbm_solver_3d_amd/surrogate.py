@@ -150,42 +150,61 @@ def lbm_trajectories(configs: list[dict], steps: int, every: int, precision: str
     return np.array(Xs).transpose(1, 0, 2), np.array(es).transpose(1, 0, 2)
 
 
-def lbm_windows(configs: list[dict], X: np.ndarray, e: np.ndarray, steps: int, warmup: int, hold_every: int = 100,
-                precision: str = "fp16", batch: int = 128):
-    """The LBM fallback for one window of each listed trajectory: a lattice
-    of the trajectory's channel with the squirmer at its current position X
-    and orientation e, a warm-up of `warmup` steps in which the body is put
-    back at (X, e) every `hold_every` steps (its velocities keep relaxing, so
-    the flow settles around the body where it is), then `steps` free steps.
-    Returns the positions and orientations after the window."""
-    Xo, eo = np.array(X, float).copy(), np.array(e, float).copy()
-    for a in range(0, len(configs), batch):
-        sl = slice(a, a + batch)
-        cfg, Xs, es = configs[sl], Xo[sl].copy(), eo[sl].copy()
-        kv, start = [], []
-        for c, x in zip(cfg, Xs):
+class LBMFollowers:
+    """The LBM fallback: a batch of lattices, each following one trajectory
+    through the windows where the ensemble is uncertain.  A lattice is built
+    from the trajectory's channel with the squirmer at the trajectory's
+    current position X and orientation e; during `warmup` steps the body is
+    put back at (X, e) every `hold_every` steps (its velocities keep
+    relaxing, so the flow settles around the body where it is).  After that
+    every `advance` runs free LBM steps and moves the followed trajectories by
+    their lattices' displacements, window after window, without another
+    warm-up.  A trajectory that the ensemble is sure of again is released:
+    its lattice no longer follows it (it keeps running with the batch until
+    every member is released)."""
+
+    def __init__(self, configs: list[dict], index, X, e, warmup: int, hold_every: int = 100,
+                 precision: str = "fp16"):
+        self.index = [int(i) for i in index]
+        self.alive = [True] * len(self.index)
+        kv, self.ref = [], []
+        for c, x in zip(configs, X):
             k = dict(c)
             x0 = np.array([x[0] % c["nx"], x[1] % c["ny"], x[2]])
             k.update(x0=float(x0[0]), y0=float(x0[1]), z0=float(x0[2]), angle=0.0)
             kv.append(k)
-            start.append(x0)
-        with Batch(kv, precision=precision) as B:
-            held = [dict(X=list(x0), e=list(ee)) for x0, ee in zip(start, es)]
-            for i, h in enumerate(held):
-                B.member(i).set_squirmer(**h)
-            done = 0
-            while done < warmup:
-                n = min(hold_every, warmup - done)
-                B.run(n, MASS_EVERY, 0)
-                done += n
-                for i, h in enumerate(held):
-                    B.member(i).set_squirmer(**h)
-            B.run(steps, MASS_EVERY, 0)
-            for i in range(len(cfg)):
-                q = B.member(i).squirmer()
-                Xo[a + i] = Xs[i] + (q["X"] - start[i])   # the displacement, in the trajectory's unwrapped frame
-                eo[a + i] = q["e"]
-    return Xo, eo
+            self.ref.append(x0)
+        self.B = Batch(kv, precision=precision)
+        held = [dict(X=list(x0), e=list(ee)) for x0, ee in zip(self.ref, e)]
+        for j, h in enumerate(held):
+            self.B.member(j).set_squirmer(**h)
+        done = 0
+        while done < warmup:
+            n = min(hold_every, warmup - done)
+            self.B.run(n, MASS_EVERY, 0)
+            done += n
+            for j, h in enumerate(held):
+                self.B.member(j).set_squirmer(**h)
+
+    def release(self, i: int):
+        self.alive[self.index.index(i)] = False
+
+    def active(self) -> bool:
+        return any(self.alive)
+
+    def advance(self, steps: int, X: np.ndarray, e: np.ndarray):
+        """`steps` LBM steps; X[i], e[i] of every followed trajectory i move
+        with its lattice (X in the trajectory's unwrapped frame)."""
+        self.B.run(steps, MASS_EVERY, 0)
+        for j, i in enumerate(self.index):
+            if self.alive[j]:
+                q = self.B.member(j).squirmer()
+                X[i] = X[i] + (q["X"] - self.ref[j])
+                e[i] = q["e"]
+                self.ref[j] = np.array(q["X"], float)
+
+    def close(self):
+        self.B.close()
 
 
 def _features_v(X, e, R, beta, nz):
@@ -248,14 +267,19 @@ class Surrogate:
         return X, e
 
     def trajectories(self, configs: list[dict], X0, e0, steps: int, every: int, window: int,
-                     spread_limit: float = 0.02, fallback_warmup: int = 2000, fallback_precision: str = "fp16"):
+                     spread_limit: float = 0.02, fallback_warmup: int = 2000, fallback_precision: str = "fp16",
+                     batch: int = 128, log=None):
         """Integrate every configuration from (X0, e0) for `steps` steps in
         windows of `window` steps.  A window whose starting configuration has
-        an ensemble spread above `spread_limit` is run by the LBM
-        (lbm_windows) instead of the surrogate.  Returns positions and
-        orientations at the window ends (runs, windows + 1, 3), the number of
-        windows run by the LBM per run, and the seconds spent in the
-        surrogate and in the fallback."""
+        an ensemble spread above `spread_limit` is run by the LBM instead of
+        the surrogate: by the lattice already following that trajectory if
+        the previous window was uncertain too, otherwise by a new one
+        (LBMFollowers, `batch` trajectories per batch, warmed up from the
+        trajectory's state).  Returns positions and orientations at the
+        window ends (runs, windows + 1, 3), the number of windows run by the
+        LBM per run, the seconds spent in the surrogate and in the fallback,
+        and counts of the fallback's work (lattices started, member-windows
+        run).  `log`, if given, receives one line per window."""
         n = len(configs)
         X, e = np.array(X0, float).copy(), np.array(e0, float).copy()
         R = np.array([c["radius"] for c in configs])
@@ -264,28 +288,50 @@ class Surrogate:
         Xs, es = [X.copy()], [e.copy()]
         fallback = np.zeros(n, int)
         t_sur = t_fb = 0.0
-        for _ in range(steps // window):
-            t0 = time.perf_counter()
-            _, sp = self.predict(_features_v(X, e, R, beta, nz)[0])
-            unc = sp > spread_limit
-            ok = np.flatnonzero(~unc)
-            if len(ok):
-                X[ok], e[ok] = self.advance([configs[i] for i in ok], X[ok], e[ok], window, every)
-            t1 = time.perf_counter()
-            bad = np.flatnonzero(unc)
-            if len(bad):
-                X[bad], e[bad] = lbm_windows([configs[i] for i in bad], X[bad], e[bad], window, fallback_warmup,
-                                             precision=fallback_precision)
-                fallback[bad] += 1
-            t_sur += t1 - t0
-            t_fb += time.perf_counter() - t1
-            Xs.append(X.copy())
-            es.append(e.copy())
-        return (np.array(Xs).transpose(1, 0, 2), np.array(es).transpose(1, 0, 2), fallback, t_sur, t_fb)
+        groups, follower = [], {}
+        started = member_windows = 0
+        try:
+            for _ in range(steps // window):
+                t0 = time.perf_counter()
+                _, sp = self.predict(_features_v(X, e, R, beta, nz)[0])
+                unc = sp > spread_limit
+                for i in [i for i in follower if not unc[i]]:
+                    follower.pop(i).release(i)
+                ok = np.flatnonzero(~unc)
+                if len(ok):
+                    X[ok], e[ok] = self.advance([configs[i] for i in ok], X[ok], e[ok], window, every)
+                t1 = time.perf_counter()
+                for g in [g for g in groups if not g.active()]:
+                    g.close()
+                    groups.remove(g)
+                new = [int(i) for i in np.flatnonzero(unc) if int(i) not in follower]
+                for a in range(0, len(new), batch):
+                    idx = new[a:a + batch]
+                    g = LBMFollowers([configs[i] for i in idx], idx, X[idx], e[idx], fallback_warmup,
+                                     precision=fallback_precision)
+                    groups.append(g)
+                    follower.update({i: g for i in idx})
+                    started += len(idx)
+                for g in groups:
+                    g.advance(window, X, e)
+                    member_windows += len(g.index)
+                fallback[unc] += 1
+                t_sur += t1 - t0
+                t_fb += time.perf_counter() - t1
+                if log:
+                    log(f"[surrogate] window {len(Xs)}: {len(ok)} by the surrogate, {int(unc.sum())} by the LBM "
+                        f"({len(new)} new lattices; {t_sur:.2f} s / {t_fb:.1f} s so far)")
+                Xs.append(X.copy())
+                es.append(e.copy())
+        finally:
+            for g in groups:
+                g.close()
+        info = dict(lattices_started=started, member_windows=member_windows)
+        return (np.array(Xs).transpose(1, 0, 2), np.array(es).transpose(1, 0, 2), fallback, t_sur, t_fb, info)
 
 
 def study(n_train_runs=256, n_test_runs=64, n_traj=1000, n_ref=16, seed=0, H=(24, 64), warmup=6000, window=3000,
-          windows=5, epochs=600, traj_warmup=3000, traj_steps=8000, every=100, traj_window=1000,
+          windows=10, epochs=600, traj_warmup=3000, traj_steps=8000, every=100, traj_window=1000,
           fallback_warmup=2000, fallback_precision="fp16", log=print):
     """The surrogate end to end: training data from batched LBM runs, 4-net
     training on the GPU, held-out error against the free-space model, 1000
@@ -302,6 +348,8 @@ def study(n_train_runs=256, n_test_runs=64, n_traj=1000, n_ref=16, seed=0, H=(24
             x, y = lbm_samples(cfg[a:a + 64], sample_at)
             xs.append(x)
             ys.append(y)
+            log(f"[surrogate] data: runs {a}..{min(a + 64, len(cfg)) - 1} of {len(cfg)} done "
+                f"({time.perf_counter() - t0:.1f} s)")
         return np.concatenate(xs), np.concatenate(ys)
 
     train_cfg = channel_configs(n_train_runs, seed, H=H)
@@ -335,14 +383,16 @@ def study(n_train_runs=256, n_test_runs=64, n_traj=1000, n_ref=16, seed=0, H=(24
         Xw, ew = lbm_trajectories(traj_cfg[a:a + 128], 0, 1, fallback_precision, warmup=traj_warmup)
         X0 += list(Xw[:, 0])
         e0 += list(ew[:, 0])
+        log(f"[surrogate] trajectory warm-up: {min(a + 128, n_traj)} of {n_traj} ({time.perf_counter() - t2:.1f} s)")
     X0, e0 = np.array(X0), np.array(e0)
     t_warm = time.perf_counter() - t2
-    Xs, es, fb, t_sur, t_fb = S.trajectories(traj_cfg, X0, e0, traj_steps, every, traj_window,
-                                             fallback_warmup=fallback_warmup, fallback_precision=fallback_precision)
+    Xs, es, fb, t_sur, t_fb, info = S.trajectories(traj_cfg, X0, e0, traj_steps, every, traj_window,
+                                                   fallback_warmup=fallback_warmup,
+                                                   fallback_precision=fallback_precision, log=log)
     nwin = traj_steps // traj_window
     rep["trajectories"] = dict(count=n_traj, steps=traj_steps, window=traj_window, windows=nwin,
                                fallback_windows=int(fb.sum()), runs_with_fallback=int(np.sum(fb > 0)),
-                               surrogate_seconds=t_sur, fallback_seconds=t_fb, warmup_seconds=t_warm)
+                               surrogate_seconds=t_sur, fallback_seconds=t_fb, warmup_seconds=t_warm, **info)
     log(f"[surrogate] {n_traj} trajectories x {traj_steps} steps: surrogate {t_sur:.2f} s, "
         f"{int(fb.sum())} of {n_traj * nwin} windows by the LBM ({t_fb:.1f} s)")
 
