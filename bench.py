@@ -298,18 +298,14 @@ def measured_ensemble_traffic(tile_width: int, xcd_remap: int):
     """HBM bytes per launch of the batched collide-stream kernel for one tile
     shape (mean of the even and odd launches, which alternate), from the
     counter runs recorded in profiles/; None for a shape not recorded there."""
-    shape = f"{tile_width},{xcd_remap}"
-    mb = {}
     try:
         for line in ENSEMBLE_TRAFFIC_FILE.read_text().splitlines():
             p = line.split()
-            if len(p) == 8 and p[0] == shape and p[1] in ("even", "odd"):
-                mb[p[1]] = float(p[5])
-    except OSError:
-        return None
-    if len(mb) != 2:
-        return None
-    return (mb["even"] + mb["odd"]) / 2 * 1e6
+            if len(p) == 4 and p[0] == "traffic_bytes_per_launch" and (int(p[1]), int(p[2])) == (tile_width, xcd_remap):
+                return float(p[3])
+    except (OSError, ValueError):
+        pass
+    return None
 
 
 def main():
