@@ -223,9 +223,9 @@ def cpu_model() -> str:
 
 def cpu_threads() -> tuple[int, int]:
     """(threads used, cores this process may run on).  The threads are
-    OMP_NUM_THREADS when the environment sets it (a GPU box allots each GPU
-    a share of its host cores and sets it to that share), else every core
-    of the affinity mask."""
+    OMP_NUM_THREADS when the environment sets it (a host shared by several
+    GPUs gives each one a share of its cores and sets it to that share),
+    else every core of the affinity mask."""
     try:
         aff = len(os.sched_getaffinity(0))
     except AttributeError:
