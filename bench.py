@@ -1,19 +1,25 @@
 #!/usr/bin/env python3
 """Benchmark: MLUPS and % of HBM peak of the D3Q19 TRT step with squirmer
-coupling on the 512x256x128 slit channel (pusher, R = 12), FP16-store vs
-FP32, on one MI355X.
+coupling on the 512x256x128 slit channel (pusher, R = 12) in FP16, FP32 and
+FP64 storage, on one MI355X.
 
 A "step" is one full time step of the driver loop (reference main.cpp:60-64):
-the fused collide-stream kernel, the device-side squirmer update + remap,
+the fused collide-stream kernels, the device-side squirmer update + remap,
 and the mass correction every 10 steps.  Populations, flags and the
 squirmer state are resident in HBM when the timed region starts.
 
 Prints one JSON line.  `value` is the FP16-store whole-step MLUPS; the FP32
-figure, the ratio and the roofline of the dominant kernel (the fused step
-kernel, timed with HIP events on its own stream) are reported beside it.
+and FP64 figures and the FP16/FP32 ratio are reported beside it, and the
+roofline of the collide-stream update (the bulk kernel plus, on even steps,
+the boundary kernel, timed with a HIP event pair on the step's stream).
 The CPU baseline is the reference's own step (compiled from its sources
 into oracle/_ref) timed on this host's cores on a bounded sample of the
-same workload, or the oracle restatement when that build is absent.
+same workload (y and z relabelled, as the reference can express it), or
+the oracle restatement when that build is absent.
+
+Other workloads (--workload): the 64-run ensemble, the learned surrogate,
+the storage-precision trajectories, the surrogate's training kernels, the
+streaming-copy probe and the host transfers.
 """
 from __future__ import annotations
 
@@ -189,6 +195,43 @@ def measure_mlp(lbm, samples: int = 4096, epochs: int = 20, batch: int = 256) ->
             "final_loss": [round(float(v), 6) for v in loss]}
 
 
+def measure_transfer(lbm, reps: int = 3) -> dict:
+    """Host transfers on the 512x256x128 channel (lbm_hip_download /
+    lbm_hip_upload_populations through the library's chunked staging):
+    the 19 N populations as doubles each way and the four moment fields,
+    against the step time, in FP16 and FP64 storage; with a download after
+    every step the rate is the PCIe-inclusive one."""
+    rows = {}
+    for prec in ("fp16", "fp64"):
+        with lbm.Lattice(**WORKLOAD, precision=prec) as L:
+            L.run(20, MASS_EVERY, lbm.STEP_ALL)
+            f = L.populations()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                f = L.populations()
+            td = (time.perf_counter() - t0) / reps
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                L.upload_populations(f)
+            tu = (time.perf_counter() - t0) / reps
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                L.fields(("rho", "ux", "uy", "uz"))
+            tf = (time.perf_counter() - t0) / reps
+            L.run(2, MASS_EVERY, lbm.STEP_ALL)
+            L.synchronize()
+            t0 = time.perf_counter()
+            L.run(200, MASS_EVERY)
+            L.synchronize()
+            ts = (time.perf_counter() - t0) / 200
+            nb = 19 * L.N * 8
+            rows[prec] = dict(download_s=round(td, 4), download_GBps=round(nb / td / 1e9, 2), upload_s=round(tu, 4),
+                              upload_GBps=round(nb / tu / 1e9, 2), fields_s=round(tf, 4), step_ms=round(ts * 1e3, 4),
+                              mlups_with_download_every_step=round(L.N / (ts + td) / 1e6, 1))
+            print(f"[transfer] {prec}: {json.dumps(rows[prec])}", file=sys.stderr, flush=True)
+    return {"workload": "host transfers, 512x256x128 channel", "reps": reps, "rows": rows}
+
+
 def measure_copy(lbm, reps: int = 10) -> dict:
     """Streaming-copy bandwidth (lbm_hip_copy_bandwidth_parts) against the
     footprint: the bytes one collide-stream launch moves for the 512x256x128
@@ -316,13 +359,14 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--precisions", default="fp16,fp32,fp64")
     ap.add_argument("--workload", default="channel",
-                    choices=["channel", "ensemble", "surrogate", "trajectory", "mlp", "copy"],
+                    choices=["channel", "ensemble", "surrogate", "trajectory", "mlp", "copy", "transfer"],
                     help="channel: the 512x256x128 slit channel (headline); ensemble: 64 slit-channel runs batched; "
                          "surrogate: the learned velocity surrogate with LBM fallback, 1000 trajectories; "
                          "trajectory: the 512x256x128 pusher in FP32 / FP16 / BF16 storage against FP64 "
                          "(--steps steps); mlp: training of the surrogate's 4-net ensemble alone on 4096 "
                          "synthetic samples, 20 epochs of 256-sample minibatches; copy: the streaming-copy "
-                         "probe over footprints from 1.27 GB to 15.2 GB, in one allocation pair and in 64")
+                         "probe over footprints from 1.27 GB to 15.2 GB, in one allocation pair and in 64; "
+                         "transfer: host download / upload rates on the 512x256x128 channel")
     ap.add_argument("--tiling", default=None,
                     help="WIDTH,XCD: use this collide-stream tile shape instead of autotuning (e.g. 256,0; "
                          "width 0: row runs); channel and ensemble workloads.  'sweep' (ensemble): time "
@@ -388,6 +432,9 @@ def main():
         return
     if args.workload == "copy":
         print(json.dumps(measure_copy(lbm)))
+        return
+    if args.workload == "transfer":
+        print(json.dumps(measure_transfer(lbm)))
         return
     if args.workload == "surrogate":
         from lbm_solver_3d_amd import surrogate as S
