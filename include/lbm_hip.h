@@ -126,12 +126,18 @@ void lbm_hip_destroy(lbm_hip_ctx* ctx);
  * forcing, halfway bounce-back on WALL links and moving-boundary bounce-back
  * on PARTICLE links, and the momentum-exchange force and torque on the
  * squirmer.  `flags` (LBM_STEP_*) selects which by-products are written:
- * the reference writes all of them every step (LBM_STEP_ALL reproduces it). */
+ * the reference writes all of them every step (LBM_STEP_ALL reproduces it).
+ * With a squirmer, the PARTICLE-link pass (bounce-back correction, force,
+ * torque) is left pending and runs in the launch of the next
+ * lbm_hip_advance_squirmer, or first in any call that reads its results
+ * (lbm_hip_step, correct_mass, get/set_squirmer, download, upload,
+ * synchronize, particle_links, autotune): the results are the same. */
 int lbm_hip_step(lbm_hip_ctx* ctx, int flags);
 
 /* LBM::advanceSquirmer (lbm.cpp:257-263): wall potential, Newton-Euler
  * update, hard gap and the sequential remap of the sphere, on the device in
- * FP64.  No-op without a squirmer. */
+ * FP64, in one launch with the pending link pass of the last step.  No-op
+ * without a squirmer. */
 int lbm_hip_advance_squirmer(lbm_hip_ctx* ctx);
 
 /* LBM::correctMass (lbm.cpp:324-339): restore the mean fluid density to 1.
@@ -177,6 +183,15 @@ int lbm_hip_upload_populations(lbm_hip_ctx* ctx, const double* f);
 int lbm_hip_synchronize(lbm_hip_ctx* ctx);
 
 int lbm_hip_get_info(lbm_hip_ctx* ctx, lbm_hip_info* out);
+
+/* The PARTICLE links of the current flags, as the device's link list holds
+ * them: (node n, direction i) with n FLUID and n + c_i PARTICLE -- the links
+ * LBM::step visits in its sweep (lbm.cpp:179-199), which the reference finds
+ * from the flags every step.  Writes at most `cap` of them, in list order,
+ * into node[] / dir[] (either may be NULL when cap is 0) and their number
+ * into *count.  No counterpart in the reference: it exposes the link list
+ * for inspection and tests. */
+int lbm_hip_particle_links(lbm_hip_ctx* ctx, uint32_t* node, uint8_t* dir, int64_t cap, int64_t* count);
 
 /* Bench support: time `steps` driver-loop steps (as lbm_hip_run, graph
  * replayed) with HIP events on the context's stream -> *total_ms; then run up

@@ -160,6 +160,8 @@ def load_library(path: Path | None = None):
     lib.lbm_hip_upload_populations.argtypes = [ctx, dp]
     lib.lbm_hip_synchronize.argtypes = [ctx]
     lib.lbm_hip_get_info.argtypes = [ctx, C.POINTER(Info)]
+    lib.lbm_hip_particle_links.argtypes = [ctx, C.POINTER(C.c_uint32), C.POINTER(C.c_uint8), C.c_int64,
+                                           C.POINTER(C.c_int64)]
     lib.lbm_hip_time_run.argtypes = [ctx, C.c_int, C.c_int, dp, dp]
     lib.lbm_hip_autotune.argtypes = [ctx, C.c_int, C.c_int, C.POINTER(Tuning)]
     lib.lbm_hip_set_tiling.argtypes = [ctx, C.c_int, C.c_int]
@@ -340,6 +342,18 @@ class Lattice:
         _check(self.lib, self.lib.lbm_hip_download(self.h, None, None, None, None, None,
                                                    flag.ctypes.data_as(C.POINTER(C.c_uint8))))
         return flag
+
+    def particle_links(self) -> tuple[np.ndarray, np.ndarray]:
+        """The moving-boundary links (node, direction) of the current flags, in
+        the order of the device's link list."""
+        cnt = C.c_int64(0)
+        _check(self.lib, self.lib.lbm_hip_particle_links(self.h, None, None, 0, C.byref(cnt)))
+        node = np.empty(cnt.value, np.uint32)
+        d = np.empty(cnt.value, np.uint8)
+        _check(self.lib, self.lib.lbm_hip_particle_links(self.h, node.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                                         d.ctypes.data_as(C.POINTER(C.c_uint8)), cnt.value,
+                                                         C.byref(cnt)))
+        return node, d
 
     def upload_populations(self, f: np.ndarray):
         f = np.ascontiguousarray(f, dtype=np.float64).reshape(-1)

@@ -22,7 +22,7 @@ struct BatchMember {
     BoundaryArgs B;
     LinkArgs<S> L;
     RemapArgs<S> R;     // layout parity and shift slot set per launch
-    int lblocks;        // blocks of this member's link kernel
+    int lblocks;        // blocks of this member's link pass (LINK_THREADS threads each)
     int has_squirmer;
 };
 
@@ -63,25 +63,29 @@ __global__ __launch_bounds__(BLOCK) void batch_boundary_kernel(const BatchMember
     boundary_block<S, MODE>(A, K, B, (int)(e & BATCH_LOCAL_MASK));
 }
 
+// The link pass of every member with a squirmer (block map: member << 24 |
+// link block), one partial per block into the member's own array.
 template <class S>
-__global__ __launch_bounds__(LINK_BLOCK) void batch_links_kernel(const BatchMember<S>* __restrict__ M,
-                                                                 const uint32_t* __restrict__ block_map) {
+__global__ __launch_bounds__(LINK_THREADS) void batch_links_kernel(const BatchMember<S>* __restrict__ M,
+                                                                   const uint32_t* __restrict__ block_map) {
     const uint32_t e = as_constant(block_map)[blockIdx.x];
     const BatchMember<S>* mb = M + (e >> 24);
-    links_block<S>(from_constant(&mb->L), (int)(e & BATCH_LOCAL_MASK), from_constant(&mb->lblocks));
+    link_pass_block<S>(from_constant(&mb->L), (int)(e & BATCH_LOCAL_MASK), from_constant(&mb->lblocks));
 }
 
-// One block per member; `t_after` is the number of steps done.
+// One block per member: the link sums, rigid-body update, remap and
+// link-list upkeep.  `t_after` is the number of steps done.
 template <class S>
-__global__ __launch_bounds__(SQ_THREADS) void batch_squirmer_kernel(const BatchMember<S>* __restrict__ M,
-                                                                    long long t_after) {
+__global__ __launch_bounds__(CPL_THREADS) void batch_couple_kernel(const BatchMember<S>* __restrict__ M,
+                                                                   long long t_after) {
     const BatchMember<S>* mb = M + blockIdx.x;
     if (!from_constant(&mb->has_squirmer)) return;
     RemapArgs<S> R = from_constant(&mb->R);
     R.shifted = (t_after > 0 && ((t_after - 1) & 1) == 0) ? 1 : 0;
     R.shift_slot = (int)(t_after & 1);
     R.accumulate = 0;
-    squirmer_block<S>(R);
+    R.maintain = 1;
+    couple_block<S>(from_constant(&mb->L), R, from_constant(&mb->lblocks), 1, 1);
 }
 
 }  // namespace lbmhip

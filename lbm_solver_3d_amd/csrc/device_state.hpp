@@ -64,8 +64,12 @@ struct DevState {
     // (lbm.cpp:333, 337: f_i += w_i d).
     double shift[2];
     double link_dmass;     // mass change of the moving-boundary corrections of the last step
-    unsigned link_ticket;  // last-block counters of the two link kernels
+    unsigned link_ticket;  // last-block counters of the coupling and wall-force kernels
     unsigned wall_ticket;
+    // PARTICLE link list (links_kernel.hpp): entries in use (live and dead)
+    // and the live ones the last link pass found
+    unsigned nlinks;
+    unsigned nvalid;
     // Mass change made by the remaps since the last step, as excess over one
     // unit per fluid node (cover removes sum_i f_i - 1, uncover adds
     // sum_i feq_i - 1).  The first remap after a step sets it, later ones
@@ -87,6 +91,7 @@ struct DevState {
 };
 
 constexpr int ERR_REMAP_OVERFLOW = 1;
+constexpr int ERR_LINK_OVERFLOW = 2;  // the link list outgrew its capacity
 
 // Arguments of the fused collide-stream kernel.
 template <class S>

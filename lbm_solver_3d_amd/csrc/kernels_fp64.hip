@@ -10,14 +10,25 @@
 namespace lbmhip {
 
 template <class S>
-hipError_t launch_squirmer(const RemapArgs<S>& a, hipStream_t s) {
-    hipLaunchKernelGGL((squirmer_kernel<S>), dim3(1), dim3(SQ_THREADS), 0, s, a);
+hipError_t launch_couple(const LinkArgs<S>& l, const RemapArgs<S>& r, int link_blocks, int links, int remap,
+                         hipStream_t s) {
+    if (links) {
+        hipLaunchKernelGGL((particle_links_kernel<S>), dim3(link_blocks), dim3(LINK_THREADS), 0, s, l);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL((couple_kernel<S>), dim3(1), dim3(CPL_THREADS), 0, s, l, r, link_blocks, links, remap);
     return hipGetLastError();
 }
 
-template <class S>
-hipError_t launch_links(const LinkArgs<S>& a, int blocks, hipStream_t s) {
-    hipLaunchKernelGGL((particle_links_kernel<S>), dim3(blocks), dim3(LINK_BLOCK), 0, s, a);
+__global__ __launch_bounds__(CPL_THREADS) void link_build_kernel(const uint8_t* flag, DevState* st, uint2* list,
+                                                                 uint32_t cap, int NX, int NY, int NZ) {
+    link_build_block(flag, st, list, cap, NX, NY, NZ);
+}
+
+hipError_t launch_link_build(const uint8_t* flag, DevState* st, uint2* list, uint32_t cap, int NX, int NY, int NZ,
+                             hipStream_t s) {
+    hipLaunchKernelGGL(link_build_kernel, dim3(1), dim3(CPL_THREADS), 0, s, flag, st, list, cap, NX, NY, NZ);
     return hipGetLastError();
 }
 
@@ -77,24 +88,22 @@ hipError_t launch_fill_populations(S* f, uint32_t SS, hipStream_t s) {
 }
 
 template <class S>
-hipError_t launch_batch_links(const BatchMember<S>* m, const uint32_t* map, int blocks, hipStream_t s) {
-    if (blocks <= 0) return hipSuccess;
-    hipLaunchKernelGGL((batch_links_kernel<S>), dim3(blocks), dim3(LINK_BLOCK), 0, s, m, map);
-    return hipGetLastError();
-}
-
-template <class S>
-hipError_t launch_batch_squirmer(const BatchMember<S>* m, int count, long long t_after, hipStream_t s) {
+hipError_t launch_batch_couple(const BatchMember<S>* m, const uint32_t* map, int link_blocks, int count,
+                               long long t_after, hipStream_t s) {
+    if (link_blocks > 0) {
+        hipLaunchKernelGGL((batch_links_kernel<S>), dim3(link_blocks), dim3(LINK_THREADS), 0, s, m, map);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
     if (count <= 0) return hipSuccess;
-    hipLaunchKernelGGL((batch_squirmer_kernel<S>), dim3(count), dim3(SQ_THREADS), 0, s, m, t_after);
+    hipLaunchKernelGGL((batch_couple_kernel<S>), dim3(count), dim3(CPL_THREADS), 0, s, m, t_after);
     return hipGetLastError();
 }
 
 #define LBMHIP_INSTANTIATE(S)                                                                                     \
-    template hipError_t launch_batch_links<S>(const BatchMember<S>*, const uint32_t*, int, hipStream_t);         \
-    template hipError_t launch_batch_squirmer<S>(const BatchMember<S>*, int, long long, hipStream_t);            \
-    template hipError_t launch_squirmer<S>(const RemapArgs<S>&, hipStream_t);                                    \
-    template hipError_t launch_links<S>(const LinkArgs<S>&, int, hipStream_t);                                   \
+    template hipError_t launch_batch_couple<S>(const BatchMember<S>*, const uint32_t*, int, int, long long,       \
+                                               hipStream_t);                                                      \
+    template hipError_t launch_couple<S>(const LinkArgs<S>&, const RemapArgs<S>&, int, int, int, hipStream_t);   \
     template hipError_t launch_wall_force<S>(const S*, const uint8_t*, const WallForceArgs&, double*, DevState*, \
                                              int, int, int, uint32_t, uint32_t, hipStream_t);                             \
     template hipError_t launch_gather<S>(const S*, const uint8_t*, double*, int, int, int, uint32_t, uint32_t, uint32_t, \

@@ -23,11 +23,16 @@ template <class S>
 hipError_t launch_boundary(const StepArgs<S>& a, const FastCoef& k, const BoundaryArgs& b, int blocks, int mode,
                            hipStream_t s);
 
+// The coupling of a step: the link pass over the link list (`links`;
+// `link_blocks` blocks of LINK_THREADS threads), then one block that adds
+// its partial sums and runs the rigid-body update and remap (`remap`).
 template <class S>
-hipError_t launch_squirmer(const RemapArgs<S>& a, hipStream_t s);
+hipError_t launch_couple(const LinkArgs<S>& l, const RemapArgs<S>& r, int link_blocks, int links, int remap,
+                         hipStream_t s);
 
-template <class S>
-hipError_t launch_links(const LinkArgs<S>& a, int blocks, hipStream_t s);
+// Link list from the flags (one block).
+hipError_t launch_link_build(const uint8_t* flag, DevState* st, uint2* list, uint32_t cap, int NX, int NY, int NZ,
+                             hipStream_t s);
 
 template <class S>
 hipError_t launch_wall_force(const S* f, const uint8_t* flag, const WallForceArgs& w, double* part, DevState* st,
@@ -59,9 +64,8 @@ template <class S>
 hipError_t launch_batch_boundary(const BatchMember<S>* m, const uint32_t* block_map, int blocks, long long t,
                                  int mode, hipStream_t s);
 template <class S>
-hipError_t launch_batch_links(const BatchMember<S>* m, const uint32_t* block_map, int blocks, hipStream_t s);
-template <class S>
-hipError_t launch_batch_squirmer(const BatchMember<S>* m, int count, long long t_after, hipStream_t s);
+hipError_t launch_batch_couple(const BatchMember<S>* m, const uint32_t* block_map, int link_blocks, int count,
+                               long long t_after, hipStream_t s);
 hipError_t launch_batch_finalize(const BatchFinalize* fa, int count, long long t_after, hipStream_t s);
 
 // Storage-independent kernels (kernels_fp64.hip).
@@ -87,8 +91,8 @@ hipError_t launch_copy_parts(const void* const* a, void* const* b, int parts, si
     extern template hipError_t launch_step<S>(const StepArgs<S>&, const FastCoef&, bool, int, hipStream_t);     \
     extern template hipError_t launch_boundary<S>(const StepArgs<S>&, const FastCoef&, const BoundaryArgs&, int, \
                                                   int, hipStream_t);                                             \
-    extern template hipError_t launch_squirmer<S>(const RemapArgs<S>&, hipStream_t);                            \
-    extern template hipError_t launch_links<S>(const LinkArgs<S>&, int, hipStream_t);                           \
+    extern template hipError_t launch_couple<S>(const LinkArgs<S>&, const RemapArgs<S>&, int, int, int,         \
+                                                hipStream_t);                                                   \
     extern template hipError_t launch_wall_force<S>(const S*, const uint8_t*, const WallForceArgs&, double*,      \
                                                     DevState*, int, int, int, uint32_t, uint32_t, hipStream_t);           \
     extern template hipError_t launch_gather<S>(const S*, const uint8_t*, double*, int, int, int, uint32_t,      \
@@ -102,8 +106,8 @@ hipError_t launch_copy_parts(const void* const* a, void* const* b, int parts, si
                                                     bool, int, hipStream_t);                                    \
     extern template hipError_t launch_batch_boundary<S>(const BatchMember<S>*, const uint32_t*, int, long long,  \
                                                         int, hipStream_t);                                      \
-    extern template hipError_t launch_batch_links<S>(const BatchMember<S>*, const uint32_t*, int, hipStream_t);  \
-    extern template hipError_t launch_batch_squirmer<S>(const BatchMember<S>*, int, long long, hipStream_t);
+    extern template hipError_t launch_batch_couple<S>(const BatchMember<S>*, const uint32_t*, int, int, long long, \
+                                                      hipStream_t);
 
 LBMHIP_DECLARE(double)
 LBMHIP_DECLARE(float)

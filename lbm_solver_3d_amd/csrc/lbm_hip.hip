@@ -69,7 +69,17 @@ struct lbm_hip_ctx {
     double* diag_part = nullptr;  // 5 * ceil(N/256) doubles
     double* diag_out = nullptr;   // 5 doubles
     double* red_part = nullptr;   // 8 * max(LINK_MAX_BLOCKS, ceil(N/256)) doubles (link / wall kernels)
-    int link_blocks = 0;
+    int cpl_blocks = 0;           // blocks of the link pass (LINK_THREADS threads each)
+    // PARTICLE link list (links_kernel.hpp) and what the host knows of it:
+    // LIST_CHECKED after a link pass or a build, LIST_UNCHECKED after a remap
+    // that followed one (the next link pass removes the links it undid),
+    // LIST_DIRTY when it must be rebuilt from the flags before the next pass
+    uint2* links = nullptr;
+    uint32_t link_cap = 0;
+    int list_state = 0;
+    // lbm_hip_step leaves the link pass to the coupling launch of
+    // lbm_hip_advance_squirmer; anything that reads its results first runs it
+    bool links_pending = false;
     uint32_t* wall_rows = nullptr;  // rows next to a wall (even-step boundary chunks)
     BoundaryArgs bargs{};
     int bblocks = 0;                // blocks of aa_boundary_kernel
@@ -263,6 +273,8 @@ RemapArgs<S> remap_args(const lbm_hip_ctx* c) {
     a.shifted = (c->t > 0 && ((c->t - 1) & 1) == 0) ? 1 : 0;
     a.shift_slot = (int)(c->t & 1);
     a.accumulate = c->remaps_since_step > 0 ? 1 : 0;
+    a.list = c->links; a.cap = c->link_cap;
+    a.maintain = 0;
     return a;
 }
 
@@ -270,17 +282,33 @@ template <class S>
 LinkArgs<S> link_args(const lbm_hip_ctx* c) {
     LinkArgs<S> a{};
     a.f = (S*)c->f; a.flag = c->flag; a.rho = c->rho; a.st = c->st; a.part = c->red_part;
+    a.list = c->links; a.cap = c->link_cap;
     a.NX = c->NX; a.NY = c->NY; a.NZ = c->NZ; a.SS = c->SS; a.PX = c->PX;
     a.walls_y = c->p.walls_y; a.walls_z = c->p.walls_z;
     return a;
 }
 
-// Blocks of particle_links_kernel: its box (around the covering sphere, of
-// radius < R + 1) is at most ceil(2R) + 7 wide per axis.
-int link_blocks_for(double R, int NX, int NY, int NZ) {
-    const int w = (int)std::ceil(2.0 * R) + 7;
-    const long long M = (long long)std::min(w, NX) * std::min(w, NY) * std::min(w, NZ);
-    return (int)((M + LINK_BLOCK - 1) / LINK_BLOCK);
+enum { LIST_DIRTY = 0, LIST_CHECKED = 1, LIST_UNCHECKED = 2 };
+
+// Blocks of the link pass.  A sphere of radius R has about 72 R^2 PARTICLE
+// links (D3Q19); 100 R^2 list entries leave room for the dead ones, at one
+// entry per thread (a longer list is strided over).
+int cpl_blocks_for(double R) {
+    const double entries = std::max(1.0, 100.0 * R * R);
+    return std::min(LINK_MAX_BLOCKS, std::max(1, (int)std::ceil(entries / LINK_THREADS)));
+}
+
+// Capacity of the link list for spheres up to radius R.  Every link-owning
+// FLUID node lies in the shell rf <= |r| < rf + sqrt(2) around the centre
+// the flags were classified at, and owns at most 18 links; the unit cube
+// around a lattice point of the shell lies in the shell widened by sqrt(3)/2
+// on both sides, which bounds the points.  Twice that bound holds the dead
+// entries the compaction tolerates, plus the appends of one full remap.
+uint32_t link_capacity(double R) {
+    const double a = std::max(0.0, R - 0.8660254037844386), b = R + 1.4142135623730951 + 0.8660254037844386;
+    const double shell = 4.0 / 3.0 * M_PI * (b * b * b - a * a * a) + 64.0;
+    const double cap = 2.0 * (Q - 1) * shell + 2.0 * (Q - 1) * REMAP_CAP + 4096.0;
+    return (uint32_t)std::min(cap, 4.0e9);
 }
 
 // Population row pitch.  A wavefront covers 64 consecutive x of one row;
@@ -361,7 +389,7 @@ void free_ctx(lbm_hip_ctx* c) {
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
     for (void* q : {c->f, (void*)c->flag, c->rho, c->ux, c->uy, c->uz, (void*)c->st, (void*)c->mass_part,
-                    (void*)c->diag_part, (void*)c->diag_out, (void*)c->red_part, (void*)c->wall_rows})
+                    (void*)c->diag_part, (void*)c->diag_out, (void*)c->red_part, (void*)c->wall_rows, (void*)c->links})
         if (q) hipFree(q);
     for (hipEvent_t e : c->events) hipEventDestroy(e);
     drop_graphs(c);
@@ -376,7 +404,45 @@ int check_device_error(lbm_hip_ctx* c) {
     if (err & ERR_REMAP_OVERFLOW)
         return fail(LBM_HIP_EOVERFLOW, "remap changed more than " + std::to_string(REMAP_CAP) +
                                            " nodes in one step; the lattice state is no longer valid");
+    if (err & ERR_LINK_OVERFLOW)
+        return fail(LBM_HIP_EOVERFLOW, "the particle link list outgrew its capacity of " +
+                                           std::to_string(c->link_cap) + " entries");
     return LBM_HIP_OK;
+}
+
+// Rebuild the link list from the flags if the host invalidated it.
+int ensure_link_list(lbm_hip_ctx* c) {
+    if (!c->has_squirmer || c->list_state != LIST_DIRTY) return LBM_HIP_OK;
+    const hipError_t e = launch_link_build(c->flag, c->st, c->links, c->link_cap, c->NX, c->NY, c->NZ, c->stream);
+    if (e != hipSuccess) return fail(LBM_HIP_EDEVICE, std::string("link list build: ") + hipGetErrorString(e));
+    c->list_state = LIST_CHECKED;
+    return LBM_HIP_OK;
+}
+
+// The coupling launch: the link pass left pending by lbm_hip_step (`links`)
+// and / or the squirmer update with its remap (`remap`).
+int launch_coupling(lbm_hip_ctx* c, bool links, bool remap) {
+    int rc;
+    if (links && (rc = ensure_link_list(c))) return rc;
+    const bool maintain = links || c->list_state == LIST_CHECKED;
+    hipError_t e = hipSuccess;
+    dispatch(c->prec, [&](auto* tag) -> int {
+        using S = std::remove_pointer_t<decltype(tag)>;
+        RemapArgs<S> r = remap_args<S>(c);
+        r.maintain = maintain ? 1 : 0;
+        e = launch_couple<S>(link_args<S>(c), r, c->cpl_blocks, links ? 1 : 0, remap ? 1 : 0, c->stream);
+        return 0;
+    });
+    if (e != hipSuccess) return fail(LBM_HIP_EDEVICE, std::string("coupling launch: ") + hipGetErrorString(e));
+    if (links) c->links_pending = false;
+    if (remap) c->list_state = maintain ? LIST_UNCHECKED : LIST_DIRTY;
+    else c->list_state = LIST_CHECKED;
+    return LBM_HIP_OK;
+}
+
+// Run the link pass of the last step if it is still pending.
+int flush_links(lbm_hip_ctx* c) {
+    return c->links_pending ? launch_coupling(c, true, false) : LBM_HIP_OK;
 }
 
 }  // namespace
@@ -517,8 +583,10 @@ int lbm_hip_create(const lbm_hip_params* pin, lbm_hip_ctx** out) {
         for (int d = 0; d < 3; ++d) hs.xf[d] = s.X[d];
         hs.rf = R;
         band_of(hs.sq, c->NX, c->NY, c->NZ, hs.band);
-        c->link_blocks = link_blocks_for(hs.sq.R, c->NX, c->NY, c->NZ);
-        if (c->link_blocks > LINK_MAX_BLOCKS) return cleanup(fail(LBM_HIP_EINVAL, "squirmer radius too large"));
+        c->cpl_blocks = cpl_blocks_for(hs.sq.R);
+        c->link_cap = link_capacity(hs.sq.R);
+        if (!dmalloc((void**)&c->links, sizeof(uint2) * c->link_cap))
+            return cleanup(fail(LBM_HIP_ENOMEM, "device allocation failed (link list)"));
     } else {
         hs.sq.R = 4.0; hs.sq.B1 = 0.015; hs.sq.planar = 1; hs.sq.e[0] = 1.0;
         hs.sq.eps = 0.2; hs.sq.hc = 2.0; hs.sq.hmin = 0.5;
@@ -550,8 +618,11 @@ int lbm_hip_create(const lbm_hip_params* pin, lbm_hip_ctx** out) {
             using S = std::remove_pointer_t<decltype(tag)>;
             return launch_fill_populations<S>((S*)c->f, c->SS, c->stream) == hipSuccess ? 0 : -1;
         }) != 0 ||
-        hipMemsetAsync(c->mass_part, 0, sizeof(MassPart) * nparts, c->stream) != hipSuccess ||
-        hipStreamSynchronize(c->stream) != hipSuccess)
+        hipMemsetAsync(c->mass_part, 0, sizeof(MassPart) * nparts, c->stream) != hipSuccess)
+        return cleanup(fail(LBM_HIP_EDEVICE, "context initialisation failed"));
+    c->list_state = LIST_DIRTY;
+    if ((rc = ensure_link_list(c))) return cleanup(rc);
+    if (hipStreamSynchronize(c->stream) != hipSuccess)
         return cleanup(fail(LBM_HIP_EDEVICE, "context initialisation failed"));
     *out = c;
     return LBM_HIP_OK;
@@ -568,6 +639,7 @@ int lbm_hip_step(lbm_hip_ctx* c, int flags) {
     if (flags & LBM_STEP_MACRO) mode |= STEP_MACRO;
     if (flags & LBM_STEP_MASS) mode |= STEP_MASS;
     const bool odd = (c->t & 1) != 0;
+    if (int rc = flush_links(c)) return rc;
     hipError_t e = hipSuccess;
     dispatch(c->prec, [&](auto* tag) -> int {
         using S = std::remove_pointer_t<decltype(tag)>;
@@ -576,14 +648,16 @@ int lbm_hip_step(lbm_hip_ctx* c, int flags) {
         if (e == hipSuccess) e = launch_step<S>(sa, c->K, odd, mode, c->stream);
         if (e == hipSuccess && !odd) e = launch_boundary<S>(sa, c->K, c->bargs, c->bblocks, mode, c->stream);
         if (e == hipSuccess && c->kev[1]) e = hipEventRecord(c->kev[1], c->stream);
-        // moving-boundary bounce-back and force/torque on the squirmer
-        if (e == hipSuccess && c->has_squirmer) e = launch_links<S>(link_args<S>(c), c->link_blocks, c->stream);
+        // the moving-boundary bounce-back and force/torque on the squirmer
+        // run in the coupling launch (lbm_hip_advance_squirmer, or whatever
+        // reads them first)
         if (e == hipSuccess && (flags & LBM_STEP_WALL_FORCE))
             e = launch_wall_force<S>((const S*)c->f, c->flag, wall_force_args(c), c->red_part, c->st, c->NX, c->NY,
                                      c->NZ, c->SS, c->PX, c->stream);
         return 0;
     });
     if (e != hipSuccess) return fail(LBM_HIP_EDEVICE, std::string("step launch: ") + hipGetErrorString(e));
+    c->links_pending = c->has_squirmer;
     c->last_bparts = odd ? 0 : c->bblocks;
     ++c->t;
     c->remaps_since_step = 0;
@@ -596,19 +670,14 @@ int lbm_hip_step(lbm_hip_ctx* c, int flags) {
 int lbm_hip_advance_squirmer(lbm_hip_ctx* c) {
     if (!c) return fail(LBM_HIP_EINVAL, "null context");
     if (!c->has_squirmer) return LBM_HIP_OK;
-    hipError_t e = hipSuccess;
-    dispatch(c->prec, [&](auto* tag) -> int {
-        using S = std::remove_pointer_t<decltype(tag)>;
-        e = launch_squirmer<S>(remap_args<S>(c), c->stream);
-        return 0;
-    });
-    if (e != hipSuccess) return fail(LBM_HIP_EDEVICE, std::string("squirmer launch: ") + hipGetErrorString(e));
+    if (int rc = launch_coupling(c, c->links_pending, true)) return rc;
     ++c->remaps_since_step;
     return LBM_HIP_OK;
 }
 
 int lbm_hip_correct_mass(lbm_hip_ctx* c) {
     if (!c) return fail(LBM_HIP_EINVAL, "null context");
+    if (int rc = flush_links(c)) return rc;
     hipError_t e;
     const int shift_slot = (int)(c->t & 1);
     if (c->mass_valid) {
@@ -648,8 +717,11 @@ static int loop_step(lbm_hip_ctx* c, int mass_every, int extra_flags) {
 // parity and the mass cadence, so a capture starting at any multiple of the
 // cycle length replays the same work.
 static int capture_cycles(lbm_hip_ctx* c, int mass_every, int cycles, hipGraphExec_t* out) {
+    int rc0;
+    if ((rc0 = flush_links(c)) || (rc0 = ensure_link_list(c))) return rc0;
     const long long t_save = c->t;
     const bool mv = c->macro_valid, wv = c->wallf_valid, ms = c->mass_valid;
+    const int ls = c->list_state, rs = c->remaps_since_step;
     c->t = 0;  // capture with a cycle-aligned step index
     HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     int rc = LBM_HIP_OK;
@@ -657,6 +729,7 @@ static int capture_cycles(lbm_hip_ctx* c, int mass_every, int cycles, hipGraphEx
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(c->stream, &g);
     c->t = t_save; c->macro_valid = mv; c->wallf_valid = wv; c->mass_valid = ms;
+    c->list_state = ls; c->remaps_since_step = rs; c->links_pending = false;
     if (rc) { if (g) hipGraphDestroy(g); return rc; }
     if (e != hipSuccess) return fail(LBM_HIP_EDEVICE, std::string("graph capture: ") + hipGetErrorString(e));
     e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
@@ -693,6 +766,8 @@ static int run_loop(lbm_hip_ctx* c, int steps, int mass_every, int last_flags, b
             if ((rc = loop_step(c, mass_every, 0))) return rc;
             --left;
         }
+        // the graphs start with a step: no link pass pending, the list valid
+        if ((rc = flush_links(c)) || (rc = ensure_link_list(c))) return rc;
         int done = 0;
         if (c->graph_long) {
             const int CL = C * c->graph_reps;
@@ -704,6 +779,7 @@ static int run_loop(lbm_hip_ctx* c, int steps, int mass_every, int last_flags, b
             c->macro_valid = c->wallf_valid = false;
             c->mass_valid = false;
             c->remaps_since_step = c->has_squirmer ? 1 : 0;
+            if (c->has_squirmer) c->list_state = LIST_UNCHECKED;
         }
     }
     while (left > 0) {
@@ -720,12 +796,14 @@ int lbm_hip_run(lbm_hip_ctx* c, int steps, int mass_every, int last_flags) {
 
 int lbm_hip_synchronize(lbm_hip_ctx* c) {
     if (!c) return fail(LBM_HIP_EINVAL, "null context");
+    if (int rc = flush_links(c)) return rc;
     HIP_TRY(hipStreamSynchronize(c->stream));
     return check_device_error(c);
 }
 
 int lbm_hip_get_squirmer(lbm_hip_ctx* c, lbm_squirmer_state* o) {
     if (!c || !o) return fail(LBM_HIP_EINVAL, "null argument");
+    if (int rc = flush_links(c)) return rc;
     DevState hs;
     HIP_TRY(hipMemcpyAsync(&hs, c->st, sizeof hs, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
@@ -757,9 +835,9 @@ int lbm_hip_set_squirmer(lbm_hip_ctx* c, const lbm_squirmer_state* in) {
     if (c->has_squirmer) {
         if (!(s.R > 0.0) || s.R > c->radius_max)
             return fail(LBM_HIP_EINVAL, "squirmer radius must be positive and at most the radius the context was "
-                                        "created with (the boundary work list is sized for it)");
-        const int lb = link_blocks_for(s.R, c->NX, c->NY, c->NZ);
-        if (lb > LINK_MAX_BLOCKS) return fail(LBM_HIP_EINVAL, "squirmer radius too large");
+                                        "created with (the boundary work list and the link list are sized for it)");
+        // the pending link pass belongs to the state it was stepped with
+        if (int rc = flush_links(c)) return rc;
         // The PARTICLE flags stay those of the centre they were classified
         // at until the next remap, which only reclassifies around the new
         // centre (as lbm.cpp:271-291 does): the centre may move by less than
@@ -776,7 +854,6 @@ int lbm_hip_set_squirmer(lbm_hip_ctx* c, const lbm_squirmer_state* in) {
                 return fail(LBM_HIP_EINVAL, "the squirmer centre may move by less than one node per axis from "
                                             "where its flags were classified");
         }
-        c->link_blocks = lb;
     }
     HIP_TRY(hipMemcpyAsync(&c->st->sq, &s, sizeof s, hipMemcpyHostToDevice, c->stream));
     int band[6];
@@ -813,6 +890,7 @@ int lbm_hip_diagnostics(lbm_hip_ctx* c, lbm_hip_diag* o) {
 
 int lbm_hip_download(lbm_hip_ctx* c, double* f, double* rho, double* ux, double* uy, double* uz, uint8_t* flag) {
     if (!c) return fail(LBM_HIP_EINVAL, "null context");
+    if (int rc = flush_links(c)) return rc;
     const size_t N = c->N;
     if ((rho || ux || uy || uz) && !c->macro_valid)
         return fail(LBM_HIP_ESTATE, "the last step did not write the macroscopic fields");
@@ -866,6 +944,7 @@ int lbm_hip_download(lbm_hip_ctx* c, double* f, double* rho, double* ux, double*
 
 int lbm_hip_upload_populations(lbm_hip_ctx* c, const double* f) {
     if (!c || !f) return fail(LBM_HIP_EINVAL, "null argument");
+    if (int rc = flush_links(c)) return rc;
     const size_t N = c->N;
     std::lock_guard<std::mutex> lock(g_stage_mu);
     double* stage = nullptr;
@@ -910,6 +989,36 @@ int lbm_hip_get_info(lbm_hip_ctx* c, lbm_hip_info* o) {
     o->tile_width = c->tx_shift ? 1 << c->tx_shift : 0;
     o->xcd_remap = c->xcd_remap;
     return LBM_HIP_OK;
+}
+
+int lbm_hip_particle_links(lbm_hip_ctx* c, uint32_t* node, uint8_t* dir, int64_t cap, int64_t* count) {
+    if (!c || !count || cap < 0 || (cap > 0 && (!node || !dir))) return fail(LBM_HIP_EINVAL, "invalid argument");
+    *count = 0;
+    if (!c->has_squirmer) return LBM_HIP_OK;
+    int rc;
+    if ((rc = flush_links(c)) || (rc = ensure_link_list(c))) return rc;
+    unsigned nl = 0;
+    HIP_TRY(hipMemcpyAsync(&nl, &c->st->nlinks, sizeof nl, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    std::vector<uint2> list(nl);
+    std::vector<uint8_t> fl(c->N);
+    if (nl) HIP_TRY(hipMemcpyAsync(list.data(), c->links, sizeof(uint2) * nl, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(fl.data(), c->flag, c->N, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    // live entries whose link the flags still hold (after a remap the list
+    // also holds the links it removed, until the next link pass marks them)
+    const uint32_t NX = (uint32_t)c->NX, NY = (uint32_t)c->NY, NZ = (uint32_t)c->NZ;
+    int64_t k = 0;
+    for (const uint2& e : list) {
+        if (e.y == LINK_DEAD || e.y >= (uint32_t)Q || e.x >= c->N) continue;
+        const uint32_t x = e.x % NX, y = (e.x / NX) % NY, z = e.x / (NX * NY);
+        const uint32_t xn = (x + NX + CX[e.y]) % NX, yn = (y + NY + CY[e.y]) % NY, zn = (z + NZ + CZ[e.y]) % NZ;
+        if ((fl[e.x] & TYPE_MASK) != FLUID || (fl[xn + NX * (yn + NY * zn)] & TYPE_MASK) != PARTICLE) continue;
+        if (k < cap) { node[k] = e.x; dir[k] = (uint8_t)e.y; }
+        ++k;
+    }
+    *count = k;
+    return check_device_error(c);
 }
 
 int lbm_hip_time_run(lbm_hip_ctx* c, int steps, int mass_every, double* total_ms, double* step_kernel_ms) {
@@ -969,21 +1078,28 @@ int lbm_hip_time_run(lbm_hip_ctx* c, int steps, int mass_every, double* total_ms
 // ---------------------------------------------------------------------------
 namespace {
 
+constexpr int SNAP_PARTS = 8;
+
 struct Snapshot {
-    void* buf[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-    size_t bytes[7] = {0, 0, 0, 0, 0, 0, 0};
+    void* buf[SNAP_PARTS] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    size_t bytes[SNAP_PARTS] = {0, 0, 0, 0, 0, 0, 0, 0};
     long long t = 0;
     bool macro_valid = false, wallf_valid = false, mass_valid = false;
     int last_bparts = 0;
     int remaps_since_step = 0;
+    int list_state = 0;
 };
 
-void snapshot_parts(lbm_hip_ctx* c, void* (&src)[7], size_t (&bytes)[7]) {
+// Device arrays of the state: populations, flags, moments, DevState and the
+// link list (whose order the later link sums follow, so it is copied rather
+// than rebuilt).
+void snapshot_parts(lbm_hip_ctx* c, void* (&src)[SNAP_PARTS], size_t (&bytes)[SNAP_PARTS]) {
     src[0] = c->f; bytes[0] = (size_t)Q * c->SS * c->ssize;
     src[1] = c->flag; bytes[1] = c->N;
     src[2] = c->rho; src[3] = c->ux; src[4] = c->uy; src[5] = c->uz;
     for (int k = 2; k < 6; ++k) bytes[k] = (size_t)c->N * c->msize;
     src[6] = c->st; bytes[6] = sizeof(DevState);
+    src[7] = c->links; bytes[7] = c->links ? sizeof(uint2) * c->link_cap : 0;
 }
 
 void snapshot_free(Snapshot& s) {
@@ -992,9 +1108,11 @@ void snapshot_free(Snapshot& s) {
 }
 
 int snapshot_take(lbm_hip_ctx* c, Snapshot& s, hipStream_t stream) {
-    void* src[7];
+    if (int rc = flush_links(c)) return rc;
+    void* src[SNAP_PARTS];
     snapshot_parts(c, src, s.bytes);
-    for (int k = 0; k < 7; ++k) {
+    for (int k = 0; k < SNAP_PARTS; ++k) {
+        if (s.bytes[k] == 0) continue;
         if (hipMalloc(&s.buf[k], s.bytes[k]) != hipSuccess) {
             s.buf[k] = nullptr;
             snapshot_free(s);
@@ -1006,14 +1124,16 @@ int snapshot_take(lbm_hip_ctx* c, Snapshot& s, hipStream_t stream) {
     s.macro_valid = c->macro_valid; s.wallf_valid = c->wallf_valid; s.mass_valid = c->mass_valid;
     s.last_bparts = c->last_bparts;
     s.remaps_since_step = c->remaps_since_step;
+    s.list_state = c->list_state;
     return LBM_HIP_OK;
 }
 
 int snapshot_restore(lbm_hip_ctx* c, const Snapshot& s, hipStream_t stream) {
-    void* dst[7];
-    size_t bytes[7];
+    void* dst[SNAP_PARTS];
+    size_t bytes[SNAP_PARTS];
     snapshot_parts(c, dst, bytes);
-    for (int k = 0; k < 7; ++k) HIP_TRY(hipMemcpyAsync(dst[k], s.buf[k], bytes[k], hipMemcpyDeviceToDevice, stream));
+    for (int k = 0; k < SNAP_PARTS; ++k)
+        if (bytes[k]) HIP_TRY(hipMemcpyAsync(dst[k], s.buf[k], bytes[k], hipMemcpyDeviceToDevice, stream));
     c->t = s.t;
     c->macro_valid = s.macro_valid; c->wallf_valid = s.wallf_valid;
     // The per-block mass partials of a pending mass step are not part of the
@@ -1022,6 +1142,8 @@ int snapshot_restore(lbm_hip_ctx* c, const Snapshot& s, hipStream_t stream) {
     c->mass_valid = false;
     c->last_bparts = s.last_bparts;
     c->remaps_since_step = s.remaps_since_step;
+    c->list_state = s.list_state;
+    c->links_pending = false;
     return LBM_HIP_OK;
 }
 
@@ -1167,7 +1289,7 @@ int batch_tables(lbm_hip_batch* b) {
         }
         for (int k = 0; k < c->bblocks; ++k) bm.push_back(tag | (uint32_t)k);
         if (c->has_squirmer)
-            for (int k = 0; k < c->link_blocks; ++k) lm.push_back(tag | (uint32_t)k);
+            for (int k = 0; k < c->cpl_blocks; ++k) lm.push_back(tag | (uint32_t)k);
         fin[(size_t)m] = BatchFinalize{c->mass_part, c->st, c->G, c->bblocks, c->has_squirmer ? 1 : 0};
     }
     if (row_order) {
@@ -1197,7 +1319,7 @@ int batch_tables(lbm_hip_batch* b) {
             x.B = c->bargs;
             x.L = link_args<S>(c);
             x.R = remap_args<S>(c);
-            x.lblocks = c->link_blocks;
+            x.lblocks = c->cpl_blocks;
             x.has_squirmer = c->has_squirmer ? 1 : 0;
         }
         if (hipMemcpyAsync(b->desc, d.data(), d.size() * sizeof(BatchMember<S>), hipMemcpyHostToDevice, b->stream) !=
@@ -1227,13 +1349,14 @@ int batch_step(lbm_hip_batch* b, int mass_every, int extra_flags) {
         const auto* d = static_cast<const BatchMember<S>*>(b->desc);
         e = launch_batch_step<S>(d, b->tile_map, b->Gtot, b->t, b->mem[0]->xcd_remap == 1, odd, mode, b->stream);
         if (e == hipSuccess && !odd) e = launch_batch_boundary<S>(d, b->bmap, b->Btot, b->t, mode, b->stream);
-        if (e == hipSuccess && b->any_squirmer) e = launch_batch_links<S>(d, b->lmap, b->Ltot, b->stream);
         if (e == hipSuccess && (extra_flags & LBM_STEP_WALL_FORCE))
             for (lbm_hip_ctx* c : b->mem)
                 if (e == hipSuccess)
                     e = launch_wall_force<S>((const S*)c->f, c->flag, wall_force_args(c), c->red_part, c->st, c->NX,
                                              c->NY, c->NZ, c->SS, c->PX, b->stream);
-        if (e == hipSuccess && b->any_squirmer) e = launch_batch_squirmer<S>(d, b->count, tnext, b->stream);
+        // link pass, rigid-body update, remap and link-list upkeep of every member
+        if (e == hipSuccess && b->any_squirmer)
+            e = launch_batch_couple<S>(d, b->lmap, b->Ltot, b->count, tnext, b->stream);
         return 0;
     });
     if (e == hipSuccess && mass) e = launch_batch_finalize(b->fin, b->count, tnext, b->stream);
@@ -1245,6 +1368,7 @@ int batch_step(lbm_hip_batch* b, int mass_every, int extra_flags) {
         c->macro_valid = (extra_flags & LBM_STEP_MACRO) != 0;
         c->wallf_valid = (extra_flags & LBM_STEP_WALL_FORCE) != 0;
         c->mass_valid = false;
+        if (c->has_squirmer) c->list_state = LIST_UNCHECKED;
     }
     return LBM_HIP_OK;
 }
@@ -1260,6 +1384,7 @@ int batch_graph(lbm_hip_batch* b, int mass_every) {
         flags[(size_t)m].t = c->t; flags[(size_t)m].last_bparts = c->last_bparts;
         flags[(size_t)m].macro_valid = c->macro_valid; flags[(size_t)m].wallf_valid = c->wallf_valid;
         flags[(size_t)m].mass_valid = c->mass_valid;
+        flags[(size_t)m].list_state = c->list_state;
     }
     b->t = 0;  // capture with a cycle-aligned step index (the kernels use its parity only)
     HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
@@ -1273,6 +1398,7 @@ int batch_graph(lbm_hip_batch* b, int mass_every) {
         c->t = flags[(size_t)m].t; c->last_bparts = flags[(size_t)m].last_bparts;
         c->macro_valid = flags[(size_t)m].macro_valid; c->wallf_valid = flags[(size_t)m].wallf_valid;
         c->mass_valid = flags[(size_t)m].mass_valid;
+        c->list_state = flags[(size_t)m].list_state;
     }
     if (rc) { if (g) hipGraphDestroy(g); return rc; }
     if (e != hipSuccess) return fail(LBM_HIP_EDEVICE, std::string("batch graph capture: ") + hipGetErrorString(e));
@@ -1292,6 +1418,13 @@ int batch_loop(lbm_hip_batch* b, int steps, int mass_every, int last_flags) {
     for (lbm_hip_ctx* c : b->mem)
         if (c->t != b->t) return fail(LBM_HIP_EINVAL, "a batch member was advanced outside the batch");
     int rc, left = steps;
+    // every member's link list valid before the batch's coupling launches
+    // (a member's own remap call may have invalidated it)
+    for (lbm_hip_ctx* c : b->mem)
+        if (c->links_pending || c->list_state == LIST_DIRTY) {
+            if ((rc = flush_links(c)) || (rc = ensure_link_list(c))) return rc;
+            HIP_TRY(hipStreamSynchronize(c->stream));
+        }
     if (steps > 1) {
         if ((rc = batch_graph(b, mass_every))) return rc;
         const int C = b->graph_cycle;
@@ -1361,7 +1494,7 @@ int lbm_hip_batch_create(const lbm_hip_params* members, int count, lbm_hip_batch
         b->mem.push_back(c);
         b->device_bytes += c->device_bytes;
         b->any_squirmer = b->any_squirmer || c->has_squirmer;
-        if (c->G_max >= (1 << 24) || c->bblocks >= (1 << 24) || c->link_blocks >= (1 << 24))
+        if (c->G_max >= (1 << 24) || c->bblocks >= (1 << 24) || c->cpl_blocks >= (1 << 24))
             return bail(fail(LBM_HIP_EINVAL, "member " + std::to_string(m) + " too large for a batch"));
     }
     if (hipSetDevice(b->device) != hipSuccess) return bail(fail(LBM_HIP_EDEVICE, "hipSetDevice failed"));
@@ -1376,7 +1509,7 @@ int lbm_hip_batch_create(const lbm_hip_params* members, int count, lbm_hip_batch
     for (lbm_hip_ctx* c : b->mem) {
         gmax += (size_t)c->G_max;
         bsum += (size_t)c->bblocks;
-        lsum += c->has_squirmer ? (size_t)c->link_blocks : 0;
+        lsum += c->has_squirmer ? (size_t)c->cpl_blocks : 0;
     }
     const size_t dsz = b->prec == LBM_HIP_FP64 ? sizeof(BatchMember<double>)
                        : b->prec == LBM_HIP_FP32 ? sizeof(BatchMember<float>)

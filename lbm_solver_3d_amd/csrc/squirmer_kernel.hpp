@@ -31,12 +31,13 @@
 // phase D runs without ordering.
 #pragma once
 #include "device_state.hpp"
+#include "links_kernel.hpp"
 #include "step_kernel.hpp"
 #include "storage.hpp"
 
 namespace lbmhip {
 
-constexpr int SQ_THREADS = 1024;
+constexpr int SQ_THREADS = CPL_THREADS;
 constexpr int REMAP_CAP = 512;   // changed nodes per remap held in LDS
 constexpr int GATHER = 128;      // changed nodes per gather pass of phase C
 constexpr int BOX_MAX = 128;     // remap box columns per axis held in LDS
@@ -58,6 +59,18 @@ struct RemapArgs {
     int shifted;     // layout after the last step: 1 after an even step
     int shift_slot;  // mass-shift slot the next step reads
     int accumulate;  // 1: another remap ran since the last step (add to remap_dmass, do not set it)
+    uint2* list;     // PARTICLE link list (links_kernel.hpp)
+    uint32_t cap;    // its capacity in entries
+    int maintain;    // 1: append the links this remap creates (a link pass ran since the last remap)
+};
+
+// The momentum-exchange sums of the link pass that ran just before the
+// remap block (thread 0: Fx Fy Fz Tx Ty Tz, mass change, live list entries)
+// and the list length (every thread).
+struct LinkSums {
+    double v[8];
+    uint32_t nl;
+    int valid;
 };
 
 // squirmer.cpp:73-77
@@ -76,11 +89,6 @@ __device__ inline double equilibrium(int i, double rho, double ux, double uy, do
 }
 
 __device__ inline int wrapi(int a, int n) { return ((a % n) + n) % n; }
-// a + d for d in {-1, 0, 1} on a periodic axis of length n, 0 <= a < n
-__device__ inline int step1(int a, int d, int n) {
-    const int b = a + d;
-    return b < 0 ? n - 1 : (b >= n ? 0 : b);
-}
 
 // Read a device-side macroscopic value as double.
 template <class S>
@@ -100,8 +108,70 @@ __device__ inline void store_macro(void* p, uint32_t n, double v) {
     else static_cast<float*>(p)[n] = (float)v;
 }
 
+// Link-list maintenance after a remap (links_kernel.hpp), by the remap
+// block once the flags are final and FLIPPED still marks the K changed
+// nodes.  Every link a flag change creates involves a changed node n: its
+// own links to PARTICLE neighbours when it was uncovered, or the links of
+// its unchanged FLUID neighbours m = n - c_i into it when it was covered (a
+// link between two changed nodes comes from the uncovered one only, so none
+// is appended twice).  The links a change removes stay in the list until the
+// next link pass marks them dead.  The list is compacted first when dead
+// entries make up more than a fifth of it or the appends could overflow it.
 template <class S>
-__device__ void squirmer_block(const RemapArgs<S>& A) {
+__device__ void append_links(const RemapArgs<S>& A, const uint32_t* changed, const uint8_t* cover, int K,
+                             const LinkSums& hs) {
+    __shared__ int wsum[SQ_THREADS / 64];
+    __shared__ uint32_t s_nv;
+    DevState* st = A.st;
+    // list length and live entries: from the link pass of this launch, or
+    // as the last one left them
+    if (threadIdx.x == 0) s_nv = hs.valid ? (uint32_t)hs.v[7] : st->nvalid;
+    const uint32_t nl0 = hs.valid ? hs.nl : st->nlinks;
+    __syncthreads();
+    const uint32_t nv = s_nv;
+    uint32_t nl = nl0;
+    const bool compact = nl > nv + nv / 4u + 1024u || (size_t)nl + (size_t)(Q - 1) * (size_t)K > (size_t)A.cap;
+    if (!compact && K == 0) return;  // uniform
+    if (compact) nl = link_compact_block(A.list, nl);
+    const int NX = A.NX, NY = A.NY, NZ = A.NZ;
+    const uint32_t uNX = (uint32_t)NX, uNXY = (uint32_t)NX * (uint32_t)NY;
+    for (int w0 = 0; w0 < K * (Q - 1); w0 += SQ_THREADS) {
+        const int w = w0 + (int)threadIdx.x;
+        int emit = 0;
+        uint2 e = make_uint2(0u, LINK_DEAD);
+        if (w < K * (Q - 1)) {
+            const int k = w / (Q - 1), i = 1 + w - k * (Q - 1);
+            const uint32_t n = changed[k];
+            const int x = (int)(n % uNX), y = (int)((n / uNX) % (uint32_t)NY), z = (int)(n / uNXY);
+            if (!cover[k]) {  // uncovered, now FLUID
+                const uint32_t nb = (uint32_t)step1(x, CX[i], NX) + uNX * (uint32_t)step1(y, CY[i], NY) +
+                                    uNXY * (uint32_t)step1(z, CZ[i], NZ);
+                if ((A.flag[nb] & TYPE_MASK) == PARTICLE) { emit = 1; e = make_uint2(n, (uint32_t)i); }
+            } else {  // covered, now PARTICLE
+                const uint32_t m = (uint32_t)step1(x, -CX[i], NX) + uNX * (uint32_t)step1(y, -CY[i], NY) +
+                                   uNXY * (uint32_t)step1(z, -CZ[i], NZ);
+                const uint8_t fm = A.flag[m];
+                if ((fm & TYPE_MASK) == FLUID && !(fm & FLIPPED)) { emit = 1; e = make_uint2(m, (uint32_t)i); }
+            }
+        }
+        int total;
+        const int pos = block_exclusive_scan<SQ_THREADS>(emit, wsum, total);
+        if (emit && nl + (uint32_t)pos < A.cap) A.list[nl + (uint32_t)pos] = e;
+        nl += (uint32_t)total;
+    }
+    if (threadIdx.x == 0) {
+        if (nl > A.cap) {
+            atomicOr(&st->err, ERR_LINK_OVERFLOW);
+            nl = A.cap;
+        }
+        st->nlinks = nl;
+    }
+}
+
+// `part` / `nparts`: block partials of the link pass that ran just before
+// (nullptr: none; Fh, Th are those published earlier).
+template <class S>
+__device__ void squirmer_block(const RemapArgs<S>& A, const double* part, int nparts) {
     __shared__ uint32_t list[REMAP_CAP];  // changed nodes in sweep order
     __shared__ uint8_t lcov[REMAP_CAP];   // 1: the node is covered, 0: uncovered
     __shared__ int lpos[REMAP_CAP];       // box position of each entry (ascending)
@@ -124,16 +194,41 @@ __device__ void squirmer_block(const RemapArgs<S>& A) {
 
     // ---------------- A: rigid body (lbm.cpp:257-262) ----------------
     {
-        // one coalesced copy of the squirmer state into LDS
+        // one coalesced copy of the squirmer state into LDS, with the loads
+        // of the link partials
         const double* src = reinterpret_cast<const double*>(&st->sq);
         double* dst = reinterpret_cast<double*>(&sq);
         constexpr int nd = (int)(sizeof(DevSquirmer) / sizeof(double));
         static_assert(sizeof(DevSquirmer) % sizeof(double) == 0, "DevSquirmer must be a whole number of doubles");
-        for (int k = tid; k < nd; k += SQ_THREADS) dst[k] = src[k];
+        if (tid < nd) dst[tid] = src[tid];
+    }
+    LinkSums hs;
+    hs.valid = part != nullptr;
+    hs.nl = 0;
+    if (part) {
+        // the first wavefront adds them, thread 0 (which uses them) among
+        // its lanes: partial b on lane b mod 64 in order, then a fixed tree
+        hs.nl = st->nlinks;
+        if (tid < 64) {
+            double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            for (int b = tid; b < nparts; b += 64)
+                for (int k = 0; k < 8; ++k) v[k] += part[(size_t)b * 8 + k];
+            for (int k = 0; k < 8; ++k) {
+                double x = v[k];
+                for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+                hs.v[k] = x;
+            }
+        }
     }
     __syncthreads();
     if (tid == 0) {
-        // sq.Fh / sq.Th: published by particle_links_kernel after the step
+        // sq.Fh / sq.Th: the link pass of this launch, or published by an
+        // earlier one (lbm.cpp:220-223)
+        if (hs.valid) {
+            for (int d = 0; d < 3; ++d) { sq.Fh[d] = hs.v[d]; sq.Th[d] = hs.v[3 + d]; }
+            st->link_dmass = hs.v[6];
+            st->nvalid = (unsigned)hs.v[7];
+        }
         // applyWallPotential (lbm.cpp:231-241)
         sq.Fw[0] = sq.Fw[1] = sq.Fw[2] = 0.0;
         if (A.walls_y) {
@@ -509,6 +604,7 @@ __device__ void squirmer_block(const RemapArgs<S>& A) {
         }
     }
     __syncthreads();
+    if (A.maintain) append_links(A, list, lcov, K, hs);
     for (int k = tid; k < K; k += SQ_THREADS) {
         const uint32_t n = list[k];
         A.flag[n] = A.flag[n] & TYPE_MASK;
@@ -553,9 +649,37 @@ __device__ void squirmer_block(const RemapArgs<S>& A) {
     }
 }
 
+// The squirmer block of a step: with `links`, first the momentum-exchange
+// sums of the link pass that ran just before it (`nparts` block partials,
+// added in a fixed order); then, with `remap`, the rigid-body update, remap
+// and link-list upkeep.  Without `remap` it only publishes Fh, Th and the
+// link mass change (the link pass of a step whose remap comes later or not
+// at all).
 template <class S>
-__global__ __launch_bounds__(SQ_THREADS) void squirmer_kernel(const RemapArgs<S> A) {
-    squirmer_block<S>(A);
+__device__ __forceinline__ void couple_block(const LinkArgs<S>& L, const RemapArgs<S>& R, int nparts, int links,
+                                             int remap) {
+    if (remap) {
+        squirmer_block<S>(R, links ? L.part : nullptr, nparts);
+        return;
+    }
+    if (!links) return;
+    __shared__ double lds[(CPL_THREADS / 64) * 8];
+    double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int b = (int)threadIdx.x; b < nparts; b += CPL_THREADS)
+        for (int k = 0; k < 8; ++k) v[k] += L.part[(size_t)b * 8 + k];
+    block_reduce<8, CPL_THREADS>(v, lds);
+    if (threadIdx.x == 0) {
+        DevState* st = L.st;
+        for (int d = 0; d < 3; ++d) { st->sq.Fh[d] = v[d]; st->sq.Th[d] = v[3 + d]; }
+        st->link_dmass = v[6];
+        st->nvalid = (unsigned)v[7];
+    }
+}
+
+template <class S>
+__global__ __launch_bounds__(CPL_THREADS) void couple_kernel(const LinkArgs<S> L, const RemapArgs<S> R, int nparts,
+                                                             int links, int remap) {
+    couple_block<S>(L, R, nparts, links, remap);
 }
 
 // ---------------------------------------------------------------------------

@@ -315,8 +315,9 @@ __device__ __forceinline__ void block_mass_sum(double excess, MassPart* out) {
     }
 }
 
-// Wavefront + block reduction of K doubles; thread 0 of the block gets the sums.
-template <int K>
+// Wavefront + block reduction of K doubles over a block of NT threads; thread
+// 0 of the block gets the sums.  `lds` holds (NT / 64) * K doubles.
+template <int K, int NT = BLOCK>
 __device__ __forceinline__ void block_reduce(double (&v)[K], double* lds) {
     static_for<0, K>([&](auto J) {
         constexpr int j = decltype(J)::value;
@@ -332,7 +333,7 @@ __device__ __forceinline__ void block_reduce(double (&v)[K], double* lds) {
         static_for<0, K>([&](auto J) {
             constexpr int j = decltype(J)::value;
             double s = lds[j];
-            for (int w = 1; w < BLOCK / 64; ++w) s += lds[w * K + j];
+            for (int w = 1; w < NT / 64; ++w) s += lds[w * K + j];
             v[j] = s;
         });
     }
