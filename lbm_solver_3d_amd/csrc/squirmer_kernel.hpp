@@ -117,9 +117,14 @@ __device__ inline void store_macro(void* p, uint32_t n, double v) {
 // is appended twice).  The links a change removes stay in the list until the
 // next link pass marks them dead.  The list is compacted first when dead
 // entries make up more than a fifth of it or the appends could overflow it.
+// `gfl` / `gidx` (when the remap gathered all K nodes in one pass, else
+// nullptr): for changed node k and direction i the type before the remap of
+// the neighbour its gather looked at (n - c_i if covered, n + c_i if
+// uncovered) and that neighbour's index in the list or -1; with them the new
+// links follow from LDS without loading a flag.
 template <class S>
 __device__ void append_links(const RemapArgs<S>& A, const uint32_t* changed, const uint8_t* cover, int K,
-                             const LinkSums& hs) {
+                             const LinkSums& hs, const uint8_t (*gfl)[Q], const int16_t (*gidx)[Q]) {
     __shared__ int wsum[SQ_THREADS / 64];
     __shared__ uint32_t s_nv;
     DevState* st = A.st;
@@ -143,15 +148,28 @@ __device__ void append_links(const RemapArgs<S>& A, const uint32_t* changed, con
             const int k = w / (Q - 1), i = 1 + w - k * (Q - 1);
             const uint32_t n = changed[k];
             const int x = (int)(n % uNX), y = (int)((n / uNX) % (uint32_t)NY), z = (int)(n / uNXY);
-            if (!cover[k]) {  // uncovered, now FLUID
-                const uint32_t nb = (uint32_t)step1(x, CX[i], NX) + uNX * (uint32_t)step1(y, CY[i], NY) +
-                                    uNXY * (uint32_t)step1(z, CZ[i], NZ);
-                if ((A.flag[nb] & TYPE_MASK) == PARTICLE) { emit = 1; e = make_uint2(n, (uint32_t)i); }
-            } else {  // covered, now PARTICLE
+            if (!cover[k]) {  // uncovered, now FLUID: is n + c_i PARTICLE now?
+                bool part;
+                if (gfl) {
+                    const int j = gidx[k][i];
+                    part = j >= 0 ? cover[j] != 0 : gfl[k][i] == PARTICLE;
+                } else {
+                    const uint32_t nb = (uint32_t)step1(x, CX[i], NX) + uNX * (uint32_t)step1(y, CY[i], NY) +
+                                        uNXY * (uint32_t)step1(z, CZ[i], NZ);
+                    part = (A.flag[nb] & TYPE_MASK) == PARTICLE;
+                }
+                if (part) { emit = 1; e = make_uint2(n, (uint32_t)i); }
+            } else {  // covered, now PARTICLE: is n - c_i an unchanged FLUID node?
                 const uint32_t m = (uint32_t)step1(x, -CX[i], NX) + uNX * (uint32_t)step1(y, -CY[i], NY) +
                                    uNXY * (uint32_t)step1(z, -CZ[i], NZ);
-                const uint8_t fm = A.flag[m];
-                if ((fm & TYPE_MASK) == FLUID && !(fm & FLIPPED)) { emit = 1; e = make_uint2(m, (uint32_t)i); }
+                bool fluid;
+                if (gfl) {
+                    fluid = gidx[k][i] < 0 && gfl[k][i] == FLUID;
+                } else {
+                    const uint8_t fm = A.flag[m];
+                    fluid = (fm & TYPE_MASK) == FLUID && !(fm & FLIPPED);
+                }
+                if (fluid) { emit = 1; e = make_uint2(m, (uint32_t)i); }
             }
         }
         int total;
@@ -180,6 +198,8 @@ __device__ void squirmer_block(const RemapArgs<S>& A, const double* part, int np
     __shared__ uint8_t gfl[GATHER][Q];    // neighbour i's type before the remap
     __shared__ int16_t gidx[GATHER][Q];   // neighbour i's list index, or -1
     __shared__ double sprune[2];          // candidate band of squared distances
+    __shared__ double srv[Q];             // refill: density of neighbour i (FLUID ones)
+    __shared__ int sisf[Q];               // refill: neighbour i is FLUID
     __shared__ int sbox[6];  // x0, y0, z0, wx, wy, wz
     __shared__ int scount[SQ_THREADS / 64];
     __shared__ int sK, sOverflow;
@@ -428,61 +448,73 @@ __device__ void squirmer_block(const RemapArgs<S>& A, const double* part, int np
     const bool wrapx = bwx == NX, wrapy = bwy == NY, wrapz = bwz == NZ;
     for (int c0 = 0; c0 < K; c0 += GATHER) {
         const int nc = (K - c0 < GATHER) ? K - c0 : GATHER;
-        // C1: gather.  Flags already changed by an earlier pass carry FLIPPED,
-        // which gives back the type before this remap.
+        // C1: gather.  For direction i > 0 each work item looks at one
+        // neighbour: the upstream n - c_i of a covered node (where its f_i
+        // comes from), the downstream n + c_i of an uncovered one (a density
+        // the refill may average): its type before the remap (flags changed
+        // by an earlier pass carry FLIPPED, which gives it back) and its
+        // index in the list if it changes too.  Every load of an item is
+        // issued at once (both slots that may hold f_i, the density whatever
+        // the type) and the type picks afterwards.
         for (int w = tid; w < nc * Q; w += SQ_THREADS) {
             const int kk = w / Q, i = w - kk * Q, k = c0 + kk;
             const bool cover = lcov[k] != 0;
             const int p = lpos[k];
             const int ix = p % bwx, iy = (p / bwx) % bwy, iz = p / (bwx * bwy);
             const int x = boxc[0][ix], y = boxc[1][iy], z = boxc[2][iz];
-            if (cover) {
-                // population f_i(n) after the last step, lbm.cpp:295-298
-                size_t slot = (size_t)i * SS + pop_index(x, y, z, A.PX, NY);
-                if (i > 0) {
-                    const int xm = step1(x, -CX[i], NX), ym = step1(y, -CY[i], NY), zm = step1(z, -CZ[i], NZ);
-                    const uint32_t m = (uint32_t)xm + (uint32_t)NX * ((uint32_t)ym + (uint32_t)NY * (uint32_t)zm);
-                    bool up_fluid = false;
-                    if (!A.shifted) {
-                        const uint8_t fm = A.flag[m];
-                        uint8_t tm = fm & TYPE_MASK;
-                        if (fm & FLIPPED) tm = (tm == FLUID) ? PARTICLE : FLUID;
-                        up_fluid = tm == FLUID;
-                    }
-                    if (!up_fluid) slot = (size_t)OPP[i] * SS + pop_index(xm, ym, zm, A.PX, NY);
+            if (i == 0) {
+                if (cover) {  // rest population f_0(n), lbm.cpp:295-298
+                    double fv = to_real<S>(A.f[pop_index(x, y, z, A.PX, NY)], 0);
+                    if (dsh != 0.0) fv += W[0] * dsh;
+                    gval[kk][0] = fv;
                 }
-                double fv = to_real<S>(A.f[slot], i);
+                continue;
+            }
+            const int sg = cover ? -1 : 1;
+            const int xm = step1(x, sg * CX[i], NX), ym = step1(y, sg * CY[i], NY), zm = step1(z, sg * CZ[i], NZ);
+            const uint32_t m = (uint32_t)xm + (uint32_t)NX * ((uint32_t)ym + (uint32_t)NY * (uint32_t)zm);
+            const uint8_t fm = A.flag[m];
+            S own{}, far{};
+            double rho_m = 0.0;
+            if (cover) {
+                own = A.f[(size_t)i * SS + pop_index(x, y, z, A.PX, NY)];
+                far = A.f[(size_t)OPP[i] * SS + pop_index(xm, ym, zm, A.PX, NY)];
+            } else {
+                rho_m = load_macro<S>(A.rho, m);
+            }
+            uint8_t tm = fm & TYPE_MASK;
+            if (fm & FLIPPED) tm = (tm == FLUID) ? PARTICLE : FLUID;
+            gfl[kk][i] = tm;
+            if (cover) {
+                // population f_i(n) after the last step, lbm.cpp:295-298: after an
+                // odd step in slot i at n when n - c_i is FLUID, else (and after
+                // an even step) in slot opp(i) at n - c_i
+                const bool up_fluid = !A.shifted && tm == FLUID;
+                double fv = to_real<S>(up_fluid ? own : far, i);
                 if (dsh != 0.0) fv += W[i] * dsh;
                 gval[kk][i] = fv;
-            } else if (i > 0) {
-                // neighbour n + c_i: type and density before the remap
-                const uint32_t m = (uint32_t)step1(x, CX[i], NX) +
-                                   (uint32_t)NX * ((uint32_t)step1(y, CY[i], NY) + (uint32_t)NY * (uint32_t)step1(z, CZ[i], NZ));
-                const uint8_t fm = A.flag[m];
-                uint8_t tm = fm & TYPE_MASK;
-                if (fm & FLIPPED) tm = (tm == FLUID) ? PARTICLE : FLUID;
-                gfl[kk][i] = tm;
-                gval[kk][i] = tm == FLUID ? load_macro<S>(A.rho, m) : 0.0;
-                // its index in the list, if it is in the box and changes too
-                int jx = ix + CX[i], jy = iy + CY[i], jz = iz + CZ[i];
-                if (wrapx) jx = jx < 0 ? bwx - 1 : (jx == bwx ? 0 : jx);
-                if (wrapy) jy = jy < 0 ? bwy - 1 : (jy == bwy ? 0 : jy);
-                if (wrapz) jz = jz < 0 ? bwz - 1 : (jz == bwz ? 0 : jz);
-                int j = -1;
-                if (jx >= 0 && jx < bwx && jy >= 0 && jy < bwy && jz >= 0 && jz < bwz) {
-                    const int q = jx + bwx * (jy + bwy * jz);
-                    int a = 0, bnd = K;  // lower bound of q in lpos[0, K)
-                    while (a < bnd) {
-                        const int mid = (a + bnd) >> 1;
-                        if (lpos[mid] < q) a = mid + 1; else bnd = mid;
-                    }
-                    if (a < K && lpos[a] == q) j = a;
-                }
-                gidx[kk][i] = (int16_t)j;
+            } else {
+                gval[kk][i] = tm == FLUID ? rho_m : 0.0;
             }
+            int jx = ix + sg * CX[i], jy = iy + sg * CY[i], jz = iz + sg * CZ[i];
+            if (wrapx) jx = jx < 0 ? bwx - 1 : (jx == bwx ? 0 : jx);
+            if (wrapy) jy = jy < 0 ? bwy - 1 : (jy == bwy ? 0 : jy);
+            if (wrapz) jz = jz < 0 ? bwz - 1 : (jz == bwz ? 0 : jz);
+            int j = -1;
+            if (jx >= 0 && jx < bwx && jy >= 0 && jy < bwy && jz >= 0 && jz < bwz) {
+                const int q = jx + bwx * (jy + bwy * jz);
+                int a = 0, bnd = K;  // lower bound of q in lpos[0, K)
+                while (a < bnd) {
+                    const int mid = (a + bnd) >> 1;
+                    if (lpos[mid] < q) a = mid + 1; else bnd = mid;
+                }
+                if (a < K && lpos[a] == q) j = a;
+            }
+            gidx[kk][i] = (int16_t)j;
         }
         __syncthreads();
-        // C2: the sweep, from LDS
+        // C2: the sweep, from LDS, by the first wavefront: every lane forms the
+        // same sums in the reference's order from broadcast LDS reads
         if (wave == 0) {
             const int i = lane;  // direction handled by this lane (lanes >= Q idle)
             for (int kk = 0; kk < nc; ++kk) {
@@ -493,10 +525,9 @@ __device__ void squirmer_block(const RemapArgs<S>& A, const double* part, int np
                 const int ix = p % bwx, iy = (p / bwx) % bwy, iz = p / (bwx * bwy);
                 const double r[3] = {boxr[0][ix], boxr[1][iy], boxr[2][iz]};  // lbm.cpp:100-104
                 if (cover) {
-                    const double fv = i < Q ? gval[kk][i] : 0.0;
                     double pm[3] = {0, 0, 0}, msum = 0.0;
                     for (int j = 0; j < Q; ++j) {
-                        const double v = __shfl(fv, j, 64);
+                        const double v = gval[kk][j];
                         pm[0] += v * CX[j]; pm[1] += v * CY[j]; pm[2] += v * CZ[j];
                         msum += v;
                     }
@@ -520,12 +551,13 @@ __device__ void squirmer_block(const RemapArgs<S>& A, const double* part, int np
                             if (isf) rv = gval[kk][i];
                         }
                     }
+                    if (i >= 1 && i < Q) { srv[i] = rv; sisf[i] = isf; }
+                    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                    __builtin_amdgcn_wave_barrier();
                     double rs = 0.0;
                     int c = 0;
-                    for (int j = 1; j < Q; ++j) {
-                        const double v = __shfl(rv, j, 64);
-                        if (__shfl(isf, j, 64)) { rs += v; ++c; }
-                    }
+                    for (int j = 1; j < Q; ++j)
+                        if (sisf[j]) { rs += srv[j]; ++c; }
                     const double rr = c ? rs / c : 1.0;
                     double ub[3];  // Squirmer::surfaceVelocity(r, ub, false)
                     ub[0] = sq.U[0] + (sq.W[1] * r[2] - sq.W[2] * r[1]);
@@ -604,7 +636,7 @@ __device__ void squirmer_block(const RemapArgs<S>& A, const double* part, int np
         }
     }
     __syncthreads();
-    if (A.maintain) append_links(A, list, lcov, K, hs);
+    if (A.maintain) append_links(A, list, lcov, K, hs, K <= GATHER ? gfl : nullptr, gidx);
     for (int k = tid; k < K; k += SQ_THREADS) {
         const uint32_t n = list[k];
         A.flag[n] = A.flag[n] & TYPE_MASK;
@@ -618,17 +650,20 @@ __device__ void squirmer_block(const RemapArgs<S>& A, const double* part, int np
                   z0 = (int)(n0 / ((uint32_t)NX * (uint32_t)NY));
         const int x = step1(x0, CX[i], NX), y = step1(y0, CY[i], NY), z = step1(z0, CZ[i], NZ);
         const uint32_t m = (uint32_t)x + (uint32_t)NX * ((uint32_t)y + (uint32_t)NY * (uint32_t)z);
-        const uint8_t t = A.flag[m] & TYPE_MASK;
-        bool b = false;
-        if (t == FLUID) {
+        // the node's flag and its neighbours' in one round of loads (only the
+        // type bits are read, which no thread changes here)
+        uint8_t nb[Q];
 #pragma unroll
-            for (int j = 1; j < Q; ++j) {
-                const uint32_t mm = (uint32_t)step1(x, CX[j], NX) +
-                                    (uint32_t)NX * ((uint32_t)step1(y, CY[j], NY) + (uint32_t)NY * (uint32_t)step1(z, CZ[j], NZ));
-                b |= (A.flag[mm] & TYPE_MASK) != FLUID;
-            }
+        for (int j = 0; j < Q; ++j) {
+            const uint32_t mm = (uint32_t)step1(x, CX[j], NX) +
+                                (uint32_t)NX * ((uint32_t)step1(y, CY[j], NY) + (uint32_t)NY * (uint32_t)step1(z, CZ[j], NZ));
+            nb[j] = A.flag[mm];
         }
-        A.flag[m] = (uint8_t)(t | (b ? BOUNDARY : 0));
+        const uint8_t t = nb[0] & TYPE_MASK;
+        bool b = false;
+#pragma unroll
+        for (int j = 1; j < Q; ++j) b |= (nb[j] & TYPE_MASK) != FLUID;
+        A.flag[m] = (uint8_t)(t | (t == FLUID && b ? BOUNDARY : 0));
     }
     if (tid == 0) {
         // rho band the next step writes: the next remap box plus neighbours,
