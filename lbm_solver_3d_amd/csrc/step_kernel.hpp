@@ -359,6 +359,10 @@ __device__ __forceinline__ void block_reduce(double (&v)[K], double* lds) {
 #ifndef LBM_STEP_WAVES_PER_EU
 #define LBM_STEP_WAVES_PER_EU 4
 #endif
+#ifndef LBM_ODD_NT
+#define LBM_ODD_NT 1
+#endif
+constexpr bool ODD_NT = LBM_ODD_NT != 0;
 
 // Neighbour offsets of node (x, y, z): node-array rows (yo, zo, pitch NX)
 // and population rows (pyo, pzo, pitch PX); x offsets in nodes and in bytes.
@@ -501,6 +505,7 @@ __device__ __forceinline__ double node_update(const StepArgs<S>& A, const FastCo
         constexpr int i = decltype(I)::value;
         if constexpr (!ODD && !BPATH) g[i] = St::load(load_stream(at(base(i, yo[1] + zo[1]), xb[1])));
         else if constexpr (!ODD) g[i] = St::load(*at(base(i, yo[1] + zo[1]), xb[1]));
+        else if constexpr (CX[i] == 0 && ODD_NT) g[i] = St::load(load_stream(at(base(OPP[i], yo[1 - CY[i]] + zo[1 - CZ[i]]), xb[1])));
         else g[i] = St::load(*at(base(OPP[i], yo[1 - CY[i]] + zo[1 - CZ[i]]), xb[1 - CX[i]]));
     });
     // even step, boundary node: populations whose upstream node is solid
@@ -536,6 +541,7 @@ __device__ __forceinline__ double node_update(const StepArgs<S>& A, const FastCo
         const S val = St::store(g[i]);
         if constexpr (!ODD && !BPATH) store_stream(at(base(OPP[i], yo[1] + zo[1]), xb[1]), val);
         else if constexpr (!ODD) *at(base(OPP[i], yo[1] + zo[1]), xb[1]) = val;
+        else if constexpr (CX[i] == 0 && ODD_NT) store_stream(at(base(i, yo[1 + CY[i]] + zo[1 + CZ[i]]), xb[1]), val);
         else *at(base(i, yo[1 + CY[i]] + zo[1 + CZ[i]]), xb[1 + CX[i]]) = val;
     });
     // even step, boundary node: bounce-back into the solid neighbour's slot i
