@@ -102,6 +102,8 @@ typedef struct lbm_hip_info {
     int row_pitch;          /* nodes per stored population row (nx, or nx rounded up to 64) */
     int tile_width;         /* collide-stream tile width in x (64 / 128 / 256; 0: row runs) */
     int xcd_remap;          /* tile order over the XCDs (lbm_hip_set_tiling) */
+    int occupancy_even;     /* collide-stream blocks per CU on even / odd steps (lbm_hip_set_occupancy) */
+    int occupancy_odd;
 } lbm_hip_info;
 
 typedef struct lbm_hip_ctx lbm_hip_ctx;
@@ -233,6 +235,15 @@ int lbm_hip_autotune(lbm_hip_ctx* ctx, int trial_steps, int mass_every, lbm_hip_
  * order 2. */
 int lbm_hip_set_tiling(lbm_hip_ctx* ctx, int tile_width, int xcd_remap);
 
+/* Cap on the collide-stream blocks (256 threads each) resident per compute
+ * unit, for even and odd steps: 2..8, or 0 for no cap (the registers then
+ * allow 8).  No counterpart in the reference.  The in-place update is bound
+ * by HBM, and fewer wavefronts in flight can move its bytes faster; the cap
+ * is set by reserving LDS per block, so results do not change.  Defaults,
+ * measured on the 512 x 256 x 128 channel: FP64 3 / 3, FP32 4 / 4, FP16 and
+ * BF16 0 / 6. */
+int lbm_hip_set_occupancy(lbm_hip_ctx* ctx, int even_blocks, int odd_blocks);
+
 /* ---- ensemble of lattices ----
  * No counterpart in the reference (one run per process there); the members
  * are what `count` separate lbm_hip_create calls would make, and each
@@ -272,6 +283,9 @@ int lbm_hip_batch_info_get(lbm_hip_batch* b, lbm_hip_batch_info* out);
 int lbm_hip_batch_autotune(lbm_hip_batch* b, int trial_steps, int mass_every, lbm_hip_tuning* out);
 /* One tile shape for all members, as lbm_hip_set_tiling. */
 int lbm_hip_batch_set_tiling(lbm_hip_batch* b, int tile_width, int xcd_remap);
+
+/* One occupancy cap for all members, as lbm_hip_set_occupancy. */
+int lbm_hip_batch_set_occupancy(lbm_hip_batch* b, int even_blocks, int odd_blocks);
 /* Bench support: time `steps` graph-replayed steps with HIP events. */
 int lbm_hip_batch_time_run(lbm_hip_batch* b, int steps, int mass_every, double* total_ms);
 

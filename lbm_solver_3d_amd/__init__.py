@@ -83,6 +83,7 @@ class Info(C.Structure):
         ("has_squirmer", C.c_int), ("walls_y", C.c_int), ("walls_z", C.c_int),
         ("count_fluid", C.c_int64),
         ("row_pitch", C.c_int), ("tile_width", C.c_int), ("xcd_remap", C.c_int),
+        ("occupancy_even", C.c_int), ("occupancy_odd", C.c_int),
     ]
 
 
@@ -165,6 +166,7 @@ def load_library(path: Path | None = None):
     lib.lbm_hip_time_run.argtypes = [ctx, C.c_int, C.c_int, dp, dp]
     lib.lbm_hip_autotune.argtypes = [ctx, C.c_int, C.c_int, C.POINTER(Tuning)]
     lib.lbm_hip_set_tiling.argtypes = [ctx, C.c_int, C.c_int]
+    lib.lbm_hip_set_occupancy.argtypes = [ctx, C.c_int, C.c_int]
     bt = C.c_void_p
     lib.lbm_hip_batch_create.argtypes = [P, C.c_int, C.POINTER(C.c_void_p)]
     lib.lbm_hip_batch_destroy.argtypes = [bt]
@@ -175,6 +177,7 @@ def load_library(path: Path | None = None):
     lib.lbm_hip_batch_info_get.argtypes = [bt, C.POINTER(BatchInfo)]
     lib.lbm_hip_batch_autotune.argtypes = [bt, C.c_int, C.c_int, C.POINTER(Tuning)]
     lib.lbm_hip_batch_set_tiling.argtypes = [bt, C.c_int, C.c_int]
+    lib.lbm_hip_batch_set_occupancy.argtypes = [bt, C.c_int, C.c_int]
     lib.lbm_hip_batch_time_run.argtypes = [bt, C.c_int, C.c_int, dp]
     lib.lbm_hip_copy_bandwidth.argtypes = [C.c_int, C.c_size_t, C.c_int, dp]
     lib.lbm_hip_copy_bandwidth_parts.argtypes = [C.c_int, C.c_size_t, C.c_int, C.c_int, dp]
@@ -376,6 +379,10 @@ class Lattice:
     def set_tiling(self, tile_width: int, xcd_remap: int = 1):
         _check(self.lib, self.lib.lbm_hip_set_tiling(self.h, tile_width, xcd_remap))
 
+    def set_occupancy(self, even_blocks: int, odd_blocks: int):
+        """Cap on collide-stream blocks per CU for even / odd steps (0: none)."""
+        _check(self.lib, self.lib.lbm_hip_set_occupancy(self.h, even_blocks, odd_blocks))
+
 
 class Batch:
     """An ensemble of lattices advanced together (one launch per phase for
@@ -438,6 +445,9 @@ class Batch:
 
     def set_tiling(self, tile_width: int, xcd_remap: int = 1):
         _check(self.lib, self.lib.lbm_hip_batch_set_tiling(self.h, tile_width, xcd_remap))
+
+    def set_occupancy(self, even_blocks: int, odd_blocks: int):
+        _check(self.lib, self.lib.lbm_hip_batch_set_occupancy(self.h, even_blocks, odd_blocks))
 
     def time_run(self, steps: int, mass_every: int = 10) -> float:
         tot = C.c_double()

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdint>
 
+#include "fast_div.hpp"
 #include "lattice.hpp"
 
 namespace lbmhip {
@@ -114,6 +115,7 @@ struct StepArgs {
     int nchx;             // 64-node chunks per row
     int ty;               // tile height in y
     int xcd_remap;        // 1: give each XCD a contiguous range of tiles; 2: whole tile rows per XCD
+    FastDiv div_gx, div_gy, div_nchx, div_ny;  // division by gx, gy, nchx and NY (tile decoding)
     double omPlus, omMinus, Fx, Fy, Fz;
     long long t;          // step index (selects double-buffer slots)
 };

@@ -4,9 +4,9 @@
 #include "launch_step_impl.hpp"
 
 namespace lbmhip {
-template hipError_t launch_step<float>(const StepArgs<float>&, const FastCoef&, bool, int, hipStream_t);
-template hipError_t launch_step<_Float16>(const StepArgs<_Float16>&, const FastCoef&, bool, int, hipStream_t);
-template hipError_t launch_step<bf16_t>(const StepArgs<bf16_t>&, const FastCoef&, bool, int, hipStream_t);
+template hipError_t launch_step<float>(const StepArgs<float>&, const FastCoef&, bool, int, unsigned, hipStream_t);
+template hipError_t launch_step<_Float16>(const StepArgs<_Float16>&, const FastCoef&, bool, int, unsigned, hipStream_t);
+template hipError_t launch_step<bf16_t>(const StepArgs<bf16_t>&, const FastCoef&, bool, int, unsigned, hipStream_t);
 template hipError_t launch_boundary<float>(const StepArgs<float>&, const FastCoef&, const BoundaryArgs&, int, int,
                                             hipStream_t);
 template hipError_t launch_boundary<_Float16>(const StepArgs<_Float16>&, const FastCoef&, const BoundaryArgs&, int, int,
@@ -15,7 +15,7 @@ template hipError_t launch_boundary<bf16_t>(const StepArgs<bf16_t>&, const FastC
                                             hipStream_t);
 #define LBMHIP_INSTANTIATE_BATCH(S)                                                                              \
     template hipError_t launch_batch_step<S>(const BatchMember<S>*, const uint32_t*, int, long long, int, bool, \
-                                             int, hipStream_t);                                                 \
+                                             int, unsigned, hipStream_t);                                       \
     template hipError_t launch_batch_boundary<S>(const BatchMember<S>*, const uint32_t*, int, long long, int,    \
                                                  hipStream_t);
 LBMHIP_INSTANTIATE_BATCH(float)

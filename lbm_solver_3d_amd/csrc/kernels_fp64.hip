@@ -120,11 +120,11 @@ LBMHIP_INSTANTIATE(_Float16)
 LBMHIP_INSTANTIATE(bf16_t)
 #undef LBMHIP_INSTANTIATE
 
-template hipError_t launch_step<double>(const StepArgs<double>&, const FastCoef&, bool, int, hipStream_t);
+template hipError_t launch_step<double>(const StepArgs<double>&, const FastCoef&, bool, int, unsigned, hipStream_t);
 template hipError_t launch_boundary<double>(const StepArgs<double>&, const FastCoef&, const BoundaryArgs&, int, int,
                                             hipStream_t);
 template hipError_t launch_batch_step<double>(const BatchMember<double>*, const uint32_t*, int, long long, int, bool,
-                                              int, hipStream_t);
+                                              int, unsigned, hipStream_t);
 template hipError_t launch_batch_boundary<double>(const BatchMember<double>*, const uint32_t*, int, long long, int,
                                                   hipStream_t);
 

@@ -16,8 +16,11 @@
 
 namespace lbmhip {
 
+// `mode`: STEP_MACRO | STEP_MASS as requested, plus STEP_BODY_FORCE when the
+// lattice (for an ensemble: any member) has a body force.  `lds`: dynamic
+// LDS bytes per block, which caps the blocks resident per CU (0: no cap).
 template <class S>
-hipError_t launch_step(const StepArgs<S>& a, const FastCoef& k, bool odd, int mode, hipStream_t s);
+hipError_t launch_step(const StepArgs<S>& a, const FastCoef& k, bool odd, int mode, unsigned lds, hipStream_t s);
 
 template <class S>
 hipError_t launch_boundary(const StepArgs<S>& a, const FastCoef& k, const BoundaryArgs& b, int blocks, int mode,
@@ -59,7 +62,7 @@ hipError_t launch_fill_populations(S* f, uint32_t SS, hipStream_t s);
 // Ensemble launches (batch_kernels.hpp): all members in one launch per phase.
 template <class S>
 hipError_t launch_batch_step(const BatchMember<S>* m, const uint32_t* tile_map, int Gtot, long long t, int xcd,
-                             bool odd, int mode, hipStream_t s);
+                             bool odd, int mode, unsigned lds, hipStream_t s);
 template <class S>
 hipError_t launch_batch_boundary(const BatchMember<S>* m, const uint32_t* block_map, int blocks, long long t,
                                  int mode, hipStream_t s);
@@ -88,7 +91,8 @@ hipError_t launch_copy_probe(const void* a, void* b, size_t n16, hipStream_t s);
 hipError_t launch_copy_parts(const void* const* a, void* const* b, int parts, size_t n16, hipStream_t s);
 
 #define LBMHIP_DECLARE(S)                                                                                          \
-    extern template hipError_t launch_step<S>(const StepArgs<S>&, const FastCoef&, bool, int, hipStream_t);     \
+    extern template hipError_t launch_step<S>(const StepArgs<S>&, const FastCoef&, bool, int, unsigned,         \
+                                              hipStream_t);                                                     \
     extern template hipError_t launch_boundary<S>(const StepArgs<S>&, const FastCoef&, const BoundaryArgs&, int, \
                                                   int, hipStream_t);                                             \
     extern template hipError_t launch_couple<S>(const LinkArgs<S>&, const RemapArgs<S>&, int, int, int,         \
@@ -103,7 +107,7 @@ hipError_t launch_copy_parts(const void* const* a, void* const* b, int parts, si
                                               uint32_t, uint32_t, int, const DevState*, int, hipStream_t);                \
     extern template hipError_t launch_fill_populations<S>(S*, uint32_t, hipStream_t);                         \
     extern template hipError_t launch_batch_step<S>(const BatchMember<S>*, const uint32_t*, int, long long, int, \
-                                                    bool, int, hipStream_t);                                    \
+                                                    bool, int, unsigned, hipStream_t);                          \
     extern template hipError_t launch_batch_boundary<S>(const BatchMember<S>*, const uint32_t*, int, long long,  \
                                                         int, hipStream_t);                                      \
     extern template hipError_t launch_batch_couple<S>(const BatchMember<S>*, const uint32_t*, int, int, long long, \

@@ -85,6 +85,10 @@ struct lbm_hip_ctx {
     int bblocks = 0;                // blocks of aa_boundary_kernel
     int gx = 0, gy = 0, gz = 0, tx_shift = 8, ty = 1, G = 0, nb256 = 0;
     int xcd_remap = 2;  // whole tile rows per XCD until set_tiling / autotune chooses
+    // cap on the collide-stream blocks resident per CU on even / odd steps
+    // (0: none) and the dynamic LDS per block that enforces it
+    int occ_cap[2] = {0, 0};
+    unsigned occ_lds[2] = {0, 0};
     int G_max = 0;  // largest tile count over the tile shapes (mass_part holds G_max + bblocks)
     double omPlus = 0, omMinus = 0, nu = 0;
     FastCoef K{};
@@ -226,6 +230,10 @@ void band_of(const DevSquirmer& s, int NX, int NY, int NZ, int band[6]) {
     }
 }
 
+// The lattice has a body force: its steps run the collision with the
+// forcing terms (STEP_BODY_FORCE).
+bool body_force(const lbm_hip_ctx* c) { return c->p.fx != 0.0 || c->p.fy != 0.0 || c->p.fz != 0.0; }
+
 FastCoef make_coef(double omP, double omM, double Fx, double Fy, double Fz, double scale) {
     FastCoef k{};
     const double gP = 1.0 - 0.5 * omP, gM = 1.0 - 0.5 * omM;
@@ -242,6 +250,7 @@ FastCoef make_coef(double omP, double omM, double Fx, double Fy, double Fz, doub
     k.gPw3[0] = (float)(3.0 * gP * w1); k.gPw3[1] = (float)(3.0 * gP * w2); k.gPw3[2] = (float)(3.0 * gP * w0);
     k.sm_axis = (float)(3.0 * scale * gM * w1); k.sm_diag = (float)(3.0 * scale * gM * w2);
     k.Fx = (float)Fx; k.Fy = (float)Fy; k.Fz = (float)Fz;
+    k.one = 1.0f;
     return k;
 }
 
@@ -255,6 +264,8 @@ StepArgs<S> step_args(const lbm_hip_ctx* c) {
     a.walls_y = c->p.walls_y; a.walls_z = c->p.walls_z; a.has_squirmer = c->has_squirmer;
     a.gx = c->gx; a.gy = c->gy; a.gz = c->gz; a.tx_shift = c->tx_shift; a.ty = c->ty;
     a.nchx = (c->NX + 63) / 64;
+    a.div_gx = make_fast_div((uint32_t)a.gx); a.div_gy = make_fast_div((uint32_t)a.gy);
+    a.div_nchx = make_fast_div((uint32_t)a.nchx); a.div_ny = make_fast_div((uint32_t)a.NY);
     a.xcd_remap = c->xcd_remap;
     a.omPlus = c->omPlus; a.omMinus = c->omMinus;
     a.Fx = c->p.fx; a.Fy = c->p.fy; a.Fz = c->p.fz;
@@ -367,6 +378,47 @@ int tiles_for(int nx, int ny, int nz, int ts) {
 // Tile shape of the collide-stream kernel (ts = log2 of the tile width, 6..8,
 // or 0 for row runs).  The boundary kernel's mass partials follow the step
 // kernel's.
+// Dynamic LDS bytes per block that leave room for at most `blocks` blocks of
+// the collide-stream kernel on a CU (0: no cap).  The kernel's own LDS
+// (under 512 bytes) fits in the 512-byte margin below the CU's LDS / blocks,
+// and the result stays above the CU's LDS / (blocks + 1).
+int occupancy_lds(int device, int blocks, unsigned* bytes) {
+    *bytes = 0;
+    if (blocks == 0) return LBM_HIP_OK;
+    int per_cu = 0, per_block = 0;
+    if (hipDeviceGetAttribute(&per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device) != hipSuccess ||
+        hipDeviceGetAttribute(&per_block, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess)
+        return fail(LBM_HIP_EDEVICE, "LDS size query failed");
+    const long b = std::min((long)(per_cu / blocks) / 512 * 512, (long)per_block) - 512;
+    if (b <= (long)(per_cu / (blocks + 1))) return fail(LBM_HIP_EINVAL, "occupancy cap out of range for this device");
+    *bytes = (unsigned)b;
+    return LBM_HIP_OK;
+}
+
+// Measured on the 512 x 256 x 128 channel: the FP32 and FP64 updates move
+// their bytes fastest with 4 and 3 blocks per CU on both parities, FP16 and
+// BF16 with the even step uncapped and the odd step at 6.
+void default_occupancy(int prec, int cap[2]) {
+    switch (prec) {
+        case LBM_HIP_FP64: cap[0] = cap[1] = 3; break;
+        case LBM_HIP_FP32: cap[0] = cap[1] = 4; break;
+        default: cap[0] = 0; cap[1] = 6; break;
+    }
+}
+
+int set_occupancy(lbm_hip_ctx* c, int even_blocks, int odd_blocks) {
+    for (int b : {even_blocks, odd_blocks})
+        if (b != 0 && (b < 2 || b > 8)) return fail(LBM_HIP_EINVAL, "blocks per CU must be 0 (no cap) or 2..8");
+    unsigned lds[2];
+    if (int rc = occupancy_lds(c->device, even_blocks, &lds[0])) return rc;
+    if (int rc = occupancy_lds(c->device, odd_blocks, &lds[1])) return rc;
+    if (c->stream && hipStreamSynchronize(c->stream) != hipSuccess) return fail(LBM_HIP_EDEVICE, "stream sync");
+    c->occ_cap[0] = even_blocks; c->occ_cap[1] = odd_blocks;
+    c->occ_lds[0] = lds[0]; c->occ_lds[1] = lds[1];
+    drop_graphs(c);  // captured launches carry the old LDS size
+    return LBM_HIP_OK;
+}
+
 void set_tile_shape(lbm_hip_ctx* c, int ts) {
     c->tx_shift = ts;
     if (ts == 0) {
@@ -522,6 +574,11 @@ int lbm_hip_create(const lbm_hip_params* pin, lbm_hip_ctx** out) {
     if (hipSetDevice(c->device) != hipSuccess) return cleanup(fail(LBM_HIP_EDEVICE, "hipSetDevice failed"));
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
         return cleanup(fail(LBM_HIP_EDEVICE, "hipStreamCreate failed"));
+    {
+        int cap[2];
+        default_occupancy(c->prec, cap);
+        if (int rc2 = set_occupancy(c, cap[0], cap[1])) return cleanup(rc2);
+    }
 
     const size_t N = c->N;
     auto dmalloc = [&](void** q, size_t bytes) -> bool {
@@ -638,6 +695,7 @@ int lbm_hip_step(lbm_hip_ctx* c, int flags) {
     int mode = 0;
     if (flags & LBM_STEP_MACRO) mode |= STEP_MACRO;
     if (flags & LBM_STEP_MASS) mode |= STEP_MASS;
+    if (body_force(c)) mode |= STEP_BODY_FORCE;
     const bool odd = (c->t & 1) != 0;
     if (int rc = flush_links(c)) return rc;
     hipError_t e = hipSuccess;
@@ -645,7 +703,7 @@ int lbm_hip_step(lbm_hip_ctx* c, int flags) {
         using S = std::remove_pointer_t<decltype(tag)>;
         const StepArgs<S> sa = step_args<S>(c);
         if (c->kev[0]) e = hipEventRecord(c->kev[0], c->stream);
-        if (e == hipSuccess) e = launch_step<S>(sa, c->K, odd, mode, c->stream);
+        if (e == hipSuccess) e = launch_step<S>(sa, c->K, odd, mode, c->occ_lds[odd], c->stream);
         if (e == hipSuccess && !odd) e = launch_boundary<S>(sa, c->K, c->bargs, c->bblocks, mode, c->stream);
         if (e == hipSuccess && c->kev[1]) e = hipEventRecord(c->kev[1], c->stream);
         // the moving-boundary bounce-back and force/torque on the squirmer
@@ -988,6 +1046,8 @@ int lbm_hip_get_info(lbm_hip_ctx* c, lbm_hip_info* o) {
     o->row_pitch = (int)c->PX;
     o->tile_width = c->tx_shift ? 1 << c->tx_shift : 0;
     o->xcd_remap = c->xcd_remap;
+    o->occupancy_even = c->occ_cap[0];
+    o->occupancy_odd = c->occ_cap[1];
     return LBM_HIP_OK;
 }
 
@@ -1174,6 +1234,11 @@ int lbm_hip_set_tiling(lbm_hip_ctx* c, int tile_width, int xcd_remap) {
     return LBM_HIP_OK;
 }
 
+int lbm_hip_set_occupancy(lbm_hip_ctx* c, int even_blocks, int odd_blocks) {
+    if (!c) return fail(LBM_HIP_EINVAL, "null context");
+    return set_occupancy(c, even_blocks, odd_blocks);
+}
+
 int lbm_hip_autotune(lbm_hip_ctx* c, int trial_steps, int mass_every, lbm_hip_tuning* out) {
     if (!c) return fail(LBM_HIP_EINVAL, "null context");
     if (trial_steps < 1) return fail(LBM_HIP_EINVAL, "trial_steps must be >= 1");
@@ -1239,6 +1304,7 @@ struct lbm_hip_batch {
     uint32_t* lmap = nullptr;         // member << 24 | link block
     int Gtot = 0, Btot = 0, Ltot = 0;
     bool any_squirmer = false;
+    bool any_force = false;           // some member has a body force (all run the forcing terms)
     long long t = 0;
     hipGraphExec_t graph = nullptr;
     int graph_cycle = 0, graph_mass_every = -1;
@@ -1342,12 +1408,14 @@ int batch_step(lbm_hip_batch* b, int mass_every, int extra_flags) {
     const bool mass = b->any_squirmer && mass_every > 0 && tnext % mass_every == 0;
     int mode = mass ? STEP_MASS : 0;
     if (extra_flags & LBM_STEP_MACRO) mode |= STEP_MACRO;
+    if (b->any_force) mode |= STEP_BODY_FORCE;
     const bool odd = (b->t & 1) != 0;
     hipError_t e = hipSuccess;
     dispatch(b->prec, [&](auto* tag) -> int {
         using S = std::remove_pointer_t<decltype(tag)>;
         const auto* d = static_cast<const BatchMember<S>*>(b->desc);
-        e = launch_batch_step<S>(d, b->tile_map, b->Gtot, b->t, b->mem[0]->xcd_remap == 1, odd, mode, b->stream);
+        e = launch_batch_step<S>(d, b->tile_map, b->Gtot, b->t, b->mem[0]->xcd_remap == 1, odd, mode,
+                                 b->mem[0]->occ_lds[odd], b->stream);
         if (e == hipSuccess && !odd) e = launch_batch_boundary<S>(d, b->bmap, b->Btot, b->t, mode, b->stream);
         if (e == hipSuccess && (extra_flags & LBM_STEP_WALL_FORCE))
             for (lbm_hip_ctx* c : b->mem)
@@ -1494,6 +1562,7 @@ int lbm_hip_batch_create(const lbm_hip_params* members, int count, lbm_hip_batch
         b->mem.push_back(c);
         b->device_bytes += c->device_bytes;
         b->any_squirmer = b->any_squirmer || c->has_squirmer;
+        b->any_force = b->any_force || body_force(c);
         if (c->G_max >= (1 << 24) || c->bblocks >= (1 << 24) || c->cpl_blocks >= (1 << 24))
             return bail(fail(LBM_HIP_EINVAL, "member " + std::to_string(m) + " too large for a batch"));
     }
@@ -1573,6 +1642,17 @@ int lbm_hip_batch_set_tiling(lbm_hip_batch* b, int tile_width, int xcd_remap) {
     for (lbm_hip_ctx* c : b->mem)
         if ((rc = lbm_hip_set_tiling(c, tile_width, xcd_remap))) return rc;
     return batch_tables(b);
+}
+
+int lbm_hip_batch_set_occupancy(lbm_hip_batch* b, int even_blocks, int odd_blocks) {
+    if (!b) return fail(LBM_HIP_EINVAL, "null batch");
+    int rc = batch_check(b);
+    if (rc) return rc;
+    for (lbm_hip_ctx* c : b->mem)
+        if ((rc = set_occupancy(c, even_blocks, odd_blocks))) return rc;
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    if (b->graph) { hipGraphExecDestroy(b->graph); b->graph = nullptr; }
+    return LBM_HIP_OK;
 }
 
 int lbm_hip_batch_autotune(lbm_hip_batch* b, int trial_steps, int mass_every, lbm_hip_tuning* out) {

@@ -42,6 +42,11 @@ namespace lbmhip {
 constexpr int STEP_MACRO = 1;
 constexpr int STEP_WALLF = 2;
 constexpr int STEP_MASS = 4;
+// Kernel-template bit only (not a caller flag): the lattice has a body
+// force, so the FP32 collision carries the forcing terms.  Chosen by the
+// host from the force, so lattices without one run no force arithmetic and
+// no branch on it.
+constexpr int STEP_BODY_FORCE = 0x100;
 
 constexpr int BLOCK = 256;
 
@@ -123,6 +128,7 @@ struct FastCoef {
     float gPw9[3], gPw3[3];           // 9 gP w (axis/diag/rest), 3 gP w
     float sm_axis, sm_diag;           // 3 s gM w
     float Fx, Fy, Fz;
+    float one;                        // 1.0f (see Store<_Float16>::load_add)
 };
 
 template <bool FORCE>
@@ -413,26 +419,25 @@ __device__ __forceinline__ void store_stream(S* p, S v) {
     else __builtin_nontemporal_store(v, p);
 }
 
-// Mass shift, collision and macroscopic outputs of one node whose 19
-// populations are in g (all update paths share it); returns the excess mass
-// sum_i f*_i - 1 when MASS.
+// Moments of a collided node (real units) and its excess mass sum_i f*_i - 1.
+template <class T>
+struct NodeOut {
+    T rho, ux, uy, uz;
+    double excess;
+};
+
+// Collision of one node whose 19 populations (mass shift included) are in g
+// (all update paths share it); the excess mass only when MASS, the velocity
+// only when MACRO.  The FP32 collision carries the body-force terms only
+// when MODE has STEP_BODY_FORCE (the FP64 one always does).
 template <class S, int MODE>
-__device__ __forceinline__ double collide_node(const StepArgs<S>& A, const FastCoef& K, typename Store<S>::C (&g)[Q],
-                                               uint32_t n, int x, int y, int z) {
-    using St = Store<S>;
-    using T = typename St::C;
+__device__ __forceinline__ NodeOut<typename Store<S>::C> collide_node(const StepArgs<S>& A, const FastCoef& K,
+                                                                      typename Store<S>::C (&g)[Q]) {
+    using T = typename Store<S>::C;
     constexpr bool EXACT = std::is_same<S, double>::value;
     constexpr bool MACRO = (MODE & STEP_MACRO) != 0;
     constexpr bool MASS = (MODE & STEP_MASS) != 0;
-    // pending mass-correction shift (lbm.cpp:337), applied on load
-    const double dsh = A.st->shift[A.t & 1];
-    if (dsh != 0.0) {
-        static_for<0, Q>([&](auto I) {
-            constexpr int i = decltype(I)::value;
-            if constexpr (EXACT) g[i] = g[i] + W[i] * dsh;
-            else g[i] += (T)(W[i] * dsh * St::scale);
-        });
-    }
+    constexpr bool FORCE = (MODE & STEP_BODY_FORCE) != 0;
 
     // ---- collide ----
     double excess = 0.0;
@@ -448,8 +453,7 @@ __device__ __forceinline__ double collide_node(const StepArgs<S>& A, const FastC
         }
     } else {
         float rho, jx, jy, jz, drs;
-        if (A.Fx != 0.0 || A.Fy != 0.0 || A.Fz != 0.0) collide_fast<true>(g, K, rho, jx, jy, jz, drs);
-        else collide_fast<false>(g, K, rho, jx, jy, jz, drs);
+        collide_fast<FORCE>(g, K, rho, jx, jy, jz, drs);
         rho_t = rho;
         if constexpr (MACRO) {
             const float isr = K.inv_s / rho;
@@ -459,15 +463,25 @@ __device__ __forceinline__ double collide_node(const StepArgs<S>& A, const FastC
         }
         if constexpr (MASS) excess = (double)drs * (double)K.inv_s;
     }
+    return NodeOut<T>{rho_t, m0, m1, m2, excess};
+}
 
-    // ---- macroscopic fields: all fluid nodes on request, else the band
-    // around the squirmer that the remap and the link kernel read
-    // (lbm.cpp:152, 198, 307)
+// Macroscopic fields of a collided node: all fluid nodes on request, else
+// the band around the squirmer that the remap and the link kernel read
+// (lbm.cpp:152, 198, 307).  Written after the node's populations, so the
+// stores of the populations sit in the block of the collision (scalar row
+// bases with 32-bit lane offsets).
+template <class S, int MODE>
+__device__ __forceinline__ void write_moments(const StepArgs<S>& A, const NodeOut<typename Store<S>::C>& o, uint32_t n,
+                                              int x, int y, int z) {
+    using T = typename Store<S>::C;
+    constexpr bool MACRO = (MODE & STEP_MACRO) != 0;
+    const T rho_t = o.rho;
     if constexpr (MACRO) {
         static_cast<T*>(A.rho)[n] = rho_t;
-        static_cast<T*>(A.ux)[n] = m0;
-        static_cast<T*>(A.uy)[n] = m1;
-        static_cast<T*>(A.uz)[n] = m2;
+        static_cast<T*>(A.ux)[n] = o.ux;
+        static_cast<T*>(A.uy)[n] = o.uy;
+        static_cast<T*>(A.uz)[n] = o.uz;
     } else {
         if (A.has_squirmer) {
             const int* b = A.st->band;
@@ -477,8 +491,6 @@ __device__ __forceinline__ double collide_node(const StepArgs<S>& A, const FastC
             if (dx < b[3] && dy < b[4] && dz < b[5]) static_cast<T*>(A.rho)[n] = rho_t;
         }
     }
-
-    return excess;
 }
 
 // Update one fluid node; returns its excess mass sum_i f*_i - 1 when MASS.
@@ -499,14 +511,30 @@ __device__ __forceinline__ double node_update(const StepArgs<S>& A, const FastCo
     const uint32_t* yo = rg.pyo;  // population rows
     const uint32_t* zo = rg.pzo;
 
+    // Pending mass-correction shift (lbm.cpp:337): W_i dsh is added to every
+    // population as it is loaded (zero on all steps but the one after a mass
+    // correction).  Adding it unconditionally keeps a node's loads and
+    // stores in one basic block, where the compiler addresses both with the
+    // scalar slot-row bases and 32-bit lane offsets; a branch between them
+    // costs 64-bit vector addresses for the stores and a third more vector
+    // registers.  FP16 converts and adds in one fused instruction.
+    const double dsh = A.st->shift[A.t & 1];
+    T sh[Q];
+    static_for<0, Q>([&](auto I) {
+        constexpr int i = decltype(I)::value;
+        if constexpr (St::shifted) sh[i] = (T)(W[i] * dsh * St::scale);
+        else sh[i] = W[i] * dsh;
+    });
+
     // ---- gather the 19 populations node n collides ----
     T g[Q];
     static_for<0, Q>([&](auto I) {
         constexpr int i = decltype(I)::value;
-        if constexpr (!ODD && !BPATH) g[i] = St::load(load_stream(at(base(i, yo[1] + zo[1]), xb[1])));
-        else if constexpr (!ODD) g[i] = St::load(*at(base(i, yo[1] + zo[1]), xb[1]));
-        else if constexpr (CX[i] == 0 && ODD_NT) g[i] = St::load(load_stream(at(base(OPP[i], yo[1 - CY[i]] + zo[1 - CZ[i]]), xb[1])));
-        else g[i] = St::load(*at(base(OPP[i], yo[1 - CY[i]] + zo[1 - CZ[i]]), xb[1 - CX[i]]));
+        if constexpr (!ODD && !BPATH) g[i] = St::load_add(load_stream(at(base(i, yo[1] + zo[1]), xb[1])), sh[i], K.one);
+        else if constexpr (!ODD) g[i] = St::load_add(*at(base(i, yo[1] + zo[1]), xb[1]), sh[i], K.one);
+        else if constexpr (CX[i] == 0 && ODD_NT)
+            g[i] = St::load_add(load_stream(at(base(OPP[i], yo[1 - CY[i]] + zo[1 - CZ[i]]), xb[1])), sh[i], K.one);
+        else g[i] = St::load_add(*at(base(OPP[i], yo[1 - CY[i]] + zo[1 - CZ[i]]), xb[1 - CX[i]]), sh[i], K.one);
     });
     // even step, boundary node: populations whose upstream node is solid
     // come back from that node's slot opp(j) (bounce-back)
@@ -517,7 +545,7 @@ __device__ __forceinline__ double node_update(const StepArgs<S>& A, const FastCo
             static_for<1, Q>([&](auto I) {
                 constexpr int j = decltype(I)::value;
                 if ((solid >> OPP[j]) & 1u)
-                    g[j] = St::load(*at(base(OPP[j], yo[1 - CY[j]] + zo[1 - CZ[j]]), xb[1 - CX[j]]));
+                    g[j] = St::load_add(*at(base(OPP[j], yo[1 - CY[j]] + zo[1 - CZ[j]]), xb[1 - CX[j]]), sh[j], K.one);
             });
         } else if (fl & BOUNDARY) {
             static_for<1, Q>([&](auto I) {
@@ -528,12 +556,12 @@ __device__ __forceinline__ double node_update(const StepArgs<S>& A, const FastCo
             static_for<1, Q>([&](auto I) {
                 constexpr int j = decltype(I)::value;
                 if ((solid >> OPP[j]) & 1u)
-                    g[j] = St::load(*at(base(OPP[j], yo[1 - CY[j]] + zo[1 - CZ[j]]), xb[1 - CX[j]]));
+                    g[j] = St::load_add(*at(base(OPP[j], yo[1 - CY[j]] + zo[1 - CZ[j]]), xb[1 - CX[j]]), sh[j], K.one);
             });
         }
     }
 
-    const double excess = collide_node<S, MODE>(A, K, g, n, x, y, z);
+    const NodeOut<T> out = collide_node<S, MODE>(A, K, g);
 
     // ---- stream ----
     static_for<0, Q>([&](auto I) {
@@ -553,7 +581,8 @@ __device__ __forceinline__ double node_update(const StepArgs<S>& A, const FastCo
             });
         }
     }
-    return excess;
+    write_moments<S, MODE>(A, out, n, x, y, z);
+    return out.excess;
 }
 
 // The hardware deals blocks to the 8 XCDs round robin; this gives each XCD
@@ -563,15 +592,16 @@ __device__ __forceinline__ int xcd_tile(int lin, int G) {
     return xcd * q8 + (xcd < r8 ? xcd : r8) + (lin >> 3);
 }
 
-// The tiles of one tile row (gx tiles across x) on one XCD, consecutive tile
-// rows dealt to the XCDs round robin: the lines a row's tiles share at
-// their x edges (the odd step's shifted accesses) stay in one L2.  The
-// rows past the last multiple of 8 keep the identity order.
-__device__ __forceinline__ int xcd_tile_rows(int lin, int G, int gx) {
-    const int rows8 = (G / gx) & ~7;
+// The tiles of one tile row (gx tiles across x; `rows` tile rows) on one
+// XCD, consecutive tile rows dealt to the XCDs round robin: the lines a
+// row's tiles share at their x edges (the odd step's shifted accesses) stay
+// in one L2.  The rows past the last multiple of 8 keep the identity order.
+__device__ __forceinline__ int xcd_tile_rows(int lin, int rows, int gx, const FastDiv& div_gx) {
+    const int rows8 = rows & ~7;
     if (lin >= rows8 * gx) return lin;
-    const int xcd = lin & 7, k = lin >> 3;
-    return ((k / gx) * 8 + xcd) * gx + k % gx;
+    const uint32_t xcd = (uint32_t)lin & 7u, k = (uint32_t)lin >> 3;
+    const uint32_t kr = quot(k, div_gx);
+    return (int)(((kr * 8u + xcd) * (uint32_t)gx) + (k - kr * (uint32_t)gx));
 }
 
 // One tile (one block of BLOCK threads) of the collide-stream update: either
@@ -584,15 +614,16 @@ __device__ __forceinline__ void step_tile(const StepArgs<S>& A, const FastCoef& 
     constexpr bool MASS = (MODE & STEP_MASS) != 0;
     int x, y, z;
     if (A.tx_shift == 0) {
-        const int chunk = __builtin_amdgcn_readfirstlane(tile * (BLOCK / 64) + (int)(threadIdx.x >> 6));
-        const int row = chunk / A.nchx;
-        x = (chunk - row * A.nchx) * 64 + (int)(threadIdx.x & 63);
-        y = row % A.NY;
-        z = row / A.NY;
+        const uint32_t chunk = __builtin_amdgcn_readfirstlane(tile * (BLOCK / 64) + (int)(threadIdx.x >> 6));
+        const uint32_t row = quot(chunk, A.div_nchx);
+        x = (int)(chunk - row * (uint32_t)A.nchx) * 64 + (int)(threadIdx.x & 63);
+        z = (int)quot(row, A.div_ny);
+        y = (int)row - z * A.NY;
         if (z >= A.NZ) { y = -1; z = 0; }  // past the last row of the lattice
     } else {
-        const int tgx = tile % A.gx, trest = tile / A.gx;
-        const int tgy = trest % A.gy, tgz = trest / A.gy;
+        const uint32_t trest = quot((uint32_t)tile, A.div_gx);
+        const uint32_t tgz_u = quot(trest, A.div_gy);
+        const int tgx = tile - (int)trest * A.gx, tgy = (int)(trest - tgz_u * (uint32_t)A.gy), tgz = (int)tgz_u;
         const int lx = threadIdx.x & ((1 << A.tx_shift) - 1);
         const int ly = __builtin_amdgcn_readfirstlane(threadIdx.x >> A.tx_shift);  // one row per wavefront
         x = (tgx << A.tx_shift) + lx; y = tgy * A.ty + ly; z = tgz;
@@ -624,7 +655,8 @@ void aa_step_kernel(const StepArgs<S> A, const FastCoef K) {
     const int G = A.gx * A.gy * A.gz;
     const int lin = (int)blockIdx.x;
     step_tile<S, ODD, MODE>(A, K,
-                            A.xcd_remap == 1 ? xcd_tile(lin, G) : (A.xcd_remap == 2 ? xcd_tile_rows(lin, G, A.gx) : lin));
+                            A.xcd_remap == 1 ? xcd_tile(lin, G)
+                                             : (A.xcd_remap == 2 ? xcd_tile_rows(lin, A.gy * A.gz, A.gx, A.div_gx) : lin));
 }
 
 // Boundary chunks of an even step.  Work list: every 64-node chunk of the

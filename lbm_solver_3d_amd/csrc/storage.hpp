@@ -30,6 +30,7 @@ struct Store<double> {
     static constexpr bool shifted = false;
     static constexpr double scale = 1.0;
     LBM_HD static inline C load(double v) { return v; }
+    LBM_HD static inline C load_add(double v, C c, float) { return v + c; }
     LBM_HD static inline double store(C v) { return v; }
 };
 
@@ -39,6 +40,7 @@ struct Store<float> {
     static constexpr bool shifted = true;
     static constexpr double scale = 1.0;
     LBM_HD static inline C load(float v) { return v; }
+    LBM_HD static inline C load_add(float v, C c, float) { return v + c; }
     LBM_HD static inline float store(C v) { return v; }
 };
 
@@ -48,6 +50,10 @@ struct Store<_Float16> {
     static constexpr bool shifted = true;
     static constexpr double scale = 32768.0;
     LBM_HD static inline C load(_Float16 v) { return (float)v; }
+    // (float)v + c, rounded once.  `one` is 1.0f held in a register: with a
+    // literal 1 the compiler folds the multiply away and emits a conversion
+    // and an add; this way it emits one mixed-precision fused multiply-add.
+    LBM_HD static inline C load_add(_Float16 v, C c, float one) { return __builtin_fmaf((float)v, one, c); }
     LBM_HD static inline _Float16 store(C v) { return (_Float16)v; }
 };
 
@@ -60,6 +66,7 @@ struct Store<bf16_t> {
         const uint32_t u = (uint32_t)v.bits << 16;
         return __builtin_bit_cast(float, u);
     }
+    LBM_HD static inline C load_add(bf16_t v, C c, float) { return load(v) + c; }
     LBM_HD static inline bf16_t store(C v) {
         uint32_t u = __builtin_bit_cast(uint32_t, v);
         u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest, ties to even (finite inputs)

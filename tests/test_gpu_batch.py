@@ -203,3 +203,44 @@ def test_c4_ensemble_full_size(lbm):
         with lbm.Lattice(**members[i], precision="fp16") as L:
             L.run(steps, 10, lbm.STEP_MACRO)
             _same(got[i], _state(L))
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp16"])
+def test_occupancy_caps_leave_results_unchanged(lbm, prec):
+    """The occupancy cap only reserves LDS per block: every cap gives the
+    same state bit for bit, through eager steps and graph replay."""
+    kv = dict(MEMBERS[0], precision=prec)
+    defaults = {"fp64": (3, 3), "fp16": (0, 6)}[prec]
+    states = []
+    for caps in (None, (0, 0), (2, 2), (8, 3)):
+        with lbm.Lattice(**kv) as L:
+            if caps is None:
+                info = L.info()
+                assert (info.occupancy_even, info.occupancy_odd) == defaults
+            else:
+                L.set_occupancy(*caps)
+                info = L.info()
+                assert (info.occupancy_even, info.occupancy_odd) == caps
+            L.run(33, 10, lbm.STEP_ALL)
+            states.append(_state(L))
+    for s in states[1:]:
+        _same(states[0], s)
+
+
+def test_occupancy_cap_rejects_out_of_range(lbm):
+    with lbm.Lattice(nx=8, ny=8, nz=8, precision="fp32") as L:
+        for caps in ((1, 0), (0, 9), (-1, 4)):
+            with pytest.raises(lbm.LbmHipError):
+                L.set_occupancy(*caps)
+        info = L.info()
+        assert (info.occupancy_even, info.occupancy_odd) == (4, 4)
+
+
+def test_batch_occupancy_cap(lbm):
+    with lbm.Batch(MEMBERS, precision="fp16") as A, lbm.Batch(MEMBERS, precision="fp16") as B:
+        B.set_occupancy(3, 2)
+        A.run(25, 10, 0)
+        B.run(25, 10, 0)
+        for i in range(len(MEMBERS)):
+            assert B.member(i).info().occupancy_odd == 2
+            _same(_state(A.member(i)), _state(B.member(i)))
