@@ -240,8 +240,8 @@ int lbm_hip_set_tiling(lbm_hip_ctx* ctx, int tile_width, int xcd_remap);
  * allow 8).  No counterpart in the reference.  The in-place update is bound
  * by HBM, and fewer wavefronts in flight can move its bytes faster; the cap
  * is set by reserving LDS per block, so results do not change.  Defaults,
- * measured on the 512 x 256 x 128 channel: FP64 3 / 3, FP32 4 / 4, FP16 and
- * BF16 0 / 6. */
+ * measured on the 512 x 256 x 128 channel: FP64 3 / 3, FP32 4 / 4, FP16
+ * 0 / 6, BF16 0 / 7. */
 int lbm_hip_set_occupancy(lbm_hip_ctx* ctx, int even_blocks, int odd_blocks);
 
 /* ---- ensemble of lattices ----

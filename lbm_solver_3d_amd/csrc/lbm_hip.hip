@@ -397,11 +397,12 @@ int occupancy_lds(int device, int blocks, unsigned* bytes) {
 
 // Measured on the 512 x 256 x 128 channel: the FP32 and FP64 updates move
 // their bytes fastest with 4 and 3 blocks per CU on both parities, FP16 and
-// BF16 with the even step uncapped and the odd step at 6.
+// BF16 with the even step uncapped and the odd step at 6 and 7.
 void default_occupancy(int prec, int cap[2]) {
     switch (prec) {
         case LBM_HIP_FP64: cap[0] = cap[1] = 3; break;
         case LBM_HIP_FP32: cap[0] = cap[1] = 4; break;
+        case LBM_HIP_BF16: cap[0] = 0; cap[1] = 7; break;
         default: cap[0] = 0; cap[1] = 6; break;
     }
 }

@@ -497,10 +497,13 @@ __device__ __forceinline__ void write_moments(const StepArgs<S>& A, const NodeOu
 // BPATH: even-step node of a boundary chunk (bounce-back redirections).
 // `row_solid` (BPATH): the solid directions, the same for every node of the
 // wavefront's row chunk (a row next to a wall, away from the squirmer), or
-// -1 to read them from the neighbours' flags.
+// -1 to read them from the neighbours' flags.  `dsh`: the pending
+// mass-correction shift, read by the caller at the start of the kernel (its
+// scalar load then overlaps the index arithmetic instead of delaying the
+// node's loads).
 template <class S, bool ODD, int MODE, bool BPATH>
 __device__ __forceinline__ double node_update(const StepArgs<S>& A, const FastCoef& K, const RowGeom& rg, int x,
-                                              int y, int z, uint8_t fl, int row_solid = -1) {
+                                              int y, int z, uint8_t fl, double dsh, int row_solid = -1) {
     using St = Store<S>;
     using T = typename St::C;
     const uint32_t SS = A.SS, n = rg.n;
@@ -518,7 +521,6 @@ __device__ __forceinline__ double node_update(const StepArgs<S>& A, const FastCo
     // scalar slot-row bases and 32-bit lane offsets; a branch between them
     // costs 64-bit vector addresses for the stores and a third more vector
     // registers.  FP16 converts and adds in one fused instruction.
-    const double dsh = A.st->shift[A.t & 1];
     T sh[Q];
     static_for<0, Q>([&](auto I) {
         constexpr int i = decltype(I)::value;
@@ -526,15 +528,14 @@ __device__ __forceinline__ double node_update(const StepArgs<S>& A, const FastCo
         else sh[i] = W[i] * dsh;
     });
 
-    // ---- gather the 19 populations node n collides ----
-    T g[Q];
+    // ---- gather the 19 populations node n collides (stored form) ----
+    S gs[Q];
     static_for<0, Q>([&](auto I) {
         constexpr int i = decltype(I)::value;
-        if constexpr (!ODD && !BPATH) g[i] = St::load_add(load_stream(at(base(i, yo[1] + zo[1]), xb[1])), sh[i], K.one);
-        else if constexpr (!ODD) g[i] = St::load_add(*at(base(i, yo[1] + zo[1]), xb[1]), sh[i], K.one);
-        else if constexpr (CX[i] == 0 && ODD_NT)
-            g[i] = St::load_add(load_stream(at(base(OPP[i], yo[1 - CY[i]] + zo[1 - CZ[i]]), xb[1])), sh[i], K.one);
-        else g[i] = St::load_add(*at(base(OPP[i], yo[1 - CY[i]] + zo[1 - CZ[i]]), xb[1 - CX[i]]), sh[i], K.one);
+        if constexpr (!ODD && !BPATH) gs[i] = load_stream(at(base(i, yo[1] + zo[1]), xb[1]));
+        else if constexpr (!ODD) gs[i] = *at(base(i, yo[1] + zo[1]), xb[1]);
+        else if constexpr (CX[i] == 0 && ODD_NT) gs[i] = load_stream(at(base(OPP[i], yo[1 - CY[i]] + zo[1 - CZ[i]]), xb[1]));
+        else gs[i] = *at(base(OPP[i], yo[1 - CY[i]] + zo[1 - CZ[i]]), xb[1 - CX[i]]);
     });
     // even step, boundary node: populations whose upstream node is solid
     // come back from that node's slot opp(j) (bounce-back)
@@ -544,8 +545,7 @@ __device__ __forceinline__ double node_update(const StepArgs<S>& A, const FastCo
             solid = (uint32_t)row_solid;
             static_for<1, Q>([&](auto I) {
                 constexpr int j = decltype(I)::value;
-                if ((solid >> OPP[j]) & 1u)
-                    g[j] = St::load_add(*at(base(OPP[j], yo[1 - CY[j]] + zo[1 - CZ[j]]), xb[1 - CX[j]]), sh[j], K.one);
+                if ((solid >> OPP[j]) & 1u) gs[j] = *at(base(OPP[j], yo[1 - CY[j]] + zo[1 - CZ[j]]), xb[1 - CX[j]]);
             });
         } else if (fl & BOUNDARY) {
             static_for<1, Q>([&](auto I) {
@@ -555,11 +555,17 @@ __device__ __forceinline__ double node_update(const StepArgs<S>& A, const FastCo
             });
             static_for<1, Q>([&](auto I) {
                 constexpr int j = decltype(I)::value;
-                if ((solid >> OPP[j]) & 1u)
-                    g[j] = St::load_add(*at(base(OPP[j], yo[1 - CY[j]] + zo[1 - CZ[j]]), xb[1 - CX[j]]), sh[j], K.one);
+                if ((solid >> OPP[j]) & 1u) gs[j] = *at(base(OPP[j], yo[1 - CY[j]] + zo[1 - CZ[j]]), xb[1 - CX[j]]);
             });
         }
     }
+    // to the collision's form, after the (possibly divergent) redirections,
+    // so no redirected load waits inside its branch
+    T g[Q];
+    static_for<0, Q>([&](auto I) {
+        constexpr int i = decltype(I)::value;
+        g[i] = St::load_add(gs[i], sh[i], K.one);
+    });
 
     const NodeOut<T> out = collide_node<S, MODE>(A, K, g);
 
@@ -612,6 +618,7 @@ __device__ __forceinline__ int xcd_tile_rows(int lin, int rows, int gx, const Fa
 template <class S, bool ODD, int MODE>
 __device__ __forceinline__ void step_tile(const StepArgs<S>& A, const FastCoef& K, int tile) {
     constexpr bool MASS = (MODE & STEP_MASS) != 0;
+    const double dsh = A.st->shift[A.t & 1];  // pending mass-correction shift (lbm.cpp:337)
     int x, y, z;
     if (A.tx_shift == 0) {
         const uint32_t chunk = __builtin_amdgcn_readfirstlane(tile * (BLOCK / 64) + (int)(threadIdx.x >> 6));
@@ -644,7 +651,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<S>& A, const FastCoef& 
     if (tile == 0 && threadIdx.x == 0) A.st->shift[(A.t + 1) & 1] = 0.0;
 
     double excess = 0.0;
-    if (active) excess = node_update<S, ODD, MODE, false>(A, K, rg, x, y, z, fl);
+    if (active) excess = node_update<S, ODD, MODE, false>(A, K, rg, x, y, z, fl, dsh);
 
     if constexpr (MASS) block_mass_sum(excess, &A.mass_part[tile]);
 }
@@ -675,6 +682,7 @@ template <class S, int MODE>
 __device__ __forceinline__ void boundary_block(const StepArgs<S>& A, const FastCoef& K, const BoundaryArgs& B,
                                                int block) {
     constexpr bool MASS = (MODE & STEP_MASS) != 0;
+    const double dsh = A.st->shift[A.t & 1];  // pending mass-correction shift (lbm.cpp:337)
     const uint32_t NX = (uint32_t)A.NX, NY = (uint32_t)A.NY, NZ = (uint32_t)A.NZ;
     const int wave = block * (BLOCK / 64) + (int)(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
@@ -724,7 +732,7 @@ __device__ __forceinline__ void boundary_block(const StepArgs<S>& A, const FastC
     bool active = (fl & TYPE_MASK) == FLUID;
     if (__builtin_amdgcn_ballot_w64(active && (fl & BOUNDARY)) == 0) active = false;  // not a boundary chunk
     double excess = 0.0;
-    if (active) excess = node_update<S, false, MODE, true>(A, K, rg, x, y, z, fl, row_solid);
+    if (active) excess = node_update<S, false, MODE, true>(A, K, rg, x, y, z, fl, dsh, row_solid);
     if constexpr (MASS) block_mass_sum(excess, &B.mass_part[block]);
 }
 
