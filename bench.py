@@ -158,15 +158,17 @@ def measure_ensemble(lbm, steps: int, warmup: int, count: int = 64, sequential: 
 
 def sweep_ensemble_tiling(lbm, steps: int, count: int = 64) -> dict:
     """Every tile shape the batch tuner tries, `steps` graph-replayed steps
-    each, in the tuner's order (the candidates come from one short tuner
-    pass that runs first): per-shape step times, and under rocprofv3 per
-    dispatch counters that follow the same order."""
+    each after two untimed mass-correction cycles, in the tuner's order (the
+    candidates come from one short tuner pass that runs first): per-shape
+    step times, and under rocprofv3 per dispatch counters that follow the
+    same order."""
     members = lbm.ensemble_members(count, 0)
     rows = []
     with lbm.Batch(members, precision="fp16") as B:
         cands = B.autotune(trial_steps=2, mass_every=MASS_EVERY)["candidates"]
         for c in cands:
             B.set_tiling(c["tile_width"], c["xcd_remap"])
+            B.time_run(2 * MASS_EVERY, MASS_EVERY)  # untimed: the new shape's graph capture and first cycles
             ms = B.time_run(steps, MASS_EVERY)
             rows.append(dict(tile_width=c["tile_width"], xcd_remap=c["xcd_remap"], ms_per_step=round(ms / steps, 4)))
             print(f"[sweep] tile width {c['tile_width']}, XCD order {c['xcd_remap']}: {ms / steps:.3f} ms per step",

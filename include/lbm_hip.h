@@ -102,8 +102,10 @@ typedef struct lbm_hip_info {
     int row_pitch;          /* nodes per stored population row (nx, or nx rounded up to 64) */
     int tile_width;         /* collide-stream tile width in x (64 / 128 / 256; 0: row runs) */
     int xcd_remap;          /* tile order over the XCDs (lbm_hip_set_tiling) */
-    int occupancy_even;     /* collide-stream blocks per CU on even / odd steps (lbm_hip_set_occupancy) */
+    int occupancy_even;     /* cap on collide-stream blocks per CU, even / odd steps (lbm_hip_set_occupancy) */
     int occupancy_odd;
+    int resident_even;      /* blocks per CU the launches allow (HIP's occupancy calculator) */
+    int resident_odd;
 } lbm_hip_info;
 
 typedef struct lbm_hip_ctx lbm_hip_ctx;
@@ -284,7 +286,9 @@ int lbm_hip_batch_autotune(lbm_hip_batch* b, int trial_steps, int mass_every, lb
 /* One tile shape for all members, as lbm_hip_set_tiling. */
 int lbm_hip_batch_set_tiling(lbm_hip_batch* b, int tile_width, int xcd_remap);
 
-/* One occupancy cap for all members, as lbm_hip_set_occupancy. */
+/* One occupancy cap for all members, as lbm_hip_set_occupancy.  A new
+ * batch uses 6 / 6 for FP16 and BF16 storage (measured on the 64-run
+ * ensemble), the single-lattice defaults otherwise. */
 int lbm_hip_batch_set_occupancy(lbm_hip_batch* b, int even_blocks, int odd_blocks);
 /* Bench support: time `steps` graph-replayed steps with HIP events. */
 int lbm_hip_batch_time_run(lbm_hip_batch* b, int steps, int mass_every, double* total_ms);

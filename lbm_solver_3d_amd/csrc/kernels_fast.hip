@@ -5,8 +5,11 @@
 
 namespace lbmhip {
 template hipError_t launch_step<float>(const StepArgs<float>&, const FastCoef&, bool, int, unsigned, hipStream_t);
+template hipError_t step_blocks_per_cu<float>(bool, unsigned, int*);
 template hipError_t launch_step<_Float16>(const StepArgs<_Float16>&, const FastCoef&, bool, int, unsigned, hipStream_t);
+template hipError_t step_blocks_per_cu<_Float16>(bool, unsigned, int*);
 template hipError_t launch_step<bf16_t>(const StepArgs<bf16_t>&, const FastCoef&, bool, int, unsigned, hipStream_t);
+template hipError_t step_blocks_per_cu<bf16_t>(bool, unsigned, int*);
 template hipError_t launch_boundary<float>(const StepArgs<float>&, const FastCoef&, const BoundaryArgs&, int, int,
                                             hipStream_t);
 template hipError_t launch_boundary<_Float16>(const StepArgs<_Float16>&, const FastCoef&, const BoundaryArgs&, int, int,

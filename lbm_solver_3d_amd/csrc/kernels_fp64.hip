@@ -121,6 +121,7 @@ LBMHIP_INSTANTIATE(bf16_t)
 #undef LBMHIP_INSTANTIATE
 
 template hipError_t launch_step<double>(const StepArgs<double>&, const FastCoef&, bool, int, unsigned, hipStream_t);
+template hipError_t step_blocks_per_cu<double>(bool, unsigned, int*);
 template hipError_t launch_boundary<double>(const StepArgs<double>&, const FastCoef&, const BoundaryArgs&, int, int,
                                             hipStream_t);
 template hipError_t launch_batch_step<double>(const BatchMember<double>*, const uint32_t*, int, long long, int, bool,

@@ -26,6 +26,12 @@ template <class S>
 hipError_t launch_boundary(const StepArgs<S>& a, const FastCoef& k, const BoundaryArgs& b, int blocks, int mode,
                            hipStream_t s);
 
+// Blocks of the collide-stream kernel (no moments, no mass partials) that
+// can reside on one CU with `lds` dynamic LDS bytes per block, as HIP's
+// occupancy calculator reports it.
+template <class S>
+hipError_t step_blocks_per_cu(bool odd, unsigned lds, int* blocks);
+
 // The coupling of a step: the link pass over the link list (`links`;
 // `link_blocks` blocks of LINK_THREADS threads), then one block that adds
 // its partial sums and runs the rigid-body update and remap (`remap`).
@@ -95,6 +101,7 @@ hipError_t launch_copy_parts(const void* const* a, void* const* b, int parts, si
                                               hipStream_t);                                                     \
     extern template hipError_t launch_boundary<S>(const StepArgs<S>&, const FastCoef&, const BoundaryArgs&, int, \
                                                   int, hipStream_t);                                             \
+    extern template hipError_t step_blocks_per_cu<S>(bool, unsigned, int*);                                     \
     extern template hipError_t launch_couple<S>(const LinkArgs<S>&, const RemapArgs<S>&, int, int, int,         \
                                                 hipStream_t);                                                   \
     extern template hipError_t launch_wall_force<S>(const S*, const uint8_t*, const WallForceArgs&, double*,      \

@@ -49,6 +49,12 @@ hipError_t launch_step(const StepArgs<S>& a, const FastCoef& k, bool odd, int mo
     return odd ? launch_step_p<S, true>(a, k, mode, lds, s) : launch_step_p<S, false>(a, k, mode, lds, s);
 }
 
+template <class S>
+hipError_t step_blocks_per_cu(bool odd, unsigned lds, int* blocks) {
+    return odd ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, aa_step_kernel<S, true, 0>, BLOCK, lds)
+               : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, aa_step_kernel<S, false, 0>, BLOCK, lds);
+}
+
 template <class S, int MODE>
 static hipError_t launch_boundary_v(const StepArgs<S>& a, const FastCoef& k, const BoundaryArgs& b, int blocks,
                                     hipStream_t s) {

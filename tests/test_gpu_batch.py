@@ -212,15 +212,21 @@ def test_occupancy_caps_leave_results_unchanged(lbm, prec):
     kv = dict(MEMBERS[0], precision=prec)
     defaults = {"fp64": (3, 3), "fp16": (0, 6)}[prec]
     states = []
-    for caps in (None, (0, 0), (2, 2), (8, 3)):
+    free = None  # blocks per CU the registers allow (no cap)
+    for caps in ((0, 0), None, (2, 2), (8, 3)):
         with lbm.Lattice(**kv) as L:
             if caps is None:
-                info = L.info()
-                assert (info.occupancy_even, info.occupancy_odd) == defaults
+                caps = defaults
             else:
                 L.set_occupancy(*caps)
-                info = L.info()
-                assert (info.occupancy_even, info.occupancy_odd) == caps
+            info = L.info()
+            assert (info.occupancy_even, info.occupancy_odd) == caps
+            if free is None:
+                free = (info.resident_even, info.resident_odd)
+                assert min(free) >= 1
+            # the LDS reservation holds each launch to its cap (HIP's occupancy calculator)
+            for cap, res, f in zip(caps, (info.resident_even, info.resident_odd), free):
+                assert res == (min(cap, f) if cap else f), (caps, res, free)
             L.run(33, 10, lbm.STEP_ALL)
             states.append(_state(L))
     for s in states[1:]:
@@ -242,5 +248,7 @@ def test_batch_occupancy_cap(lbm):
         A.run(25, 10, 0)
         B.run(25, 10, 0)
         for i in range(len(MEMBERS)):
-            assert B.member(i).info().occupancy_odd == 2
+            a, b = A.member(i).info(), B.member(i).info()
+            assert (a.occupancy_even, a.occupancy_odd) == (6, 6)  # the batch default for FP16
+            assert (b.occupancy_even, b.occupancy_odd) == (3, 2)
             _same(_state(A.member(i)), _state(B.member(i)))
