@@ -106,6 +106,7 @@ typedef struct lbm_hip_info {
     int occupancy_odd;
     int resident_even;      /* blocks per CU the launches allow (HIP's occupancy calculator) */
     int resident_odd;
+    int odd_row_waves;      /* odd steps: wavefronts per block of the one-block-per-row kernel (0: tiles) */
 } lbm_hip_info;
 
 typedef struct lbm_hip_ctx lbm_hip_ctx;

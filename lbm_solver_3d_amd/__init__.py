@@ -84,7 +84,7 @@ class Info(C.Structure):
         ("count_fluid", C.c_int64),
         ("row_pitch", C.c_int), ("tile_width", C.c_int), ("xcd_remap", C.c_int),
         ("occupancy_even", C.c_int), ("occupancy_odd", C.c_int),
-        ("resident_even", C.c_int), ("resident_odd", C.c_int),
+        ("resident_even", C.c_int), ("resident_odd", C.c_int), ("odd_row_waves", C.c_int),
     ]
 
 

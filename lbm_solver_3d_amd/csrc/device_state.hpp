@@ -115,6 +115,7 @@ struct StepArgs {
     int nchx;             // 64-node chunks per row
     int ty;               // tile height in y
     int xcd_remap;        // 1: give each XCD a contiguous range of tiles; 2: whole tile rows per XCD
+    int odd_rows;         // odd steps one block per row (aa_odd_row_kernel): 0 no, 1 rows in order, 2 row ranges per XCD
     FastDiv div_gx, div_gy, div_nchx, div_ny;  // division by gx, gy, nchx and NY (tile decoding)
     double omPlus, omMinus, Fx, Fy, Fz;
     long long t;          // step index (selects double-buffer slots)

@@ -34,6 +34,10 @@ static hipError_t with_kernel_mode(int mode, Fn&& fn) {
 
 template <class S, bool ODD, int MODE>
 static hipError_t launch_step_v(const StepArgs<S>& a, const FastCoef& k, unsigned lds, hipStream_t s) {
+    if (ODD && a.odd_rows) {  // one block per row; the host checked NX (multiple of 64, <= 1024, rows unpadded)
+        hipLaunchKernelGGL((aa_odd_row_kernel<S, MODE>), dim3(a.NY * a.NZ), dim3(a.NX), lds, s, a, k);
+        return hipGetLastError();
+    }
     const int G = a.gx * a.gy * a.gz;
     hipLaunchKernelGGL((aa_step_kernel<S, ODD, MODE>), dim3(G), dim3(BLOCK), lds, s, a, k);
     return hipGetLastError();

@@ -89,6 +89,11 @@ struct lbm_hip_ctx {
     // (0: none) and the dynamic LDS per block that enforces it
     int occ_cap[2] = {0, 0};
     unsigned occ_lds[2] = {0, 0};
+    // odd steps one block per row (aa_odd_row_kernel; 0: tiles, 1: rows in
+    // order, 2: row ranges per XCD) and the dynamic LDS that holds those
+    // blocks to the odd cap's wavefronts per CU
+    int odd_rows = 0;
+    unsigned occ_lds_rows = 0;
     int G_max = 0;  // largest tile count over the tile shapes (mass_part holds G_max + bblocks)
     double omPlus = 0, omMinus = 0, nu = 0;
     FastCoef K{};
@@ -97,6 +102,7 @@ struct lbm_hip_ctx {
     bool macro_valid = true;  // initial rho = 1, u = 0 are the reference's values
     bool wallf_valid = false;
     bool mass_valid = false;
+    int last_parts = 0;   // collide-stream mass partials written by the last step (tiles or rows)
     int last_bparts = 0;  // boundary-kernel mass partials written by the last step
     size_t device_bytes = 0;
     std::vector<hipEvent_t> events;
@@ -267,6 +273,7 @@ StepArgs<S> step_args(const lbm_hip_ctx* c) {
     a.div_gx = make_fast_div((uint32_t)a.gx); a.div_gy = make_fast_div((uint32_t)a.gy);
     a.div_nchx = make_fast_div((uint32_t)a.nchx); a.div_ny = make_fast_div((uint32_t)a.NY);
     a.xcd_remap = c->xcd_remap;
+    a.odd_rows = c->odd_rows;
     a.omPlus = c->omPlus; a.omMinus = c->omMinus;
     a.Fx = c->p.fx; a.Fy = c->p.fy; a.Fz = c->p.fz;
     a.t = c->t;
@@ -397,25 +404,45 @@ int occupancy_lds(int device, int blocks, unsigned* bytes) {
 
 // Measured on the 512 x 256 x 128 channel: the FP32 and FP64 updates move
 // their bytes fastest with 4 and 3 blocks per CU on both parities, FP16 and
-// BF16 with the even step uncapped and the odd step at 6 and 7.
-void default_occupancy(int prec, int cap[2]) {
+// BF16 with the even step uncapped and the odd step at 6 and 7 -- with the
+// odd step in tiles.  The one-block-per-row odd step is fastest uncapped
+// (FP16 225.5 us per step against 257.2 us at the 24 wavefronts per CU of
+// 6 tile blocks; FP32 alike within 0.5 %).
+void default_occupancy(int prec, bool odd_rows, int cap[2]) {
     switch (prec) {
         case LBM_HIP_FP64: cap[0] = cap[1] = 3; break;
         case LBM_HIP_FP32: cap[0] = cap[1] = 4; break;
         case LBM_HIP_BF16: cap[0] = 0; cap[1] = 7; break;
         default: cap[0] = 0; cap[1] = 6; break;
     }
+    if (odd_rows && prec != LBM_HIP_FP64) cap[1] = 0;
 }
 
 int set_occupancy(lbm_hip_ctx* c, int even_blocks, int odd_blocks) {
     for (int b : {even_blocks, odd_blocks})
         if (b != 0 && (b < 2 || b > 8)) return fail(LBM_HIP_EINVAL, "blocks per CU must be 0 (no cap) or 2..8");
-    unsigned lds[2];
+    unsigned lds[2], lds_rows = 0;
     if (int rc = occupancy_lds(c->device, even_blocks, &lds[0])) return rc;
     if (int rc = occupancy_lds(c->device, odd_blocks, &lds[1])) return rc;
+    if (c->odd_rows && odd_blocks) {
+        // the same wavefronts per CU in row blocks of NX / 64 wavefronts, where
+        // the LDS allocation (512-byte granules, ROW_LDS_WORDS static words per
+        // block) can express it; else no cap
+        const int row_waves = c->NX / 64, rb = odd_blocks * (BLOCK / 64) / row_waves;
+        int per_cu = 0, per_block = 0;
+        if (hipDeviceGetAttribute(&per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, c->device) != hipSuccess ||
+            hipDeviceGetAttribute(&per_block, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) != hipSuccess)
+            return fail(LBM_HIP_EDEVICE, "LDS size query failed");
+        const long stat = (long)ROW_LDS_WORDS * 8;  // a multiple of 512
+        if (rb >= 1 && rb * row_waves < 32) {
+            const long total = std::min((long)(per_cu / rb) / 512 * 512, (long)per_block);
+            if (total > stat && total * (rb + 1) > per_cu) lds_rows = (unsigned)(total - stat);
+        }
+    }
     if (c->stream && hipStreamSynchronize(c->stream) != hipSuccess) return fail(LBM_HIP_EDEVICE, "stream sync");
     c->occ_cap[0] = even_blocks; c->occ_cap[1] = odd_blocks;
     c->occ_lds[0] = lds[0]; c->occ_lds[1] = lds[1];
+    c->occ_lds_rows = lds_rows;
     drop_graphs(c);  // captured launches carry the old LDS size
     return LBM_HIP_OK;
 }
@@ -520,6 +547,15 @@ int lbm_hip_create(const lbm_hip_params* pin, lbm_hip_ctx** out) {
     c->N = (uint32_t)((size_t)pin->nx * pin->ny * pin->nz);
     c->ssize = storage_bytes(c->prec);
     c->PX = row_pitch(pin->nx, pin->ny, pin->nz);
+    // Odd steps one block per row where a row is whole wavefronts and fits a
+    // block, for 2-byte storage (measured on 512 x 256 x 128: FP16 odd step
+    // 199.5 against 210 us in tiles; FP32 and FP64 no faster).
+    // LBM_HIP_ODD_ROWS: 0 off, 1 rows in order, 2 row ranges per XCD, for
+    // any storage.
+    if (pin->nx % 64 == 0 && pin->nx <= 1024 && c->PX == (uint32_t)pin->nx) {
+        c->odd_rows = c->ssize == 2 ? 1 : 0;
+        if (const char* e = std::getenv("LBM_HIP_ODD_ROWS")) c->odd_rows = std::max(0, std::min(2, std::atoi(e)));
+    }
     c->SS = slot_stride(c->PX * (uint32_t)pin->ny * (uint32_t)pin->nz, c->ssize);
     c->msize = c->prec == LBM_HIP_FP64 ? 8 : 4;
     // lbm.cpp:67-69
@@ -577,7 +613,7 @@ int lbm_hip_create(const lbm_hip_params* pin, lbm_hip_ctx** out) {
         return cleanup(fail(LBM_HIP_EDEVICE, "hipStreamCreate failed"));
     {
         int cap[2];
-        default_occupancy(c->prec, cap);
+        default_occupancy(c->prec, c->odd_rows != 0, cap);
         if (int rc2 = set_occupancy(c, cap[0], cap[1])) return cleanup(rc2);
     }
 
@@ -587,7 +623,7 @@ int lbm_hip_create(const lbm_hip_params* pin, lbm_hip_ctx** out) {
         c->device_bytes += bytes;
         return true;
     };
-    const int nparts = std::max(c->G_max + c->bblocks, c->nb256);
+    const int nparts = std::max({c->G_max + c->bblocks, c->nb256, c->p.ny * c->p.nz});
     if (!dmalloc(&c->f, (size_t)Q * c->SS * c->ssize) || !dmalloc((void**)&c->flag, N) ||
         !dmalloc(&c->rho, N * c->msize) || !dmalloc(&c->ux, N * c->msize) || !dmalloc(&c->uy, N * c->msize) ||
         !dmalloc(&c->uz, N * c->msize) || !dmalloc((void**)&c->st, sizeof(DevState)) ||
@@ -704,7 +740,8 @@ int lbm_hip_step(lbm_hip_ctx* c, int flags) {
         using S = std::remove_pointer_t<decltype(tag)>;
         const StepArgs<S> sa = step_args<S>(c);
         if (c->kev[0]) e = hipEventRecord(c->kev[0], c->stream);
-        if (e == hipSuccess) e = launch_step<S>(sa, c->K, odd, mode, c->occ_lds[odd], c->stream);
+        const unsigned lds = odd && c->odd_rows ? c->occ_lds_rows : c->occ_lds[odd];
+        if (e == hipSuccess) e = launch_step<S>(sa, c->K, odd, mode, lds, c->stream);
         if (e == hipSuccess && !odd) e = launch_boundary<S>(sa, c->K, c->bargs, c->bblocks, mode, c->stream);
         if (e == hipSuccess && c->kev[1]) e = hipEventRecord(c->kev[1], c->stream);
         // the moving-boundary bounce-back and force/torque on the squirmer
@@ -717,6 +754,7 @@ int lbm_hip_step(lbm_hip_ctx* c, int flags) {
     });
     if (e != hipSuccess) return fail(LBM_HIP_EDEVICE, std::string("step launch: ") + hipGetErrorString(e));
     c->links_pending = c->has_squirmer;
+    c->last_parts = odd && c->odd_rows ? c->NY * c->NZ : c->G;
     c->last_bparts = odd ? 0 : c->bblocks;
     ++c->t;
     c->remaps_since_step = 0;
@@ -741,7 +779,7 @@ int lbm_hip_correct_mass(lbm_hip_ctx* c) {
     const int shift_slot = (int)(c->t & 1);
     if (c->mass_valid) {
         const int terms = MASS_TERM_LINKS | (c->remaps_since_step > 0 ? MASS_TERM_REMAP : 0);
-        e = launch_mass_finalize(c->mass_part, c->G + c->last_bparts, c->st, terms, shift_slot, c->stream);
+        e = launch_mass_finalize(c->mass_part, c->last_parts + c->last_bparts, c->st, terms, shift_slot, c->stream);
     } else {
         const int shifted = layout_shifted(c) ? 1 : 0;
         e = hipSuccess;
@@ -1051,6 +1089,7 @@ int lbm_hip_get_info(lbm_hip_ctx* c, lbm_hip_info* o) {
     o->xcd_remap = c->xcd_remap;
     o->occupancy_even = c->occ_cap[0];
     o->occupancy_odd = c->occ_cap[1];
+    o->odd_row_waves = c->odd_rows ? c->NX / 64 : 0;
     hipError_t e = hipSuccess;
     dispatch(c->prec, [&](auto* tag) -> int {
         using S = std::remove_pointer_t<decltype(tag)>;
@@ -1156,7 +1195,7 @@ struct Snapshot {
     size_t bytes[SNAP_PARTS] = {0, 0, 0, 0, 0, 0, 0, 0};
     long long t = 0;
     bool macro_valid = false, wallf_valid = false, mass_valid = false;
-    int last_bparts = 0;
+    int last_parts = 0, last_bparts = 0;
     int remaps_since_step = 0;
     int list_state = 0;
 };
@@ -1193,7 +1232,7 @@ int snapshot_take(lbm_hip_ctx* c, Snapshot& s, hipStream_t stream) {
     }
     s.t = c->t;
     s.macro_valid = c->macro_valid; s.wallf_valid = c->wallf_valid; s.mass_valid = c->mass_valid;
-    s.last_bparts = c->last_bparts;
+    s.last_parts = c->last_parts; s.last_bparts = c->last_bparts;
     s.remaps_since_step = c->remaps_since_step;
     s.list_state = c->list_state;
     return LBM_HIP_OK;
@@ -1211,7 +1250,7 @@ int snapshot_restore(lbm_hip_ctx* c, const Snapshot& s, hipStream_t stream) {
     // copy (the trials overwrite them, possibly with another tile shape):
     // a following correct_mass sums the populations instead.
     c->mass_valid = false;
-    c->last_bparts = s.last_bparts;
+    c->last_parts = s.last_parts; c->last_bparts = s.last_bparts;
     c->remaps_since_step = s.remaps_since_step;
     c->list_state = s.list_state;
     c->links_pending = false;
@@ -1442,6 +1481,7 @@ int batch_step(lbm_hip_batch* b, int mass_every, int extra_flags) {
     if (e != hipSuccess) return fail(LBM_HIP_EDEVICE, std::string("batch step launch: ") + hipGetErrorString(e));
     b->t = tnext;
     for (lbm_hip_ctx* c : b->mem) {
+        c->last_parts = c->G;
         c->last_bparts = odd ? 0 : c->bblocks;
         c->t = tnext;
         c->macro_valid = (extra_flags & LBM_STEP_MACRO) != 0;
@@ -1519,6 +1559,7 @@ int batch_loop(lbm_hip_batch* b, int steps, int mass_every, int last_flags) {
             for (lbm_hip_ctx* c : b->mem) {
                 c->t = b->t;
                 c->macro_valid = c->wallf_valid = c->mass_valid = false;
+                c->last_parts = c->G;
                 c->last_bparts = 0;  // a cycle ends on an odd step
             }
         }
@@ -1579,7 +1620,7 @@ int lbm_hip_batch_create(const lbm_hip_params* members, int count, lbm_hip_batch
             // 64-run workload, 2-byte storage runs fastest at 6 blocks per CU
             // on both parities
             int cap[2];
-            default_occupancy(c->prec, cap);
+            default_occupancy(c->prec, false, cap);  // the batch's odd steps run in tiles
             if (c->prec == LBM_HIP_FP16 || c->prec == LBM_HIP_BF16) cap[0] = cap[1] = 6;
             if (const int rc2 = set_occupancy(c, cap[0], cap[1])) return bail(rc2);
         }

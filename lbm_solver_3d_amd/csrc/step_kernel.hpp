@@ -297,10 +297,10 @@ __device__ inline bool in_link_box(const LinkBoxDims& b, int c, int y, int z, in
     return dy < b.wy && dz < b.wz && dc < b.nck;
 }
 
-// Block sum (blocks of BLOCK threads) of the nodes' excess mass in 128-bit
-// fixed point (fixed_point.hpp); thread 0 stores it.
-__device__ __forceinline__ void block_mass_sum(double excess, MassPart* out) {
-    __shared__ unsigned long long lds[2 * (BLOCK / 64)];
+// Block sum (blocks of `waves` wavefronts, at most 16) of the nodes' excess
+// mass in 128-bit fixed point (fixed_point.hpp); thread 0 stores it.  `lds`
+// holds 2 * waves words.
+__device__ __forceinline__ void block_mass_sum(double excess, MassPart* out, int waves, unsigned long long* lds) {
     u128 v = fixed_from_double(excess);
     for (int off = 32; off > 0; off >>= 1) {
         const unsigned long long h = __shfl_xor((unsigned long long)fixed_hi(v), off, 64);
@@ -315,10 +315,14 @@ __device__ __forceinline__ void block_mass_sum(double excess, MassPart* out) {
     __syncthreads();
     if (threadIdx.x == 0) {
         u128 s = 0;
-        for (int k = 0; k < BLOCK / 64; ++k) s += fixed_join(lds[2 * k], lds[2 * k + 1]);
+        for (int k = 0; k < waves; ++k) s += fixed_join(lds[2 * k], lds[2 * k + 1]);
         out->hi = fixed_hi(s);
         out->lo = fixed_lo(s);
     }
+}
+__device__ __forceinline__ void block_mass_sum(double excess, MassPart* out) {
+    __shared__ unsigned long long lds[2 * (BLOCK / 64)];
+    block_mass_sum(excess, out, BLOCK / 64, lds);
 }
 
 // Wavefront + block reduction of K doubles over a block of NT threads; thread
@@ -664,6 +668,163 @@ void aa_step_kernel(const StepArgs<S> A, const FastCoef K) {
     step_tile<S, ODD, MODE>(A, K,
                             A.xcd_remap == 1 ? xcd_tile(lin, G)
                                              : (A.xcd_remap == 2 ? xcd_tile_rows(lin, A.gy * A.gz, A.gx, A.div_gx) : lin));
+}
+
+// ---------------------------------------------------------------------------
+// Odd step over whole rows (NX a multiple of 64, at most 1024, rows unpadded).
+//
+// In the odd step node n stores f*_i into slot i at n + c_i.  For the ten
+// populations with c_x != 0 a wavefront's 64 stores then straddle two lines
+// of the slot, each line shared with a neighbouring wavefront.  Here one
+// block updates one whole row (NX / 64 wavefronts) and those ten stores are
+// made line-aligned: each value moves one lane up (c_x = +1) or down
+// (c_x = -1) by a wavefront shift (DPP), the value that crosses into the
+// neighbouring wavefront's line comes from that wavefront through LDS (the
+// row is periodic in x, so the last wavefront's neighbour is the first), and
+// lane l stores at its own x what node x - c_x computed, if that node is
+// fluid.  Every slot address and value is the one the per-node form stores;
+// only the lane that issues the store changes.  The loads keep the per-node
+// form (straddling reads cost nothing measurable once the stores are whole
+// lines).  One block barrier separates the collisions from those stores.
+constexpr int ROW_MAX_WAVES = 16;
+constexpr int NXS = 5;                             // populations per sign of c_x
+// The row kernel's static LDS, the same for every storage and mode (the
+// occupancy cap's LDS reservation counts on it): edge values, fluid masks,
+// mass sums.
+constexpr int ROW_LDS_WORDS = 256;
+static_assert(2 * ROW_MAX_WAVES * NXS + ROW_MAX_WAVES + 2 * ROW_MAX_WAVES <= ROW_LDS_WORDS, "row kernel LDS");
+constexpr int XPLUS[NXS] = {1, 7, 9, 11, 13};      // c_x = +1
+constexpr int XMINUS[NXS] = {2, 8, 10, 12, 14};    // c_x = -1
+
+// Wavefront shift of a stored value by one lane: UP gives lane l the value
+// of lane l - 1 and lane 0 `edge`; otherwise lane l gets lane l + 1 and lane
+// 63 `edge` (DPP wave_shr:1 / wave_shl:1, all lanes enabled).
+template <bool UP, class S>
+__device__ __forceinline__ S lane_shift(S v, S edge) {
+    constexpr int CTRL = UP ? 0x138 : 0x130;
+    if constexpr (sizeof(S) == 8) {
+        const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+        const unsigned long long e = __builtin_bit_cast(unsigned long long, edge);
+        const int lo = __builtin_amdgcn_update_dpp((int)(uint32_t)e, (int)(uint32_t)u, CTRL, 0xF, 0xF, false);
+        const int hi = __builtin_amdgcn_update_dpp((int)(uint32_t)(e >> 32), (int)(uint32_t)(u >> 32), CTRL, 0xF, 0xF,
+                                                   false);
+        return __builtin_bit_cast(S, ((unsigned long long)(uint32_t)hi << 32) | (uint32_t)lo);
+    } else if constexpr (sizeof(S) == 4) {
+        return __builtin_bit_cast(S, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, edge),
+                                                                 __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false));
+    } else {
+        const int r = __builtin_amdgcn_update_dpp((int)__builtin_bit_cast(uint16_t, edge),
+                                                  (int)__builtin_bit_cast(uint16_t, v), CTRL, 0xF, 0xF, false);
+        return __builtin_bit_cast(S, (uint16_t)r);
+    }
+}
+
+template <class S, int MODE>
+__device__ __forceinline__ void odd_row_block(const StepArgs<S>& A, const FastCoef& K, uint32_t row) {
+    using St = Store<S>;
+    using T = typename St::C;
+    constexpr bool MASS = (MODE & STEP_MASS) != 0;
+    __shared__ unsigned long long row_lds[ROW_LDS_WORDS];
+    S* edge_up = reinterpret_cast<S*>(row_lds);                      // [wave][k]: lane 63's c_x = +1 values
+    S* edge_dn = reinterpret_cast<S*>(row_lds + ROW_MAX_WAVES * NXS);  // [wave][k]: lane 0's c_x = -1 values
+    unsigned long long* fluid = row_lds + 2 * ROW_MAX_WAVES * NXS;   // each wavefront's fluid lanes
+    unsigned long long* mass_lds = fluid + ROW_MAX_WAVES;
+    const double dsh = A.st->shift[A.t & 1];  // pending mass-correction shift (lbm.cpp:337)
+    const uint32_t NX = (uint32_t)A.NX, NY = (uint32_t)A.NY, NZ = (uint32_t)A.NZ;
+    const int nw = A.NX >> 6;
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int lane = threadIdx.x & 63;
+    const uint32_t z = quot(row, A.div_ny), y = row - z * NY;
+    const int x = w * 64 + lane;
+    const RowGeom rg = row_geom<S>((uint32_t)x, y, z, NX, NY, NZ, A.PX);
+    const uint8_t fl = A.flag[rg.n];
+    const bool active = (fl & TYPE_MASK) == FLUID;
+
+    // clear the slot step t+1 reads its mass shift from
+    if (row == 0 && threadIdx.x == 0) A.st->shift[(A.t + 1) & 1] = 0.0;
+
+    S* __restrict__ f = A.f;
+    const uint32_t SS = A.SS;
+    auto base = [&](int s, uint32_t r) -> char* { return (char*)(f + ((size_t)s * SS + r)); };
+    auto at = [](char* b, uint32_t off) -> S* { return (S*)(b + off); };
+    const uint32_t* xb = rg.xb;
+    const uint32_t* yo = rg.pyo;
+    const uint32_t* zo = rg.pzo;
+
+    S up[NXS], dn[NXS];
+    const S zero = St::store((T)0);
+    static_for<0, NXS>([&](auto K_) { up[decltype(K_)::value] = zero; dn[decltype(K_)::value] = zero; });
+    double excess = 0.0;
+    if (active) {
+        T sh[Q];
+        static_for<0, Q>([&](auto I) {
+            constexpr int i = decltype(I)::value;
+            if constexpr (St::shifted) sh[i] = (T)(W[i] * dsh * St::scale);
+            else sh[i] = W[i] * dsh;
+        });
+        S gs[Q];
+        static_for<0, Q>([&](auto I) {
+            constexpr int i = decltype(I)::value;
+            if constexpr (CX[i] == 0 && ODD_NT) gs[i] = load_stream(at(base(OPP[i], yo[1 - CY[i]] + zo[1 - CZ[i]]), xb[1]));
+            else gs[i] = *at(base(OPP[i], yo[1 - CY[i]] + zo[1 - CZ[i]]), xb[1 - CX[i]]);
+        });
+        T g[Q];
+        static_for<0, Q>([&](auto I) {
+            constexpr int i = decltype(I)::value;
+            g[i] = St::load_add(gs[i], sh[i], K.one);
+        });
+        const NodeOut<T> out = collide_node<S, MODE>(A, K, g);
+        static_for<0, Q>([&](auto I) {
+            constexpr int i = decltype(I)::value;
+            if constexpr (CX[i] == 0) {
+                const S val = St::store(g[i]);
+                if constexpr (ODD_NT) store_stream(at(base(i, yo[1 + CY[i]] + zo[1 + CZ[i]]), xb[1]), val);
+                else *at(base(i, yo[1 + CY[i]] + zo[1 + CZ[i]]), xb[1]) = val;
+            }
+        });
+        static_for<0, NXS>([&](auto K_) {
+            constexpr int k = decltype(K_)::value;
+            up[k] = St::store(g[XPLUS[k]]);
+            dn[k] = St::store(g[XMINUS[k]]);
+        });
+        write_moments<S, MODE>(A, out, rg.n, x, (int)y, (int)z);
+        excess = out.excess;
+    }
+
+    const unsigned long long act = __builtin_amdgcn_ballot_w64(active);
+    if (lane == 63) static_for<0, NXS>([&](auto K_) { edge_up[w * NXS + decltype(K_)::value] = up[decltype(K_)::value]; });
+    if (lane == 0) {
+        static_for<0, NXS>([&](auto K_) { edge_dn[w * NXS + decltype(K_)::value] = dn[decltype(K_)::value]; });
+        fluid[w] = act;
+    }
+    __syncthreads();
+    const int wp = w == 0 ? nw - 1 : w - 1, wn = w + 1 == nw ? 0 : w + 1;
+    // node x - 1 (x + 1) fluid: lane l stores for it at x
+    const unsigned long long src_up = (act << 1) | (fluid[wp] >> 63);
+    const unsigned long long src_dn = (act >> 1) | (fluid[wn] << 63);
+    static_for<0, NXS>([&](auto K_) {
+        constexpr int k = decltype(K_)::value;
+        constexpr int i = XPLUS[k];
+        const S v = lane_shift<true>(up[k], edge_up[wp * NXS + k]);
+        if ((src_up >> lane) & 1ull) store_stream(at(base(i, yo[1 + CY[i]] + zo[1 + CZ[i]]), xb[1]), v);
+    });
+    static_for<0, NXS>([&](auto K_) {
+        constexpr int k = decltype(K_)::value;
+        constexpr int i = XMINUS[k];
+        const S v = lane_shift<false>(dn[k], edge_dn[wn * NXS + k]);
+        if ((src_dn >> lane) & 1ull) store_stream(at(base(i, yo[1 + CY[i]] + zo[1 + CZ[i]]), xb[1]), v);
+    });
+    if constexpr (MASS) block_mass_sum(excess, &A.mass_part[row], nw, mass_lds);
+}
+
+// One block of NX threads per row; `rows_order` 1 gives each XCD a
+// contiguous range of rows (else consecutive rows go to consecutive XCDs).
+template <class S, int MODE>
+__global__ __launch_bounds__(64 * ROW_MAX_WAVES) __attribute__((amdgpu_waves_per_eu(LBM_STEP_WAVES_PER_EU)))
+void aa_odd_row_kernel(const StepArgs<S> A, const FastCoef K) {
+    const int rows = A.NY * A.NZ;
+    const int lin = (int)blockIdx.x;
+    odd_row_block<S, MODE>(A, K, (uint32_t)(A.odd_rows == 2 ? xcd_tile(lin, rows) : lin));
 }
 
 // Boundary chunks of an even step.  Work list: every 64-node chunk of the
