@@ -71,7 +71,7 @@ def measure(lbm, precision: str, steps: int, warmup: int, tiling=None) -> dict:
         bytes_per_lup=b_lup, achieved_gbs=achieved, frac=achieved / HBM_PEAK_GBS,
         U=list(sq["U"]), X=list(sq["X"]),
         tile_width=tuning["tile_width"] if tuning else None, xcd_remap=tuning["xcd_remap"] if tuning else None,
-        tuning=tuning,
+        tuning=tuning, odd_row_waves=int(info.odd_row_waves),
     )
 
 
@@ -509,7 +509,9 @@ def main():
             "frac": round(head["frac"], 4),
             "traffic": measured_traffic(head["precision"], head["tile_width"], head["xcd_remap"]),
             "traffic_source": f"{TRAFFIC_FILE.relative_to(ROOT)} for tiles {head['tile_width']},{head['xcd_remap']}",
-            "kernel": "aa_step_kernel + aa_boundary_kernel (even steps): the collide-stream update of a step",
+            "kernel": ("aa_step_kernel + aa_boundary_kernel (even steps), "
+                       + ("aa_odd_row_kernel" if head.get("odd_row_waves") else "aa_step_kernel")
+                       + " (odd steps): the collide-stream update of a step"),
             "copy_gbs": round(copy_gbs, 1),
             "frac_of_copy": round(head["achieved_gbs"] / copy_gbs, 4),
             "bytes_per_lup": head["bytes_per_lup"],
