@@ -115,7 +115,9 @@ struct StepArgs {
     int nchx;             // 64-node chunks per row
     int ty;               // tile height in y
     int xcd_remap;        // 1: give each XCD a contiguous range of tiles; 2: whole tile rows per XCD
-    int odd_rows;         // odd steps one block per row (aa_odd_row_kernel): 0 no, 1 rows in order, 2 row ranges per XCD
+    int odd_rows;         // odd steps in row blocks (aa_odd_row_kernel): 0 no, 1 blocks in order, 2 block ranges per XCD
+    int row_waves;        // wavefronts per row (PX / 64) of the row kernel
+    int rows_per_block;   // rows per block of the row kernel
     FastDiv div_gx, div_gy, div_nchx, div_ny;  // division by gx, gy, nchx and NY (tile decoding)
     double omPlus, omMinus, Fx, Fy, Fz;
     long long t;          // step index (selects double-buffer slots)

@@ -34,8 +34,10 @@ static hipError_t with_kernel_mode(int mode, Fn&& fn) {
 
 template <class S, bool ODD, int MODE>
 static hipError_t launch_step_v(const StepArgs<S>& a, const FastCoef& k, unsigned lds, hipStream_t s) {
-    if (ODD && a.odd_rows) {  // one block per row; the host checked NX (multiple of 64, <= 1024, rows unpadded)
-        hipLaunchKernelGGL((aa_odd_row_kernel<S, MODE>), dim3(a.NY * a.NZ), dim3(a.NX), lds, s, a, k);
+    if (ODD && a.odd_rows) {  // row blocks; the host checked the rows (PX a multiple of 64, at most 1024)
+        const int rows = a.NY * a.NZ, rpb = a.rows_per_block;
+        hipLaunchKernelGGL((aa_odd_row_kernel<S, MODE>), dim3((rows + rpb - 1) / rpb), dim3(64 * a.row_waves * rpb), lds,
+                           s, a, k);
         return hipGetLastError();
     }
     const int G = a.gx * a.gy * a.gz;

@@ -93,6 +93,7 @@ struct lbm_hip_ctx {
     // order, 2: row ranges per XCD) and the dynamic LDS that holds those
     // blocks to the odd cap's wavefronts per CU
     int odd_rows = 0;
+    int row_waves = 0, rows_per_block = 1, row_blocks = 0;  // PX / 64, rows per block, blocks per odd step
     unsigned occ_lds_rows = 0;
     int G_max = 0;  // largest tile count over the tile shapes (mass_part holds G_max + bblocks)
     double omPlus = 0, omMinus = 0, nu = 0;
@@ -274,6 +275,7 @@ StepArgs<S> step_args(const lbm_hip_ctx* c) {
     a.div_nchx = make_fast_div((uint32_t)a.nchx); a.div_ny = make_fast_div((uint32_t)a.NY);
     a.xcd_remap = c->xcd_remap;
     a.odd_rows = c->odd_rows;
+    a.row_waves = c->row_waves; a.rows_per_block = c->rows_per_block;
     a.omPlus = c->omPlus; a.omMinus = c->omMinus;
     a.Fx = c->p.fx; a.Fy = c->p.fy; a.Fz = c->p.fz;
     a.t = c->t;
@@ -428,7 +430,7 @@ int set_occupancy(lbm_hip_ctx* c, int even_blocks, int odd_blocks) {
         // the same wavefronts per CU in row blocks of NX / 64 wavefronts, where
         // the LDS allocation (512-byte granules, ROW_LDS_WORDS static words per
         // block) can express it; else no cap
-        const int row_waves = c->NX / 64, rb = odd_blocks * (BLOCK / 64) / row_waves;
+        const int row_waves = c->row_waves * c->rows_per_block, rb = odd_blocks * (BLOCK / 64) / row_waves;
         int per_cu = 0, per_block = 0;
         if (hipDeviceGetAttribute(&per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, c->device) != hipSuccess ||
             hipDeviceGetAttribute(&per_block, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) != hipSuccess)
@@ -547,14 +549,18 @@ int lbm_hip_create(const lbm_hip_params* pin, lbm_hip_ctx** out) {
     c->N = (uint32_t)((size_t)pin->nx * pin->ny * pin->nz);
     c->ssize = storage_bytes(c->prec);
     c->PX = row_pitch(pin->nx, pin->ny, pin->nz);
-    // Odd steps one block per row where a row is whole wavefronts and fits a
-    // block, for 2-byte storage (measured on 512 x 256 x 128: FP16 odd step
-    // 199.5 against 210 us in tiles; FP32 and FP64 no faster).
-    // LBM_HIP_ODD_ROWS: 0 off, 1 rows in order, 2 row ranges per XCD, for
-    // any storage.
-    if (pin->nx % 64 == 0 && pin->nx <= 1024 && c->PX == (uint32_t)pin->nx) {
+    // Odd steps in row blocks where a stored row is whole wavefronts (padded
+    // or not) and fits a block, for 2-byte storage (measured on 512 x 256 x
+    // 128: FP16 odd step 198 against 210 us in tiles; FP32 and FP64 no
+    // faster).  Rows of fewer than 4 wavefronts share a block.
+    // LBM_HIP_ODD_ROWS: 0 off, 1 blocks in order, 2 block ranges per XCD,
+    // for any storage.
+    if (c->PX % 64 == 0 && c->PX <= 64 * ROW_MAX_WAVES) {
         c->odd_rows = c->ssize == 2 ? 1 : 0;
         if (const char* e = std::getenv("LBM_HIP_ODD_ROWS")) c->odd_rows = std::max(0, std::min(2, std::atoi(e)));
+        c->row_waves = (int)(c->PX / 64);
+        c->rows_per_block = c->row_waves >= 4 ? 1 : 4 / c->row_waves;
+        c->row_blocks = (pin->ny * pin->nz + c->rows_per_block - 1) / c->rows_per_block;
     }
     c->SS = slot_stride(c->PX * (uint32_t)pin->ny * (uint32_t)pin->nz, c->ssize);
     c->msize = c->prec == LBM_HIP_FP64 ? 8 : 4;
@@ -754,7 +760,7 @@ int lbm_hip_step(lbm_hip_ctx* c, int flags) {
     });
     if (e != hipSuccess) return fail(LBM_HIP_EDEVICE, std::string("step launch: ") + hipGetErrorString(e));
     c->links_pending = c->has_squirmer;
-    c->last_parts = odd && c->odd_rows ? c->NY * c->NZ : c->G;
+    c->last_parts = odd && c->odd_rows ? c->row_blocks : c->G;
     c->last_bparts = odd ? 0 : c->bblocks;
     ++c->t;
     c->remaps_since_step = 0;
@@ -1089,7 +1095,7 @@ int lbm_hip_get_info(lbm_hip_ctx* c, lbm_hip_info* o) {
     o->xcd_remap = c->xcd_remap;
     o->occupancy_even = c->occ_cap[0];
     o->occupancy_odd = c->occ_cap[1];
-    o->odd_row_waves = c->odd_rows ? c->NX / 64 : 0;
+    o->odd_row_waves = c->odd_rows ? c->row_waves : 0;
     hipError_t e = hipSuccess;
     dispatch(c->prec, [&](auto* tag) -> int {
         using S = std::remove_pointer_t<decltype(tag)>;

@@ -692,7 +692,7 @@ constexpr int NXS = 5;                             // populations per sign of c_
 // occupancy cap's LDS reservation counts on it): edge values, fluid masks,
 // mass sums.
 constexpr int ROW_LDS_WORDS = 256;
-static_assert(2 * ROW_MAX_WAVES * NXS + ROW_MAX_WAVES + 2 * ROW_MAX_WAVES <= ROW_LDS_WORDS, "row kernel LDS");
+static_assert(2 * ROW_MAX_WAVES * NXS + ROW_MAX_WAVES + 2 * ROW_MAX_WAVES <= ROW_LDS_WORDS, "row kernel LDS");  // edges, flag words, mass
 constexpr int XPLUS[NXS] = {1, 7, 9, 11, 13};      // c_x = +1
 constexpr int XMINUS[NXS] = {2, 8, 10, 12, 14};    // c_x = -1
 
@@ -719,29 +719,42 @@ __device__ __forceinline__ S lane_shift(S v, S edge) {
     }
 }
 
+// One block of the odd-step row kernel: `rpb` consecutive rows from
+// `first_row`, each of A.row_waves wavefronts (PX / 64; waves past
+// rpb * row_waves, and rows past the last, idle but take part in the
+// barrier).  Rows may be padded (NX < PX): the row's last node is then lane
+// (NX - 1) % 64 of its last wavefront, and the lanes past it neither load
+// nor store.  `mass_out`: the block's excess-mass partial (MASS).
 template <class S, int MODE>
-__device__ __forceinline__ void odd_row_block(const StepArgs<S>& A, const FastCoef& K, uint32_t row) {
+__device__ __forceinline__ void odd_rows_block(const StepArgs<S>& A, const FastCoef& K, int first_row, int rpb,
+                                               MassPart* mass_out) {
     using St = Store<S>;
     using T = typename St::C;
     constexpr bool MASS = (MODE & STEP_MASS) != 0;
     __shared__ unsigned long long row_lds[ROW_LDS_WORDS];
-    S* edge_up = reinterpret_cast<S*>(row_lds);                      // [wave][k]: lane 63's c_x = +1 values
-    S* edge_dn = reinterpret_cast<S*>(row_lds + ROW_MAX_WAVES * NXS);  // [wave][k]: lane 0's c_x = -1 values
-    unsigned long long* fluid = row_lds + 2 * ROW_MAX_WAVES * NXS;   // each wavefront's fluid lanes
-    unsigned long long* mass_lds = fluid + ROW_MAX_WAVES;
+    S* edge_up = reinterpret_cast<S*>(row_lds);                        // [wave][k]: c_x = +1 values of the wave's last node
+    S* edge_dn = reinterpret_cast<S*>(row_lds + ROW_MAX_WAVES * NXS);  // [wave][k]: c_x = -1 values of lane 0
+    uint32_t* fl_up = reinterpret_cast<uint32_t*>(row_lds + 2 * ROW_MAX_WAVES * NXS);  // that node fluid
+    uint32_t* fl_dn = fl_up + ROW_MAX_WAVES;                           // lane 0 fluid
+    unsigned long long* mass_lds = row_lds + 2 * ROW_MAX_WAVES * NXS + ROW_MAX_WAVES;
     const double dsh = A.st->shift[A.t & 1];  // pending mass-correction shift (lbm.cpp:337)
     const uint32_t NX = (uint32_t)A.NX, NY = (uint32_t)A.NY, NZ = (uint32_t)A.NZ;
-    const int nw = A.NX >> 6;
-    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int nw = A.row_waves;
+    const int wb = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));  // wave in the block
     const int lane = threadIdx.x & 63;
-    const uint32_t z = quot(row, A.div_ny), y = row - z * NY;
+    const int rib = wb / nw, w = wb - rib * nw;  // row in the block, wave in the row
+    const int row = first_row + rib;
+    const bool row_ok = rib < rpb && row < A.NY * A.NZ;  // wavefront-uniform
+    const int rb0 = rib * nw;                             // the row's first wave in the block
+    const uint32_t z = row_ok ? quot((uint32_t)row, A.div_ny) : 0u, y = row_ok ? (uint32_t)row - z * NY : 0u;
     const int x = w * 64 + lane;
-    const RowGeom rg = row_geom<S>((uint32_t)x, y, z, NX, NY, NZ, A.PX);
-    const uint8_t fl = A.flag[rg.n];
+    const bool valid = row_ok && (uint32_t)x < NX;
+    const RowGeom rg = row_geom<S>(valid ? (uint32_t)x : 0u, y, z, NX, NY, NZ, A.PX);
+    const uint8_t fl = valid ? A.flag[rg.n] : (uint8_t)WALL;
     const bool active = (fl & TYPE_MASK) == FLUID;
 
     // clear the slot step t+1 reads its mass shift from
-    if (row == 0 && threadIdx.x == 0) A.st->shift[(A.t + 1) & 1] = 0.0;
+    if (first_row == 0 && threadIdx.x == 0) A.st->shift[(A.t + 1) & 1] = 0.0;
 
     S* __restrict__ f = A.f;
     const uint32_t SS = A.SS;
@@ -791,17 +804,29 @@ __device__ __forceinline__ void odd_row_block(const StepArgs<S>& A, const FastCo
         excess = out.excess;
     }
 
+    const int last = (int)((NX - 1) & 63u);            // lane of the row's last node in its last wave
+    const int lu = w == nw - 1 ? last : 63;            // this wave's last node
     const unsigned long long act = __builtin_amdgcn_ballot_w64(active);
-    if (lane == 63) static_for<0, NXS>([&](auto K_) { edge_up[w * NXS + decltype(K_)::value] = up[decltype(K_)::value]; });
+    const unsigned long long vm = __builtin_amdgcn_ballot_w64(valid);
+    if (lane == lu) {
+        static_for<0, NXS>([&](auto K_) { edge_up[wb * NXS + decltype(K_)::value] = up[decltype(K_)::value]; });
+        fl_up[wb] = active ? 1u : 0u;
+    }
     if (lane == 0) {
-        static_for<0, NXS>([&](auto K_) { edge_dn[w * NXS + decltype(K_)::value] = dn[decltype(K_)::value]; });
-        fluid[w] = act;
+        static_for<0, NXS>([&](auto K_) { edge_dn[wb * NXS + decltype(K_)::value] = dn[decltype(K_)::value]; });
+        fl_dn[wb] = active ? 1u : 0u;
     }
     __syncthreads();
-    const int wp = w == 0 ? nw - 1 : w - 1, wn = w + 1 == nw ? 0 : w + 1;
-    // node x - 1 (x + 1) fluid: lane l stores for it at x
-    const unsigned long long src_up = (act << 1) | (fluid[wp] >> 63);
-    const unsigned long long src_dn = (act >> 1) | (fluid[wn] << 63);
+    // neighbours in the row (x periodic: the last wave's next is the first)
+    const int wp = rb0 + (w == 0 ? nw - 1 : w - 1), wn = rb0 + (w + 1 == nw ? 0 : w + 1);
+    // node x - 1 (x + 1) fluid: lane l stores for it at x.  The last node's
+    // x + 1 is node 0 of the row, lane 0 of wave wn of the last wave.
+    unsigned long long src_up = 0, src_dn = 0;
+    if (row_ok) {
+        src_up = ((act << 1) | (unsigned long long)fl_up[wp]) & vm;
+        const unsigned long long d = (unsigned long long)fl_dn[wn];
+        src_dn = ((act >> 1) | (d << 63) | (w == nw - 1 ? d << last : 0ull)) & vm;
+    }
     static_for<0, NXS>([&](auto K_) {
         constexpr int k = decltype(K_)::value;
         constexpr int i = XPLUS[k];
@@ -811,20 +836,23 @@ __device__ __forceinline__ void odd_row_block(const StepArgs<S>& A, const FastCo
     static_for<0, NXS>([&](auto K_) {
         constexpr int k = decltype(K_)::value;
         constexpr int i = XMINUS[k];
-        const S v = lane_shift<false>(dn[k], edge_dn[wn * NXS + k]);
+        S v = lane_shift<false>(dn[k], edge_dn[wn * NXS + k]);
+        if ((uint32_t)x == NX - 1) v = edge_dn[wn * NXS + k];  // a padded row's last node: from node 0
         if ((src_dn >> lane) & 1ull) store_stream(at(base(i, yo[1 + CY[i]] + zo[1 + CZ[i]]), xb[1]), v);
     });
-    if constexpr (MASS) block_mass_sum(excess, &A.mass_part[row], nw, mass_lds);
+    if constexpr (MASS) block_mass_sum(excess, mass_out, (int)(blockDim.x >> 6), mass_lds);
 }
 
-// One block of NX threads per row; `rows_order` 1 gives each XCD a
-// contiguous range of rows (else consecutive rows go to consecutive XCDs).
+// Odd step, A.rows_per_block rows of A.row_waves wavefronts per block;
+// `odd_rows` 2 gives each XCD a contiguous range of blocks (else consecutive
+// blocks go to consecutive XCDs).
 template <class S, int MODE>
 __global__ __launch_bounds__(64 * ROW_MAX_WAVES) __attribute__((amdgpu_waves_per_eu(LBM_STEP_WAVES_PER_EU)))
 void aa_odd_row_kernel(const StepArgs<S> A, const FastCoef K) {
-    const int rows = A.NY * A.NZ;
+    const int nb = (int)gridDim.x;
     const int lin = (int)blockIdx.x;
-    odd_row_block<S, MODE>(A, K, (uint32_t)(A.odd_rows == 2 ? xcd_tile(lin, rows) : lin));
+    const int b = A.odd_rows == 2 ? xcd_tile(lin, nb) : lin;
+    odd_rows_block<S, MODE>(A, K, b * A.rows_per_block, A.rows_per_block, &A.mass_part[b]);
 }
 
 // Boundary chunks of an even step.  Work list: every 64-node chunk of the

@@ -1,15 +1,18 @@
-"""Odd steps with one block per row (aa_odd_row_kernel).
+"""Odd steps in row blocks (aa_odd_row_kernel).
 
-On lattices whose rows are whole wavefronts (NX a multiple of 64, at most
-1024) the odd step of 2-byte storage runs one block per row: the ten stores
-with c_x != 0 are moved one lane by a wavefront shift and the value that
-crosses a wavefront edge goes through LDS, so every store is line-aligned.
-Only the lane that issues a store changes, never its address or value:
+On lattices whose stored rows are whole wavefronts (the row pitch a multiple
+of 64, at most 1024: NX a multiple of 64, or rows padded to one) the odd step
+of 2-byte storage runs whole rows per block: the ten stores with c_x != 0 are
+moved one lane by a wavefront shift and the value that crosses a wavefront
+edge goes through LDS, so every store is line-aligned.  Only the lane that
+issues a store changes, never its address or value:
 
 - FP64 with the row kernel forced on matches the oracle exactly as the tile
   kernel does (bit-identical without a squirmer, 1e-12 with one), for rows of
-  one wavefront (the shift wraps within the wavefront), two and sixteen, and
-  equals the tile kernel bit for bit through mass corrections;
+  one wavefront (the shift wraps within the wavefront), two and sixteen,
+  padded rows (the row's last node inside its last wavefront) and several
+  rows per block, and equals the tile kernel bit for bit through mass
+  corrections;
 - FP16 and BF16 are bit-identical with the row kernel and with the odd step
   in tiles, through eager steps, mass corrections, remaps and graph replay.
 The environment variable LBM_HIP_ODD_ROWS is read when a lattice is created
@@ -43,12 +46,15 @@ def _exact(o, L):
     dict(nx=64, ny=9, nz=7, tau=0.7, fx=2e-5, fy=1e-5, walls_y=1, walls_z=1),     # one wavefront per row
     dict(nx=128, ny=6, nz=9, tau=0.9, fx=1e-5, fz=-2e-5, walls_y=0, walls_z=1),   # two
     dict(nx=1024, ny=5, nz=6, tau=0.6, fx=3e-5, walls_y=1, walls_z=0),            # sixteen, the largest block
+    dict(nx=50, ny=7, nz=9, tau=0.8, fx=2e-5, fz=1e-5, walls_y=1, walls_z=1),     # padded to 64, 4 rows per block
+    dict(nx=100, ny=9, nz=5, tau=0.7, fx=-1e-5, fy=2e-5, walls_y=0, walls_z=1),   # padded to 128, 2 rows per block
+    dict(nx=200, ny=5, nz=7, tau=0.9, fx=2e-5, walls_y=1, walls_z=0),             # padded to 256
 ])
 def test_fp64_rows_bit_identical_to_oracle(oracle_mod, lbm, odd_rows, kv):
     odd_rows(1)
     o = oracle_mod.Oracle(**kv)
     with lbm.Lattice(**kv, precision="fp64") as L:
-        assert L.info().odd_row_waves == kv["nx"] // 64
+        assert L.info().odd_row_waves == (kv["nx"] + 63) // 64
         for _ in range(43):
             o.step()
             L.step(lbm.STEP_ALL)
@@ -74,15 +80,17 @@ def test_fp64_rows_equal_tiles_with_mass_correction(lbm, odd_rows, kv):
     assert np.array_equal(out[0], out[1])
 
 
-def test_fp64_rows_squirmer_matches_oracle(oracle_mod, lbm, odd_rows):
-    """A squirmer across x = 0 on one-wavefront rows: the remap, link pass and
-    mass correction after the row kernel's odd steps, against the oracle."""
+@pytest.mark.parametrize("nx,x0", [(64, 63.2), (100, 98.8)])
+def test_fp64_rows_squirmer_matches_oracle(oracle_mod, lbm, odd_rows, nx, x0):
+    """A squirmer across x = 0 on one-wavefront rows and on padded rows: the
+    remap, link pass and mass correction after the row kernel's odd steps,
+    against the oracle."""
     odd_rows(1)
-    kv = dict(nx=64, ny=22, nz=20, walls_y=0, walls_z=1, squirmer=1, radius=4.3, b1=0.03, beta=-2.0, angle=20,
-              planar=1, x0=63.2)
+    kv = dict(nx=nx, ny=22, nz=20, walls_y=0, walls_z=1, squirmer=1, radius=4.3, b1=0.03, beta=-2.0, angle=20,
+              planar=1, x0=x0)
     o = oracle_mod.Oracle(**kv)
     with lbm.Lattice(**kv, precision="fp64") as L:
-        assert L.info().odd_row_waves == 1
+        assert L.info().odd_row_waves == (nx + 63) // 64
         for t in range(1, 201):
             o.step(); L.step(lbm.STEP_ALL)
             o.advance_squirmer(); L.advance_squirmer()
@@ -107,10 +115,10 @@ def _state(L):
     return out
 
 
-@pytest.mark.parametrize("prec", ["fp16", "bf16"])
-def test_rows_equal_tiles_bit_for_bit(lbm, odd_rows, prec):
-    kv = dict(nx=192, ny=48, nz=40, walls_y=0, walls_z=1, squirmer=1, radius=6.0, b1=2e-3, beta=-3.0,
-              angle=15.0, planar=1, plane_xz=1, z0=20.0, x0=190.5, hmin=1.0)
+@pytest.mark.parametrize("prec,nx", [("fp16", 192), ("bf16", 192), ("fp16", 132)])
+def test_rows_equal_tiles_bit_for_bit(lbm, odd_rows, prec, nx):
+    kv = dict(nx=nx, ny=48, nz=40, walls_y=0, walls_z=1, squirmer=1, radius=6.0, b1=2e-3, beta=-3.0,
+              angle=15.0, planar=1, plane_xz=1, z0=20.0, x0=nx - 1.5, hmin=1.0)
     states = []
     for mode in (0, 1):
         odd_rows(mode)
@@ -137,7 +145,8 @@ def test_row_kernel_selection(lbm, odd_rows, monkeypatch):
         (dict(nx=512, ny=8, nz=8), "bf16", 8),
         (dict(nx=512, ny=8, nz=8), "fp32", 0),    # no faster in rows: tiles
         (dict(nx=512, ny=8, nz=8), "fp64", 0),
-        (dict(nx=100, ny=8, nz=8), "fp16", 0),    # rows not whole wavefronts
+        (dict(nx=100, ny=8, nz=8), "fp16", 2),    # rows padded to 128 nodes
+        (dict(nx=40, ny=8, nz=8), "fp16", 0),     # rows not padded (64 would add more than half): not whole wavefronts
         (dict(nx=2048, ny=4, nz=4), "fp16", 0),   # more than 1024 threads per row
     ]:
         with lbm.Lattice(**kv, precision=prec) as L:
