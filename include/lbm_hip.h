@@ -267,6 +267,8 @@ typedef struct lbm_hip_batch_info {
     int link_blocks;          /* blocks of one particle-link launch */
     int tile_width, xcd_remap;
     size_t device_bytes;
+    int odd_row_blocks;       /* blocks of one odd-step launch in row blocks (0: odd steps in tiles) */
+    int odd_row_block_waves;  /* wavefronts per row block */
 } lbm_hip_batch_info;
 
 int lbm_hip_batch_create(const lbm_hip_params* members, int count, lbm_hip_batch** out);

@@ -130,7 +130,7 @@ class BatchInfo(C.Structure):
     _fields_ = [("count", C.c_int), ("precision", C.c_int), ("steps_done", C.c_longlong),
                 ("total_nodes", C.c_longlong), ("total_tiles", C.c_int), ("boundary_blocks", C.c_int),
                 ("link_blocks", C.c_int), ("tile_width", C.c_int), ("xcd_remap", C.c_int),
-                ("device_bytes", C.c_size_t)]
+                ("device_bytes", C.c_size_t), ("odd_row_blocks", C.c_int), ("odd_row_block_waves", C.c_int)]
 
 
 def load_library(path: Path | None = None):

@@ -31,6 +31,7 @@ struct BatchFinalize {
     const MassPart* part;
     DevState* st;
     int G, bblocks, has_squirmer;
+    int G_odd;  // collide-stream partials of an odd step (G, or the member's row blocks)
 };
 
 constexpr uint32_t BATCH_LOCAL_MASK = 0xFFFFFFu;
@@ -49,6 +50,22 @@ void batch_step_kernel(const BatchMember<S>* __restrict__ M, const uint32_t* __r
     const FastCoef K = from_constant(&mb->K);
     A.t = t;
     step_tile<S, ODD, MODE>(A, K, (int)(e & BATCH_LOCAL_MASK));
+}
+
+// Odd step in row blocks (odd_rows_block) for every member: block map
+// member << 24 | the member's row block; the member's A.rows_per_block rows
+// of A.row_waves wavefronts per block, the launch's block holding the most
+// wavefronts any member needs.
+template <class S, int MODE>
+__global__ __launch_bounds__(64 * ROW_MAX_WAVES) __attribute__((amdgpu_waves_per_eu(LBM_STEP_WAVES_PER_EU)))
+void batch_odd_row_kernel(const BatchMember<S>* __restrict__ M, const uint32_t* __restrict__ row_map, long long t) {
+    const uint32_t e = as_constant(row_map)[blockIdx.x];
+    const BatchMember<S>* mb = M + (e >> 24);
+    StepArgs<S> A = from_constant(&mb->A);
+    const FastCoef K = from_constant(&mb->K);
+    A.t = t;
+    const int lb = (int)(e & BATCH_LOCAL_MASK);
+    odd_rows_block<S, MODE>(A, K, lb * A.rows_per_block, A.rows_per_block, &A.mass_part[lb]);
 }
 
 template <class S, int MODE>

@@ -20,7 +20,9 @@ template hipError_t launch_boundary<bf16_t>(const StepArgs<bf16_t>&, const FastC
     template hipError_t launch_batch_step<S>(const BatchMember<S>*, const uint32_t*, int, long long, int, bool, \
                                              int, unsigned, hipStream_t);                                       \
     template hipError_t launch_batch_boundary<S>(const BatchMember<S>*, const uint32_t*, int, long long, int,    \
-                                                 hipStream_t);
+                                                 hipStream_t);                                                  \
+    template hipError_t launch_batch_odd_rows<S>(const BatchMember<S>*, const uint32_t*, int, int, long long, int, \
+                                                 unsigned, hipStream_t);
 LBMHIP_INSTANTIATE_BATCH(float)
 LBMHIP_INSTANTIATE_BATCH(_Float16)
 LBMHIP_INSTANTIATE_BATCH(bf16_t)

@@ -128,6 +128,8 @@ template hipError_t launch_batch_step<double>(const BatchMember<double>*, const 
                                               int, unsigned, hipStream_t);
 template hipError_t launch_batch_boundary<double>(const BatchMember<double>*, const uint32_t*, int, long long, int,
                                                   hipStream_t);
+template hipError_t launch_batch_odd_rows<double>(const BatchMember<double>*, const uint32_t*, int, int, long long,
+                                                  int, unsigned, hipStream_t);
 
 // ---------------------------------------------------------------------------
 // Mass correction (lbm.cpp:324-339): d = 1 - m / n_fluid with
@@ -219,7 +221,7 @@ __global__ __launch_bounds__(1024) void batch_finalize_kernel(const BatchFinaliz
     const BatchFinalize& f = fa[blockIdx.x];
     if (!f.has_squirmer) return;
     const bool last_odd = ((t_after - 1) & 1) != 0;
-    mass_finalize_block(f.part, f.G + (last_odd ? 0 : f.bblocks), f.st, MASS_TERM_LINKS | MASS_TERM_REMAP,
+    mass_finalize_block(f.part, last_odd ? f.G_odd : f.G + f.bblocks, f.st, MASS_TERM_LINKS | MASS_TERM_REMAP,
                         (int)(t_after & 1));
 }
 

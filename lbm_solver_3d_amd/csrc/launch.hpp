@@ -72,6 +72,10 @@ hipError_t launch_batch_step(const BatchMember<S>* m, const uint32_t* tile_map, 
 template <class S>
 hipError_t launch_batch_boundary(const BatchMember<S>* m, const uint32_t* block_map, int blocks, long long t,
                                  int mode, hipStream_t s);
+// Odd step in row blocks of `threads` threads (row_map: member << 24 | row block).
+template <class S>
+hipError_t launch_batch_odd_rows(const BatchMember<S>* m, const uint32_t* row_map, int blocks, int threads,
+                                 long long t, int mode, unsigned lds, hipStream_t s);
 template <class S>
 hipError_t launch_batch_couple(const BatchMember<S>* m, const uint32_t* block_map, int link_blocks, int count,
                                long long t_after, hipStream_t s);
@@ -117,6 +121,8 @@ hipError_t launch_copy_parts(const void* const* a, void* const* b, int parts, si
                                                     bool, int, unsigned, hipStream_t);                          \
     extern template hipError_t launch_batch_boundary<S>(const BatchMember<S>*, const uint32_t*, int, long long,  \
                                                         int, hipStream_t);                                      \
+    extern template hipError_t launch_batch_odd_rows<S>(const BatchMember<S>*, const uint32_t*, int, int,        \
+                                                        long long, int, unsigned, hipStream_t);                 \
     extern template hipError_t launch_batch_couple<S>(const BatchMember<S>*, const uint32_t*, int, int, long long, \
                                                       hipStream_t);
 

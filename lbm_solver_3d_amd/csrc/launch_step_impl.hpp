@@ -98,6 +98,21 @@ hipError_t launch_batch_step(const BatchMember<S>* m, const uint32_t* map, int G
 }
 
 template <class S, int MODE>
+static hipError_t launch_batch_odd_rows_v(const BatchMember<S>* m, const uint32_t* map, int blocks, int threads,
+                                          long long t, unsigned lds, hipStream_t s) {
+    hipLaunchKernelGGL((batch_odd_row_kernel<S, MODE>), dim3(blocks), dim3(threads), lds, s, m, map, t);
+    return hipGetLastError();
+}
+
+template <class S>
+hipError_t launch_batch_odd_rows(const BatchMember<S>* m, const uint32_t* map, int blocks, int threads, long long t,
+                                 int mode, unsigned lds, hipStream_t s) {
+    if (blocks <= 0) return hipSuccess;
+    return with_kernel_mode<S>(
+        mode, [&](auto M) { return launch_batch_odd_rows_v<S, decltype(M)::value>(m, map, blocks, threads, t, lds, s); });
+}
+
+template <class S, int MODE>
 static hipError_t launch_batch_boundary_v(const BatchMember<S>* m, const uint32_t* map, int blocks, long long t,
                                           hipStream_t s) {
     hipLaunchKernelGGL((batch_boundary_kernel<S, MODE>), dim3(blocks), dim3(BLOCK), 0, s, m, map, t);

@@ -94,6 +94,7 @@ struct lbm_hip_ctx {
     // blocks to the odd cap's wavefronts per CU
     int odd_rows = 0;
     int row_waves = 0, rows_per_block = 1, row_blocks = 0;  // PX / 64, rows per block, blocks per odd step
+    int batch_row_blocks = 0;  // row blocks of this member in its batch's odd steps
     unsigned occ_lds_rows = 0;
     int G_max = 0;  // largest tile count over the tile shapes (mass_part holds G_max + bblocks)
     double omPlus = 0, omMinus = 0, nu = 0;
@@ -420,27 +421,32 @@ void default_occupancy(int prec, bool odd_rows, int cap[2]) {
     if (odd_rows && prec != LBM_HIP_FP64) cap[1] = 0;
 }
 
+// Dynamic LDS per block of the row kernel (ROW_LDS_WORDS words of static LDS,
+// `block_waves` wavefronts per block) that holds a CU to the wavefronts of
+// `cap` 256-thread blocks, where the LDS allocation (512-byte granules) can
+// express it; else 0 (no cap).
+int row_occupancy_lds(int device, int cap, int block_waves, unsigned* bytes) {
+    *bytes = 0;
+    const int rb = cap * (BLOCK / 64) / block_waves;
+    if (cap == 0 || rb < 1 || rb * block_waves >= 32) return LBM_HIP_OK;
+    int per_cu = 0, per_block = 0;
+    if (hipDeviceGetAttribute(&per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device) != hipSuccess ||
+        hipDeviceGetAttribute(&per_block, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess)
+        return fail(LBM_HIP_EDEVICE, "LDS size query failed");
+    const long stat = (long)ROW_LDS_WORDS * 8;  // a multiple of 512
+    const long total = std::min((long)(per_cu / rb) / 512 * 512, (long)per_block);
+    if (total > stat && total * (rb + 1) > per_cu) *bytes = (unsigned)(total - stat);
+    return LBM_HIP_OK;
+}
+
 int set_occupancy(lbm_hip_ctx* c, int even_blocks, int odd_blocks) {
     for (int b : {even_blocks, odd_blocks})
         if (b != 0 && (b < 2 || b > 8)) return fail(LBM_HIP_EINVAL, "blocks per CU must be 0 (no cap) or 2..8");
     unsigned lds[2], lds_rows = 0;
     if (int rc = occupancy_lds(c->device, even_blocks, &lds[0])) return rc;
     if (int rc = occupancy_lds(c->device, odd_blocks, &lds[1])) return rc;
-    if (c->odd_rows && odd_blocks) {
-        // the same wavefronts per CU in row blocks of NX / 64 wavefronts, where
-        // the LDS allocation (512-byte granules, ROW_LDS_WORDS static words per
-        // block) can express it; else no cap
-        const int row_waves = c->row_waves * c->rows_per_block, rb = odd_blocks * (BLOCK / 64) / row_waves;
-        int per_cu = 0, per_block = 0;
-        if (hipDeviceGetAttribute(&per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, c->device) != hipSuccess ||
-            hipDeviceGetAttribute(&per_block, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) != hipSuccess)
-            return fail(LBM_HIP_EDEVICE, "LDS size query failed");
-        const long stat = (long)ROW_LDS_WORDS * 8;  // a multiple of 512
-        if (rb >= 1 && rb * row_waves < 32) {
-            const long total = std::min((long)(per_cu / rb) / 512 * 512, (long)per_block);
-            if (total > stat && total * (rb + 1) > per_cu) lds_rows = (unsigned)(total - stat);
-        }
-    }
+    if (c->odd_rows)
+        if (int rc = row_occupancy_lds(c->device, odd_blocks, c->row_waves * c->rows_per_block, &lds_rows)) return rc;
     if (c->stream && hipStreamSynchronize(c->stream) != hipSuccess) return fail(LBM_HIP_EDEVICE, "stream sync");
     c->occ_cap[0] = even_blocks; c->occ_cap[1] = odd_blocks;
     c->occ_lds[0] = lds[0]; c->occ_lds[1] = lds[1];
@@ -1358,7 +1364,11 @@ struct lbm_hip_batch {
     uint32_t* tile_map = nullptr;     // member << 24 | tile, sum of G_max entries
     uint32_t* bmap = nullptr;         // member << 24 | boundary block
     uint32_t* lmap = nullptr;         // member << 24 | link block
-    int Gtot = 0, Btot = 0, Ltot = 0;
+    uint32_t* row_map = nullptr;      // member << 24 | row block (odd steps in row blocks)
+    int Gtot = 0, Btot = 0, Ltot = 0, Rtot = 0;
+    bool odd_rows = false;            // odd steps in row blocks (every member's rows whole wavefronts)
+    int row_block_waves = 0;          // wavefronts per row block
+    unsigned occ_lds_rows = 0;        // dynamic LDS of the row blocks for the odd cap
     bool any_squirmer = false;
     bool any_force = false;           // some member has a body force (all run the forcing terms)
     long long t = 0;
@@ -1374,7 +1384,7 @@ void free_batch(lbm_hip_batch* b) {
     if (!b) return;
     hipSetDevice(b->device);
     if (b->stream) hipStreamSynchronize(b->stream);
-    for (void* q : {b->desc, (void*)b->fin, (void*)b->tile_map, (void*)b->bmap, (void*)b->lmap})
+    for (void* q : {b->desc, (void*)b->fin, (void*)b->tile_map, (void*)b->bmap, (void*)b->lmap, (void*)b->row_map})
         if (q) hipFree(q);
     if (b->graph) hipGraphExecDestroy(b->graph);
     for (hipEvent_t e : {b->ev[0], b->ev[1]})
@@ -1391,7 +1401,24 @@ void free_batch(lbm_hip_batch* b) {
 // (Re)build the member argument table and the block maps for the members'
 // current tile shapes.
 int batch_tables(lbm_hip_batch* b) {
-    std::vector<uint32_t> tmap, bm, lm;
+    std::vector<uint32_t> tmap, bm, lm, rmap;
+    if (b->odd_rows) {
+        // wavefronts per row block: the smallest power of two that holds the
+        // longest row (whole blocks fill a CU's 32 wavefront slots), each
+        // member's rows packed floor(block / row) to a block
+        int lo = 1;
+        for (lbm_hip_ctx* c : b->mem) lo = std::max(lo, c->row_waves);
+        b->row_block_waves = 1;
+        while (b->row_block_waves < lo) b->row_block_waves *= 2;
+        for (int m = 0; m < b->count; ++m) {
+            lbm_hip_ctx* c = b->mem[(size_t)m];
+            const int rpb = b->row_block_waves / c->row_waves;
+            c->batch_row_blocks = (c->NY * c->NZ + rpb - 1) / rpb;
+            for (int k = 0; k < c->batch_row_blocks; ++k) rmap.push_back((uint32_t)m << 24 | (uint32_t)k);
+        }
+        if (int rc = row_occupancy_lds(b->device, b->mem[0]->occ_cap[1], b->row_block_waves, &b->occ_lds_rows))
+            return rc;
+    }
     std::vector<BatchFinalize> fin((size_t)b->count);
     const bool row_order = b->mem[0]->xcd_remap == 2;
     std::vector<uint32_t> queue[8];  // tile order 2: whole tile rows per XCD
@@ -1412,7 +1439,8 @@ int batch_tables(lbm_hip_batch* b) {
         for (int k = 0; k < c->bblocks; ++k) bm.push_back(tag | (uint32_t)k);
         if (c->has_squirmer)
             for (int k = 0; k < c->cpl_blocks; ++k) lm.push_back(tag | (uint32_t)k);
-        fin[(size_t)m] = BatchFinalize{c->mass_part, c->st, c->G, c->bblocks, c->has_squirmer ? 1 : 0};
+        fin[(size_t)m] = BatchFinalize{c->mass_part, c->st, c->G, c->bblocks, c->has_squirmer ? 1 : 0,
+                                       b->odd_rows ? c->batch_row_blocks : c->G};
     }
     if (row_order) {
         // block i runs on XCD i mod 8: interleave the queues
@@ -1422,7 +1450,10 @@ int batch_tables(lbm_hip_batch* b) {
                 if (j < queue[q].size()) { tmap.push_back(queue[q][j]); ++left; }
         }
     }
-    b->Gtot = (int)tmap.size(); b->Btot = (int)bm.size(); b->Ltot = (int)lm.size();
+    b->Gtot = (int)tmap.size(); b->Btot = (int)bm.size(); b->Ltot = (int)lm.size(); b->Rtot = (int)rmap.size();
+    if (!rmap.empty())
+        HIP_TRY(hipMemcpyAsync(b->row_map, rmap.data(), rmap.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                               b->stream));
     HIP_TRY(hipMemcpyAsync(b->tile_map, tmap.data(), tmap.size() * sizeof(uint32_t), hipMemcpyHostToDevice, b->stream));
     if (!bm.empty())
         HIP_TRY(hipMemcpyAsync(b->bmap, bm.data(), bm.size() * sizeof(uint32_t), hipMemcpyHostToDevice, b->stream));
@@ -1437,6 +1468,7 @@ int batch_tables(lbm_hip_batch* b) {
             lbm_hip_ctx* c = b->mem[(size_t)m];
             BatchMember<S>& x = d[(size_t)m];
             x.A = step_args<S>(c);
+            if (b->odd_rows) x.A.rows_per_block = b->row_block_waves / c->row_waves;
             x.K = c->K;
             x.B = c->bargs;
             x.L = link_args<S>(c);
@@ -1470,8 +1502,12 @@ int batch_step(lbm_hip_batch* b, int mass_every, int extra_flags) {
     dispatch(b->prec, [&](auto* tag) -> int {
         using S = std::remove_pointer_t<decltype(tag)>;
         const auto* d = static_cast<const BatchMember<S>*>(b->desc);
-        e = launch_batch_step<S>(d, b->tile_map, b->Gtot, b->t, b->mem[0]->xcd_remap == 1, odd, mode,
-                                 b->mem[0]->occ_lds[odd], b->stream);
+        if (odd && b->odd_rows)
+            e = launch_batch_odd_rows<S>(d, b->row_map, b->Rtot, 64 * b->row_block_waves, b->t, mode, b->occ_lds_rows,
+                                         b->stream);
+        else
+            e = launch_batch_step<S>(d, b->tile_map, b->Gtot, b->t, b->mem[0]->xcd_remap == 1, odd, mode,
+                                     b->mem[0]->occ_lds[odd], b->stream);
         if (e == hipSuccess && !odd) e = launch_batch_boundary<S>(d, b->bmap, b->Btot, b->t, mode, b->stream);
         if (e == hipSuccess && (extra_flags & LBM_STEP_WALL_FORCE))
             for (lbm_hip_ctx* c : b->mem)
@@ -1487,7 +1523,7 @@ int batch_step(lbm_hip_batch* b, int mass_every, int extra_flags) {
     if (e != hipSuccess) return fail(LBM_HIP_EDEVICE, std::string("batch step launch: ") + hipGetErrorString(e));
     b->t = tnext;
     for (lbm_hip_ctx* c : b->mem) {
-        c->last_parts = c->G;
+        c->last_parts = odd && b->odd_rows ? c->batch_row_blocks : c->G;
         c->last_bparts = odd ? 0 : c->bblocks;
         c->t = tnext;
         c->macro_valid = (extra_flags & LBM_STEP_MACRO) != 0;
@@ -1565,7 +1601,7 @@ int batch_loop(lbm_hip_batch* b, int steps, int mass_every, int last_flags) {
             for (lbm_hip_ctx* c : b->mem) {
                 c->t = b->t;
                 c->macro_valid = c->wallf_valid = c->mass_valid = false;
-                c->last_parts = c->G;
+                c->last_parts = b->odd_rows ? c->batch_row_blocks : c->G;
                 c->last_bparts = 0;  // a cycle ends on an odd step
             }
         }
@@ -1626,7 +1662,7 @@ int lbm_hip_batch_create(const lbm_hip_params* members, int count, lbm_hip_batch
             // 64-run workload, 2-byte storage runs fastest at 6 blocks per CU
             // on both parities
             int cap[2];
-            default_occupancy(c->prec, false, cap);  // the batch's odd steps run in tiles
+            default_occupancy(c->prec, c->odd_rows != 0, cap);
             if (c->prec == LBM_HIP_FP16 || c->prec == LBM_HIP_BF16) cap[0] = cap[1] = 6;
             if (const int rc2 = set_occupancy(c, cap[0], cap[1])) return bail(rc2);
         }
@@ -1641,12 +1677,25 @@ int lbm_hip_batch_create(const lbm_hip_params* members, int count, lbm_hip_batch
     // members finished their set-up on their own streams
     for (lbm_hip_ctx* c : b->mem)
         if (hipStreamSynchronize(c->stream) != hipSuccess) return bail(fail(LBM_HIP_EDEVICE, "member set-up failed"));
-    size_t gmax = 0, bsum = 0, lsum = 0;
+    size_t gmax = 0, bsum = 0, lsum = 0, rsum = 0;
+    // Odd steps in row blocks only on request (LBM_HIP_ODD_ROWS = 1 or 2) and
+    // where every member's rows are whole wavefronts: measured on the 64-run
+    // ensemble (FP16, tuned tiles), row blocks of 8 / 12 / 16 wavefronts took
+    // 3.42 / 3.40 / 3.59 ms per step against 3.27 ms with the odd step in tiles.
+    {
+        const char* e = std::getenv("LBM_HIP_ODD_ROWS");
+        b->odd_rows = e && std::atoi(e) > 0;
+    }
     for (lbm_hip_ctx* c : b->mem) {
         gmax += (size_t)c->G_max;
         bsum += (size_t)c->bblocks;
         lsum += c->has_squirmer ? (size_t)c->cpl_blocks : 0;
+        rsum += (size_t)c->NY * (size_t)c->NZ;
+        b->odd_rows = b->odd_rows && c->odd_rows != 0;
     }
+    if (b->odd_rows)  // the row blocks run uncapped
+        for (lbm_hip_ctx* c : b->mem)
+            if (const int rc2 = set_occupancy(c, c->occ_cap[0], 0)) return bail(rc2);
     const size_t dsz = b->prec == LBM_HIP_FP64 ? sizeof(BatchMember<double>)
                        : b->prec == LBM_HIP_FP32 ? sizeof(BatchMember<float>)
                        : b->prec == LBM_HIP_FP16 ? sizeof(BatchMember<_Float16>)
@@ -1658,7 +1707,8 @@ int lbm_hip_batch_create(const lbm_hip_params* members, int count, lbm_hip_batch
     };
     if (!dmalloc(&b->desc, dsz * (size_t)count) || !dmalloc((void**)&b->fin, sizeof(BatchFinalize) * (size_t)count) ||
         !dmalloc((void**)&b->tile_map, sizeof(uint32_t) * gmax) || !dmalloc((void**)&b->bmap, sizeof(uint32_t) * bsum) ||
-        !dmalloc((void**)&b->lmap, sizeof(uint32_t) * lsum))
+        !dmalloc((void**)&b->lmap, sizeof(uint32_t) * lsum) ||
+        (b->odd_rows && !dmalloc((void**)&b->row_map, sizeof(uint32_t) * rsum)))
         return bail(fail(LBM_HIP_ENOMEM, "batch: device allocation failed"));
     const int rc = batch_tables(b);
     if (rc) return bail(rc);
@@ -1699,6 +1749,8 @@ int lbm_hip_batch_info_get(lbm_hip_batch* b, lbm_hip_batch_info* o) {
     o->tile_width = b->mem[0]->tx_shift ? 1 << b->mem[0]->tx_shift : 0;
     o->xcd_remap = b->mem[0]->xcd_remap;
     o->device_bytes = b->device_bytes;
+    o->odd_row_blocks = b->odd_rows ? b->Rtot : 0;
+    o->odd_row_block_waves = b->odd_rows ? b->row_block_waves : 0;
     return LBM_HIP_OK;
 }
 
@@ -1717,6 +1769,7 @@ int lbm_hip_batch_set_occupancy(lbm_hip_batch* b, int even_blocks, int odd_block
     if (rc) return rc;
     for (lbm_hip_ctx* c : b->mem)
         if ((rc = set_occupancy(c, even_blocks, odd_blocks))) return rc;
+    if (b->odd_rows && (rc = row_occupancy_lds(b->device, odd_blocks, b->row_block_waves, &b->occ_lds_rows))) return rc;
     HIP_TRY(hipStreamSynchronize(b->stream));
     if (b->graph) { hipGraphExecDestroy(b->graph); b->graph = nullptr; }
     return LBM_HIP_OK;

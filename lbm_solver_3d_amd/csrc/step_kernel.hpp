@@ -818,7 +818,7 @@ __device__ __forceinline__ void odd_rows_block(const StepArgs<S>& A, const FastC
     }
     __syncthreads();
     // neighbours in the row (x periodic: the last wave's next is the first)
-    const int wp = rb0 + (w == 0 ? nw - 1 : w - 1), wn = rb0 + (w + 1 == nw ? 0 : w + 1);
+    const int wp = row_ok ? rb0 + (w == 0 ? nw - 1 : w - 1) : wb, wn = row_ok ? rb0 + (w + 1 == nw ? 0 : w + 1) : wb;
     // node x - 1 (x + 1) fluid: lane l stores for it at x.  The last node's
     // x + 1 is node 0 of the row, lane 0 of wave wn of the last wave.
     unsigned long long src_up = 0, src_dn = 0;

@@ -154,3 +154,41 @@ def test_row_kernel_selection(lbm, odd_rows, monkeypatch):
             assert info.odd_row_waves == waves, (kv, prec)
             if waves:
                 assert info.occupancy_odd == 0  # the row kernel runs uncapped by default
+
+
+BATCH_MEMBERS = [
+    # slit channels of the ensemble's kind: rows of 2, 3 (padded) and 6 wavefronts
+    dict(nx=128, ny=64, nz=32, walls_y=0, walls_z=1, squirmer=1, radius=5.0, b1=1e-3, beta=-2.0, angle=20.0,
+         z0=12.0, planar=1, plane_xz=1, hmin=1.0, x0=126.5),
+    dict(nx=164, ny=82, nz=41, walls_y=0, walls_z=1, squirmer=1, radius=6.0, b1=1e-3, beta=3.0, angle=40.0,
+         z0=25.0, planar=1, plane_xz=1, hmin=1.0),
+    dict(nx=384, ny=20, nz=24, walls_y=0, walls_z=1, squirmer=1, radius=4.0, b1=1e-3, beta=0.5, angle=5.0,
+         z0=8.0, planar=1, plane_xz=1, hmin=1.0),
+    # a duct with a body force and no squirmer, one wavefront per row
+    dict(nx=64, ny=10, nz=12, tau=0.8, fx=5e-5, walls_y=1, walls_z=1),
+]
+
+
+@pytest.mark.parametrize("prec", ["fp16", "fp64"])
+def test_batch_rows_equal_members_alone(lbm, odd_rows, prec):
+    """A batch whose members' rows are whole wavefronts runs its odd steps in
+    row blocks on request (the same launch for all members; by default its
+    odd steps run in tiles): every member equals the same lattice run alone
+    bit for bit."""
+    with lbm.Batch(BATCH_MEMBERS, precision=prec) as B:
+        assert B.info()["odd_row_blocks"] == 0
+    odd_rows(1)
+    with lbm.Batch(BATCH_MEMBERS, precision=prec) as B:
+        info = B.info()
+        assert info["odd_row_blocks"] > 0 and info["odd_row_block_waves"] == 8
+        B.run(127, 10, lbm.STEP_ALL)
+        got = [_state(B.member(i)) for i in range(len(BATCH_MEMBERS))]
+    for kv, a in zip(BATCH_MEMBERS, got):
+        with lbm.Lattice(**kv, precision=prec) as L:
+            L.run(127, 10, lbm.STEP_ALL)
+            b = _state(L)
+        assert np.array_equal(a["flag"], b["flag"])
+        assert np.array_equal(a["f"], b["f"]), (kv["nx"], float(np.abs(a["f"] - b["f"]).max()))
+        if "sq" in a:
+            for k in ("X", "e", "U", "W"):
+                assert np.array_equal(a["sq"][k], b["sq"][k]), (kv["nx"], k)
