@@ -65,7 +65,7 @@ void batch_odd_row_kernel(const BatchMember<S>* __restrict__ M, const uint32_t* 
     const FastCoef K = from_constant(&mb->K);
     A.t = t;
     const int lb = (int)(e & BATCH_LOCAL_MASK);
-    odd_rows_block<S, MODE>(A, K, lb * A.rows_per_block, A.rows_per_block, &A.mass_part[lb]);
+    odd_rows_block<S, MODE, true>(A, K, lb * A.rows_per_block, A.rows_per_block, &A.mass_part[lb]);
 }
 
 template <class S, int MODE>

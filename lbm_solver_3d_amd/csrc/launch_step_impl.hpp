@@ -36,8 +36,11 @@ template <class S, bool ODD, int MODE>
 static hipError_t launch_step_v(const StepArgs<S>& a, const FastCoef& k, unsigned lds, hipStream_t s) {
     if (ODD && a.odd_rows) {  // row blocks; the host checked the rows (PX a multiple of 64, at most 1024)
         const int rows = a.NY * a.NZ, rpb = a.rows_per_block;
-        hipLaunchKernelGGL((aa_odd_row_kernel<S, MODE>), dim3((rows + rpb - 1) / rpb), dim3(64 * a.row_waves * rpb), lds,
-                           s, a, k);
+        const dim3 grid((rows + rpb - 1) / rpb), block(64 * a.row_waves * rpb);
+        if (rpb == 1 && (uint32_t)a.NX == a.PX)  // one whole row per block
+            hipLaunchKernelGGL((aa_odd_row_kernel<S, MODE, false>), grid, block, lds, s, a, k);
+        else
+            hipLaunchKernelGGL((aa_odd_row_kernel<S, MODE, true>), grid, block, lds, s, a, k);
         return hipGetLastError();
     }
     const int G = a.gx * a.gy * a.gz;

@@ -408,9 +408,9 @@ int occupancy_lds(int device, int blocks, unsigned* bytes) {
 // Measured on the 512 x 256 x 128 channel: the FP32 and FP64 updates move
 // their bytes fastest with 4 and 3 blocks per CU on both parities, FP16 and
 // BF16 with the even step uncapped and the odd step at 6 and 7 -- with the
-// odd step in tiles.  The one-block-per-row odd step is fastest uncapped
+// odd step in tiles.  The odd step in row blocks is fastest uncapped
 // (FP16 225.5 us per step against 257.2 us at the 24 wavefronts per CU of
-// 6 tile blocks; FP32 alike within 0.5 %).
+// 6 tile blocks).
 void default_occupancy(int prec, bool odd_rows, int cap[2]) {
     switch (prec) {
         case LBM_HIP_FP64: cap[0] = cap[1] = 3; break;
@@ -556,13 +556,13 @@ int lbm_hip_create(const lbm_hip_params* pin, lbm_hip_ctx** out) {
     c->ssize = storage_bytes(c->prec);
     c->PX = row_pitch(pin->nx, pin->ny, pin->nz);
     // Odd steps in row blocks where a stored row is whole wavefronts (padded
-    // or not) and fits a block, for 2-byte storage (measured on 512 x 256 x
-    // 128: FP16 odd step 198 against 210 us in tiles; FP32 and FP64 no
-    // faster).  Rows of fewer than 4 wavefronts share a block.
-    // LBM_HIP_ODD_ROWS: 0 off, 1 blocks in order, 2 block ranges per XCD,
-    // for any storage.
+    // or not) and fits a block, for FP16 storage (measured on 512 x 256 x
+    // 128: FP16 odd step 198-208 against 210-215 us in tiles; BF16 2 %
+    // slower in rows, FP32 and FP64 within 1 %).  Rows of fewer than 4
+    // wavefronts share a block.  LBM_HIP_ODD_ROWS: 0 off, 1 blocks in order,
+    // 2 block ranges per XCD, for any storage.
     if (c->PX % 64 == 0 && c->PX <= 64 * ROW_MAX_WAVES) {
-        c->odd_rows = c->ssize == 2 ? 1 : 0;
+        c->odd_rows = c->prec == LBM_HIP_FP16 ? 1 : 0;
         if (const char* e = std::getenv("LBM_HIP_ODD_ROWS")) c->odd_rows = std::max(0, std::min(2, std::atoi(e)));
         c->row_waves = (int)(c->PX / 64);
         c->rows_per_block = c->row_waves >= 4 ? 1 : 4 / c->row_waves;

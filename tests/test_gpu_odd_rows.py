@@ -2,7 +2,7 @@
 
 On lattices whose stored rows are whole wavefronts (the row pitch a multiple
 of 64, at most 1024: NX a multiple of 64, or rows padded to one) the odd step
-of 2-byte storage runs whole rows per block: the ten stores with c_x != 0 are
+of FP16 storage runs whole rows per block (other storages on request): the ten stores with c_x != 0 are
 moved one lane by a wavefront shift and the value that crosses a wavefront
 edge goes through LDS, so every store is line-aligned.  Only the lane that
 issues a store changes, never its address or value:
@@ -14,7 +14,9 @@ issues a store changes, never its address or value:
   rows per block, and equals the tile kernel bit for bit through mass
   corrections;
 - FP16 and BF16 are bit-identical with the row kernel and with the odd step
-  in tiles, through eager steps, mass corrections, remaps and graph replay.
+  in tiles, through eager steps, mass corrections, remaps and graph replay;
+- a batch runs its odd steps in row blocks on request, every member then
+  equal to the same lattice alone.
 The environment variable LBM_HIP_ODD_ROWS is read when a lattice is created
 (0: odd steps in tiles; 1: one block per row).
 """
@@ -142,7 +144,7 @@ def test_row_kernel_selection(lbm, odd_rows, monkeypatch):
     monkeypatch.delenv("LBM_HIP_ODD_ROWS", raising=False)
     for kv, prec, waves in [
         (dict(nx=512, ny=8, nz=8), "fp16", 8),
-        (dict(nx=512, ny=8, nz=8), "bf16", 8),
+        (dict(nx=512, ny=8, nz=8), "bf16", 0),    # 2 % slower in rows: tiles
         (dict(nx=512, ny=8, nz=8), "fp32", 0),    # no faster in rows: tiles
         (dict(nx=512, ny=8, nz=8), "fp64", 0),
         (dict(nx=100, ny=8, nz=8), "fp16", 2),    # rows padded to 128 nodes
