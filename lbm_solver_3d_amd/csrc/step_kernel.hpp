@@ -671,13 +671,14 @@ void aa_step_kernel(const StepArgs<S> A, const FastCoef K) {
 }
 
 // ---------------------------------------------------------------------------
-// Odd step over whole rows (NX a multiple of 64, at most 1024, rows unpadded).
+// Odd step in row blocks (stored rows of whole wavefronts: the row pitch PX a
+// multiple of 64, at most 1024).
 //
 // In the odd step node n stores f*_i into slot i at n + c_i.  For the ten
 // populations with c_x != 0 a wavefront's 64 stores then straddle two lines
-// of the slot, each line shared with a neighbouring wavefront.  Here one
-// block updates one whole row (NX / 64 wavefronts) and those ten stores are
-// made line-aligned: each value moves one lane up (c_x = +1) or down
+// of the slot, each line shared with a neighbouring wavefront.  Here a block
+// updates whole rows (PX / 64 wavefronts each) and those ten stores are made
+// line-aligned: each value moves one lane up (c_x = +1) or down
 // (c_x = -1) by a wavefront shift (DPP), the value that crosses into the
 // neighbouring wavefront's line comes from that wavefront through LDS (the
 // row is periodic in x, so the last wavefront's neighbour is the first), and
