@@ -104,9 +104,10 @@ typedef struct lbm_hip_info {
     int xcd_remap;          /* tile order over the XCDs (lbm_hip_set_tiling) */
     int occupancy_even;     /* cap on collide-stream blocks per CU, even / odd steps (lbm_hip_set_occupancy) */
     int occupancy_odd;
-    int resident_even;      /* blocks per CU the launches allow (HIP's occupancy calculator) */
+    int resident_even;      /* blocks per CU the tile launches allow (HIP's occupancy calculator) */
     int resident_odd;
-    int odd_row_waves;      /* odd steps: wavefronts per block of the one-block-per-row kernel (0: tiles) */
+    int odd_row_waves;      /* odd steps in row blocks: wavefronts per row (0: odd steps in tiles; the
+                               occupancy_odd cap then applies to the row blocks' wavefronts) */
 } lbm_hip_info;
 
 typedef struct lbm_hip_ctx lbm_hip_ctx;
