@@ -140,6 +140,29 @@ def test_rows_equal_tiles_bit_for_bit(lbm, odd_rows, prec, nx):
         assert np.array_equal(a["sq"][k], b["sq"][k]), k
 
 
+@pytest.mark.parametrize("kv", [
+    # a duct with a body force along x and y: the forcing variants of the FP16 row kernel
+    dict(nx=256, ny=20, nz=18, tau=0.8, fx=4e-5, fy=-1e-5, walls_y=1, walls_z=1),
+    # walls in y, squirmer in the x-y plane near the lower wall, rows padded to 192
+    dict(nx=150, ny=40, nz=24, walls_y=1, walls_z=0, squirmer=1, radius=5.0, b1=2e-3, beta=2.0, angle=-30.0,
+         planar=1, y0=8.0, hmin=0.5),
+])
+def test_rows_equal_tiles_forced_and_walls_y(lbm, odd_rows, kv):
+    states = []
+    for mode in (0, 1):
+        odd_rows(mode)
+        with lbm.Lattice(**kv, precision="fp16") as L:
+            assert L.info().odd_row_waves == ((kv["nx"] + 63) // 64 if mode else 0)
+            L.run(101, 10, lbm.STEP_ALL)
+            states.append(_state(L))
+    a, b = states
+    assert np.array_equal(a["flag"], b["flag"])
+    assert np.array_equal(a["f"], b["f"]), float(np.abs(a["f"] - b["f"]).max())
+    if "sq" in a:
+        for k in ("X", "e", "U", "W"):
+            assert np.array_equal(a["sq"][k], b["sq"][k]), k
+
+
 def test_row_kernel_selection(lbm, odd_rows, monkeypatch):
     monkeypatch.delenv("LBM_HIP_ODD_ROWS", raising=False)
     for kv, prec, waves in [
