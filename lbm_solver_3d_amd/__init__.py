@@ -14,6 +14,7 @@ or without a GPU raises.
 from __future__ import annotations
 
 import ctypes as C
+import json
 from pathlib import Path
 
 import numpy as np
@@ -247,6 +248,8 @@ class Lattice:
     def __init__(self, _handle=None, **kv):
         self.lib = load_library()
         self.owned = _handle is None
+        self.kv = dict(kv)  # creation parameters (a checkpoint records them)
+        self.steps_before = 0  # steps of the run a checkpoint restarted from
         if self.owned:
             self.params = make_params(**kv)
             h = C.c_void_p()
@@ -364,6 +367,56 @@ class Lattice:
         if f.size != Q * self.N:
             raise ValueError("populations must have 19*N entries")
         _check(self.lib, self.lib.lbm_hip_upload_populations(self.h, _dp(f)))
+
+    _SQ_ARRAYS = ("X", "e", "U", "W", "Fh", "Th", "Fc", "Tc", "Fw")
+
+    def save_checkpoint(self, path) -> None:
+        """Dump of the state for a restart: the populations in the reference
+        layout (a pending mass-correction shift included), the flags, the
+        squirmer state, the steps taken and the creation parameters, as an
+        .npz of plain arrays (nothing pickled).  Lattice.from_checkpoint
+        restarts from it.  The reference has no checkpoint: its VTK holds no
+        populations (io.cpp:50-65)."""
+        if not self.owned:
+            raise ValueError("a batch member cannot be checkpointed on its own")
+        arrays = dict(f=self.populations(), flag=self.flags(), steps=np.int64(self.steps_before + self.info().steps_done),
+                      params=np.array(json.dumps(self.kv)))
+        if self.has_squirmer:
+            sq = self.squirmer()
+            for k in self._SQ_ARRAYS:
+                arrays["sq_" + k] = sq[k]
+            arrays["sq_contacts"] = np.int64(sq["contacts"])
+        np.savez(path, **arrays)
+
+    @classmethod
+    def from_checkpoint(cls, path, **overrides) -> "Lattice":
+        """A lattice restarted from save_checkpoint's dump.  It is created from
+        the saved parameters (`overrides` may change e.g. the device) with the
+        squirmer at its saved centre, so that its initial flags are the
+        classification the saved flags hold (checked); then the squirmer state
+        and the populations are uploaded.  `steps_before` holds the saved run's
+        steps.  The link list is rebuilt from the flags, so the first force
+        sums after a restart add the same links in another order: an FP64 run
+        continues within round-off of the uninterrupted one."""
+        with np.load(path, allow_pickle=False) as d:
+            data = {k: d[k] for k in d.files}
+        kv = json.loads(str(data["params"]))
+        kv.update(overrides)
+        if "sq_X" in data:
+            X = data["sq_X"]
+            kv.update(x0=float(X[0]), y0=float(X[1]), z0=float(X[2]))
+        L = cls(**kv)
+        try:
+            if L.N != data["flag"].size or not np.array_equal(L.flags(), data["flag"]):
+                raise LbmHipError(-4, "checkpoint flags differ from the classification at the saved centre")
+            if "sq_X" in data:
+                L.set_squirmer(**{k: data["sq_" + k] for k in cls._SQ_ARRAYS}, contacts=int(data["sq_contacts"]))
+            L.upload_populations(data["f"])
+        except BaseException:
+            L.close()
+            raise
+        L.steps_before = int(data["steps"])
+        return L
 
     def time_run(self, steps: int, mass_every: int = 10) -> tuple[float, float]:
         tot, k = C.c_double(), C.c_double()
