@@ -89,8 +89,8 @@ struct lbm_hip_ctx {
     // (0: none) and the dynamic LDS per block that enforces it
     int occ_cap[2] = {0, 0};
     unsigned occ_lds[2] = {0, 0};
-    // odd steps one block per row (aa_odd_row_kernel; 0: tiles, 1: rows in
-    // order, 2: row ranges per XCD) and the dynamic LDS that holds those
+    // odd steps in row blocks (aa_odd_row_kernel; 0: tiles, 1: blocks in
+    // order, 2: block ranges per XCD) and the dynamic LDS that holds those
     // blocks to the odd cap's wavefronts per CU
     int odd_rows = 0;
     int row_waves = 0, rows_per_block = 1, row_blocks = 0;  // PX / 64, rows per block, blocks per odd step

@@ -18,7 +18,7 @@ issues a store changes, never its address or value:
 - a batch runs its odd steps in row blocks on request, every member then
   equal to the same lattice alone.
 The environment variable LBM_HIP_ODD_ROWS is read when a lattice is created
-(0: odd steps in tiles; 1: one block per row).
+(0: odd steps in tiles; 1: in row blocks).
 """
 import numpy as np
 import pytest
